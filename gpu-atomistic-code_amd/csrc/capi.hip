@@ -1,0 +1,688 @@
+// capi.hip - the C ABI declared in include/llg_capi.h.
+//
+// llg_step reproduces llg::integrate(t) (src/llg.cpp:639-664) for t = t0 .. t0+nsteps-1:
+//   Heun: [dipolar refresh if t % refresh == 0] -> noise -> predictor -> corrector
+//         (cullg::llgGPU, src/cuda.cu:43-258)
+//   RK4 / RK5: [dipolar refresh] -> noise -> stages -> combine
+//         (cullg::llgGPURK4 / llgGPURK5, src/cudaRK4.cu:38-233, src/cudaRK5.cu:38-273)
+// The FP64 plain path launches the reference's kernel sequence (SpMV, stage, SpMV,
+// stage); the FP32 path launches one fused kernel per field evaluation.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace sllg {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+const char* last_error() { return g_err.c_str(); }
+
+// Events around every launch of a step, for llg_profile_kernels.
+struct KernelTimer {
+    struct Rec {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    ~KernelTimer() {
+        for (auto& r : recs) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+    }
+};
+
+namespace {
+
+struct Scope {
+    llg_ctx* c;
+    const char* name;
+    hipEvent_t a{}, b{};
+    Scope(llg_ctx* c_, const char* n) : c(c_), name(n) {
+        if (c->timer) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Scope() {
+        if (c->timer) {
+            (void)hipEventRecord(b, c->stream);
+            c->timer->recs.push_back({name, a, b});
+        }
+    }
+};
+
+#define SLLG_TIMED(name, expr)      \
+    do {                            \
+        Scope sc_(c, name);         \
+        SLLG_TRY(expr);             \
+    } while (0)
+
+size_t prec_bytes(const llg_ctx* c) { return c->prec == LLG_FP64_PLAIN ? sizeof(double) : sizeof(float); }
+
+// Upload a host SoA double[3n] into a state buffer of the path's precision.
+int upload_state(llg_ctx* c, DevBuf& dst, const double* x, const double* y, const double* z) {
+    const uint32_t n = c->n;
+    if (c->prec == LLG_FP64_PLAIN) {
+        SLLG_HIP(hipMemcpyAsync(dst.as<double>(), x, n * 8, hipMemcpyHostToDevice, c->stream));
+        SLLG_HIP(hipMemcpyAsync(dst.as<double>() + n, y, n * 8, hipMemcpyHostToDevice, c->stream));
+        SLLG_HIP(hipMemcpyAsync(dst.as<double>() + 2 * (size_t)n, z, n * 8, hipMemcpyHostToDevice, c->stream));
+    } else {
+        std::vector<float> t(3 * (size_t)n);
+        for (uint32_t i = 0; i < n; ++i) {
+            t[i] = (float)x[i];
+            t[n + i] = (float)y[i];
+            t[2 * (size_t)n + i] = (float)z[i];
+        }
+        SLLG_HIP(hipMemcpyAsync(dst.p, t.data(), t.size() * 4, hipMemcpyHostToDevice, c->stream));
+        SLLG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int download_state(llg_ctx* c, const void* src_dev, bool src_f64, double* x, double* y, double* z) {
+    const uint32_t n = c->n;
+    if (src_f64) {
+        const double* s = (const double*)src_dev;
+        SLLG_HIP(hipMemcpyAsync(x, s, n * 8, hipMemcpyDeviceToHost, c->stream));
+        SLLG_HIP(hipMemcpyAsync(y, s + n, n * 8, hipMemcpyDeviceToHost, c->stream));
+        SLLG_HIP(hipMemcpyAsync(z, s + 2 * (size_t)n, n * 8, hipMemcpyDeviceToHost, c->stream));
+        SLLG_HIP(hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> t(3 * (size_t)n);
+        SLLG_HIP(hipMemcpyAsync(t.data(), src_dev, t.size() * 4, hipMemcpyDeviceToHost, c->stream));
+        SLLG_HIP(hipStreamSynchronize(c->stream));
+        for (uint32_t i = 0; i < n; ++i) {
+            x[i] = t[i];
+            y[i] = t[n + i];
+            z[i] = t[2 * (size_t)n + i];
+        }
+    }
+    return 0;
+}
+
+template <class R>
+void make_anis(const llg_anisotropy& an, double basemm, AnisG<R>& g) {
+    // prefactor 2K/(mu_0 mu_B) as in src/cuint.cu:98-137
+    g.k2perp_pf = (R)(2.0 * an.k2perp / basemm);
+    g.k2par_pf = (R)(2.0 * an.k2par / basemm);
+    g.k4perp_pf = (R)(2.0 * an.k4perp / basemm);
+    g.k4par_pf = (R)(2.0 * an.k4par / basemm);
+    for (int a = 0; a < 3; ++a) {
+        g.k2perp_dir[a] = (R)an.k2perp_dir[a];
+        g.k2par_dir[a] = (R)an.k2par_dir[a];
+    }
+    for (int a = 0; a < 9; ++a) {
+        g.k4perp_dirs[a] = (R)an.k4perp_dirs[a];
+        g.k4par_dirs[a] = (R)an.k4par_dirs[a];
+    }
+    g.has_k2perp = an.k2perp != 0.0;
+    g.has_k2par = an.k2par != 0.0;
+    g.has_k4perp = an.k4perp != 0.0;
+    g.has_k4par = an.k4par != 0.0;
+}
+
+int take_noise(llg_ctx* c, const double** dev_xi) {
+    *dev_xi = nullptr;
+    if (c->noise_mode != LLG_NOISE_INJECTED) return 0;
+    if (c->noise_q.empty()) SLLG_FAIL(LLG_E_STATE, "injected noise exhausted: push more with llg_push_noise");
+    std::vector<double> v = std::move(c->noise_q.front());
+    c->noise_q.pop_front();
+    SLLG_HIP(hipMemcpyAsync(c->XI.p, v.data(), v.size() * 8, hipMemcpyHostToDevice, c->stream));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    *dev_xi = c->XI.as<double>();
+    return 0;
+}
+
+int step_plain(llg_ctx* c, uint64_t t) {
+    double* S = c->S.as<double>();
+    double* Y = c->Y.as<double>();
+    double* H = c->H.as<double>();
+    if (c->dip && t % c->dip_refresh == 0) SLLG_TIMED("dipolar_refresh", dipolar_refresh(c, S, c->DEM.p));
+    const double* xi = nullptr;
+    SLLG_TRY(take_noise(c, &xi));
+    if (c->noise_mode == LLG_NOISE_COUNTER && c->T > 0.0) SLLG_TIMED("plain_noise", plain_noise(c, t));
+    if (c->scheme == LLG_HEUN) {
+        SLLG_TIMED("plain_spmv", plain_spmv(c, S, H));
+        SLLG_TIMED("plain_heun1", plain_heun1(c));
+        if (c->dip && c->dip_every_stage) SLLG_TIMED("dipolar_refresh", dipolar_refresh(c, Y, c->DEM.p));
+        SLLG_TIMED("plain_spmv", plain_spmv(c, Y, H));
+        SLLG_TIMED("plain_heun2", plain_heun2(c));
+        c->det_valid = true;
+    } else {
+        const int ns = c->scheme == LLG_RK4 ? 4 : 6;
+        for (int m = 0; m < ns; ++m) {
+            if (m > 0 && c->dip && c->dip_every_stage) SLLG_TIMED("dipolar_refresh", dipolar_refresh(c, Y, c->DEM.p));
+            SLLG_TIMED("plain_spmv", plain_spmv(c, m == 0 ? S : Y, H));
+            SLLG_TIMED("plain_rk_stage", plain_rk_stage(c, m));
+        }
+        SLLG_TIMED("plain_rk_combine", plain_rk_comb(c));
+        c->det_valid = false;
+    }
+    return 0;
+}
+
+int step_fused(llg_ctx* c, uint64_t t) {
+    if (c->dip && t % c->dip_refresh == 0) SLLG_TIMED("dipolar_refresh", dipolar_refresh(c, c->S.p, c->DEM.p));
+    StepSched s{t, nullptr};
+    SLLG_TRY(take_noise(c, &s.xi));
+    if (c->scheme == LLG_HEUN) {
+        SLLG_TIMED("heun_predictor", fused_heun_pred(c, s));
+        if (c->dip && c->dip_every_stage) SLLG_TIMED("dipolar_refresh", dipolar_refresh(c, c->Y.p, c->DEM.p));
+        SLLG_TIMED("heun_corrector", fused_heun_corr(c, s));
+        c->det_valid = true;
+    } else {
+        const int ns = c->scheme == LLG_RK4 ? 4 : 6;
+        void* P[2] = {c->Y.p, c->A.p};
+        for (int m = 0; m < ns; ++m) {
+            if (m > 0 && c->dip && c->dip_every_stage)
+                SLLG_TIMED("dipolar_refresh", dipolar_refresh(c, P[m & 1], c->DEM.p));
+            SLLG_TIMED(c->scheme == LLG_RK4 ? "rk4_stage" : "rk5_stage", fused_rk_stage(c, s, m));
+        }
+        c->det_valid = false;
+    }
+    return 0;
+}
+
+int do_steps(llg_ctx* c, uint64_t t0, uint32_t nsteps) {
+    for (uint32_t k = 0; k < nsteps; ++k) {
+        if (c->prec == LLG_FP64_PLAIN)
+            SLLG_TRY(step_plain(c, t0 + k));
+        else
+            SLLG_TRY(step_fused(c, t0 + k));
+    }
+    return 0;
+}
+
+}  // namespace
+}  // namespace sllg
+
+using namespace sllg;
+
+extern "C" {
+
+uint32_t llg_abi_version(void) { return LLG_ABI_VERSION; }
+const char* llg_last_error(void) { return sllg::last_error(); }
+
+int llg_device_count(int* count) {
+    if (!count) SLLG_FAIL(LLG_E_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return 0;
+}
+
+void llg_desc_init(llg_desc* d) {
+    if (!d) return;
+    std::memset(d, 0, sizeof(*d));
+    d->abi_version = LLG_ABI_VERSION;
+    d->update = 1;
+    d->scheme = LLG_HEUN;
+    d->precision = LLG_FP32;
+    d->noise = LLG_NOISE_COUNTER;
+    d->n_sublattice = 1;
+    d->exch.ncomp = 1;
+}
+
+int llg_create(const llg_desc* d, llg_ctx** out) {
+    if (!d || !out) SLLG_FAIL(LLG_E_INVALID, "descriptor or output pointer is NULL");
+    *out = nullptr;
+    if (d->abi_version != LLG_ABI_VERSION) SLLG_FAIL(LLG_E_INVALID, "abi_version %u != %u", d->abi_version, LLG_ABI_VERSION);
+    if (d->n == 0) SLLG_FAIL(LLG_E_INVALID, "n must be positive");
+    if (d->n >= 0x7FFFFFFFu) SLLG_FAIL(LLG_E_UNSUPPORTED, "n above 2^31-1");
+    if (d->n_species == 0 || d->n_species > LLG_MAX_SPECIES || !d->species)
+        SLLG_FAIL(LLG_E_INVALID, "n_species must be 1..%u with a species table", LLG_MAX_SPECIES);
+    const uint32_t nsub = d->n_sublattice ? d->n_sublattice : 1;
+    if (nsub > kMaxSub) SLLG_FAIL(LLG_E_UNSUPPORTED, "more than %d sublattices", kMaxSub);
+    if (!(d->dt > 0) || !std::isfinite(d->dt)) SLLG_FAIL(LLG_E_INVALID, "dt must be positive");
+    if (d->update == 0) SLLG_FAIL(LLG_E_INVALID, "update must be >= 1");
+    if (d->scheme < LLG_HEUN || d->scheme > LLG_RK5) SLLG_FAIL(LLG_E_INVALID, "unknown scheme %d", d->scheme);
+    if (d->precision != LLG_FP64_PLAIN && d->precision != LLG_FP32) SLLG_FAIL(LLG_E_INVALID, "unknown precision %d", d->precision);
+    if (d->noise != LLG_NOISE_COUNTER && d->noise != LLG_NOISE_INJECTED) SLLG_FAIL(LLG_E_INVALID, "unknown noise mode %d", d->noise);
+    for (uint32_t s = 0; s < d->n_species; ++s) {
+        const llg_species& p = d->species[s];
+        if (!(p.mu > 0) || !(p.lambda >= 0) || !(p.gamma_rel > 0))
+            SLLG_FAIL(LLG_E_INVALID, "species %u: mu and gamma_rel must be positive, lambda >= 0", s);
+        if (p.sublattice >= nsub) SLLG_FAIL(LLG_E_INVALID, "species %u: sublattice %u >= n_sublattice", s, p.sublattice);
+        const double nn = p.k1u_dir[0] * p.k1u_dir[0] + p.k1u_dir[1] * p.k1u_dir[1] + p.k1u_dir[2] * p.k1u_dir[2];
+        if (p.k1u != 0.0 && std::fabs(nn - 1.0) > 1e-6) SLLG_FAIL(LLG_E_INVALID, "species %u: k1u axis is not a unit vector", s);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) SLLG_FAIL(LLG_E_NODEVICE, "no HIP device visible");
+    if (d->device < 0 || d->device >= ndev) SLLG_FAIL(LLG_E_INVALID, "device %d not in [0, %d)", d->device, ndev);
+    SLLG_HIP(hipSetDevice(d->device));
+
+    llg_ctx* c = new llg_ctx();
+    struct Guard {
+        llg_ctx*& c;
+        ~Guard() { if (c) llg_destroy(c); }
+    } guard{c};
+    c->device = d->device;
+    SLLG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->n = d->n;
+    c->ns = d->n_species;
+    c->nsub = nsub;
+    c->scheme = d->scheme;
+    c->prec = d->precision;
+    c->noise_mode = d->noise;
+    c->seed = d->seed;
+    c->stream_id = d->stream;
+    c->update = d->update;
+    c->flags = d->flags;
+    c->dt = d->dt;
+    c->rdt = d->dt * LLG_GAMMA_E;  // src/llg.cpp:306
+    c->anis_in = d->anis;
+    c->basemm = d->species[0].mu * LLG_MU_B;  // src/cuint.cu:56
+    const uint32_t n = c->n;
+
+    // per-species constants: src/llg.cpp:591-592, src/geom.cpp:244
+    c->sp64.resize(c->ns);
+    c->mu_sp.resize(c->ns);
+    c->sub_sp.resize(c->ns);
+    std::vector<Species<float>> sp32(c->ns);
+    for (uint32_t s = 0; s < c->ns; ++s) {
+        const llg_species& p = d->species[s];
+        Species<double>& q = c->sp64[s];
+        q.sigma = std::sqrt(2.0 * LLG_K_B * p.lambda / (p.mu * LLG_MU_B * d->dt * p.gamma_rel * LLG_GAMMA_E));
+        q.llgpf = -1. / (1.0 + p.lambda * p.lambda);
+        q.lambda = p.lambda;
+        q.k1u = 2.0 * p.k1u / (LLG_MU_B * p.mu);
+        q.kc = 2.0 * d->anis.kc / (LLG_MU_B * p.mu);
+        for (int a = 0; a < 3; ++a) q.k1dir[a] = p.k1u_dir[a];
+        c->mu_sp[s] = p.mu;
+        c->sub_sp[s] = p.sublattice;
+        Species<float>& f = sp32[s];
+        f.sigma = (float)q.sigma; f.llgpf = (float)q.llgpf; f.lambda = (float)q.lambda;
+        f.k1u = (float)q.k1u; f.kc = (float)q.kc;
+        for (int a = 0; a < 3; ++a) f.k1dir[a] = (float)q.k1dir[a];
+    }
+    make_anis<double>(d->anis, c->basemm, c->an64);
+    make_anis<float>(d->anis, c->basemm, c->an32);
+    std::vector<uint32_t> spec(n);
+    std::vector<uint8_t> spec8(n), sub8(n);
+    c->sub_count.assign(nsub, 0);
+    bool multi = false;
+    for (uint32_t i = 0; i < n; ++i) {
+        spec[i] = d->spin_species ? d->spin_species[i] : (i % c->ns);
+        if (spec[i] >= c->ns) SLLG_FAIL(LLG_E_INVALID, "spin %u: species %u >= n_species", i, spec[i]);
+        if (spec[i] != 0) multi = true;
+        spec8[i] = (uint8_t)spec[i];
+        sub8[i] = (uint8_t)c->sub_sp[spec[i]];
+        c->sub_count[sub8[i]]++;
+    }
+    c->multi_species = multi;
+    SLLG_TRY(c->sp32.alloc(sp32.size() * sizeof(Species<float>)));
+    SLLG_HIP(hipMemcpy(c->sp32.p, sp32.data(), sp32.size() * sizeof(Species<float>), hipMemcpyHostToDevice));
+    SLLG_TRY(c->sp64d.alloc(c->sp64.size() * sizeof(Species<double>)));
+    SLLG_HIP(hipMemcpy(c->sp64d.p, c->sp64.data(), c->sp64.size() * sizeof(Species<double>), hipMemcpyHostToDevice));
+    SLLG_TRY(c->spec.alloc(n));
+    SLLG_HIP(hipMemcpy(c->spec.p, spec8.data(), n, hipMemcpyHostToDevice));
+    SLLG_TRY(c->sub.alloc(n));
+    SLLG_HIP(hipMemcpy(c->sub.p, sub8.data(), n, hipMemcpyHostToDevice));
+
+    const size_t pb = prec_bytes(c);
+    const size_t v3 = 3 * (size_t)n * pb;
+    SLLG_TRY(c->S.alloc(v3));
+    SLLG_TRY(c->Y.alloc(v3));
+    SLLG_TRY(c->STG.alloc(v3));
+    SLLG_TRY(c->XI.alloc(3 * (size_t)n * sizeof(double)));
+    if (c->prec == LLG_FP64_PLAIN) {
+        SLLG_TRY(c->H.alloc(v3));
+        SLLG_TRY(c->FN.alloc(v3));
+        SLLG_TRY(c->DET.alloc(v3));
+        // per-spin parameter arrays, as the reference uploads them (src/cuda_util.cu:282-556)
+        std::vector<double> sg(n), lp(n), lm(n), k1(n), kd(3 * (size_t)n), kc(n);
+        bool any_kc = false;
+        for (uint32_t i = 0; i < n; ++i) {
+            const Species<double>& q = c->sp64[spec[i]];
+            sg[i] = q.sigma; lp[i] = q.llgpf; lm[i] = q.lambda; k1[i] = q.k1u; kc[i] = q.kc;
+            any_kc = any_kc || q.kc != 0.0;
+            for (int a = 0; a < 3; ++a) kd[(size_t)a * n + i] = q.k1dir[a];
+        }
+        DevBuf* bufs[5] = {&c->p_sigma, &c->p_llgpf, &c->p_lambda, &c->p_k1u, &c->p_kc};
+        std::vector<double>* srcs[5] = {&sg, &lp, &lm, &k1, &kc};
+        for (int k = 0; k < 5; ++k) {
+            if (k == 4 && !any_kc) break;
+            SLLG_TRY(bufs[k]->alloc(n * 8));
+            SLLG_HIP(hipMemcpy(bufs[k]->p, srcs[k]->data(), n * 8, hipMemcpyHostToDevice));
+        }
+        SLLG_TRY(c->p_k1udir.alloc(3 * (size_t)n * 8));
+        SLLG_HIP(hipMemcpy(c->p_k1udir.p, kd.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
+        if (c->scheme != LLG_HEUN) {
+            SLLG_TRY(plain_rk_init_tables());
+            for (int m = 0; m < 6; ++m) SLLG_TRY(c->K[m].alloc(v3));
+        }
+        SLLG_TRY(build_exchange_fp64(c, d->exch));
+    } else {
+        SLLG_TRY(c->A.alloc(v3));
+        if (c->scheme == LLG_RK4) SLLG_TRY(c->K[0].alloc(v3));
+        if (c->scheme == LLG_RK5) {
+            SLLG_TRY(fused_rk_init_tables());
+            for (int m = 0; m < 5; ++m) SLLG_TRY(c->K[m].alloc(v3));
+        }
+        SLLG_TRY(build_exchange_fp32(c, d->exch));
+    }
+    if (d->dip.enabled) {
+        if (!d->dip.mesh) SLLG_FAIL(LLG_E_INVALID, "dipolar enabled without mesh coordinates");
+        c->dip = true;
+        c->dip_every_stage = d->dip.every_stage ? 1 : 0;
+        c->dip_refresh = d->dip.refresh ? d->dip.refresh : d->update;
+        SLLG_TRY(c->DEM.alloc(v3));
+        SLLG_HIP(hipMemset(c->DEM.p, 0, v3));
+        SLLG_TRY(dipolar_create(c, d->dip, d->spin_species));
+    }
+    SLLG_HIP(hipMemset(c->STG.p, 0, v3));
+    // initial state: all spins along +z
+    {
+        std::vector<double> zx(n, 0.0), zz(n, 1.0);
+        SLLG_TRY(upload_state(c, c->S, zx.data(), zx.data(), zz.data()));
+        SLLG_TRY(upload_state(c, c->Y, zx.data(), zx.data(), zz.data()));
+    }
+    SLLG_HIP(hipDeviceSynchronize());
+    *out = c;
+    c = nullptr;  // disarm guard
+    return 0;
+}
+
+void llg_destroy(llg_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dipolar_destroy(c);
+    delete c->timer;
+    c->timer = nullptr;
+    hipStream_t s = c->stream;
+    delete c;
+    if (s) (void)hipStreamDestroy(s);
+}
+
+int llg_set_spins(llg_ctx* c, const double* sx, const double* sy, const double* sz) {
+    if (!c || !sx || !sy || !sz) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    for (uint32_t i = 0; i < c->n; ++i) {
+        const double m = sx[i] * sx[i] + sy[i] * sy[i] + sz[i] * sz[i];
+        if (!(m > 0.0) || !std::isfinite(m)) SLLG_FAIL(LLG_E_INVALID, "spin %u has zero or non-finite length", i);
+    }
+    SLLG_HIP(hipSetDevice(c->device));
+    SLLG_TRY(upload_state(c, c->S, sx, sy, sz));
+    c->det_valid = false;
+    return 0;
+}
+
+int llg_get_spins(llg_ctx* c, double* sx, double* sy, double* sz) {
+    if (!c || !sx || !sy || !sz) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    SLLG_HIP(hipSetDevice(c->device));
+    return download_state(c, c->S.p, c->prec == LLG_FP64_PLAIN, sx, sy, sz);
+}
+
+int llg_set_temperature(llg_ctx* c, double T) {
+    if (!c || !(T >= 0.0) || !std::isfinite(T)) SLLG_FAIL(LLG_E_INVALID, "temperature must be finite and >= 0");
+    c->T = T;
+    return 0;
+}
+
+int llg_set_applied(llg_ctx* c, const double B[3]) {
+    if (!c || !B) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    for (int a = 0; a < 3; ++a) {
+        if (!std::isfinite(B[a])) SLLG_FAIL(LLG_E_INVALID, "applied field not finite");
+        c->B[a] = B[a];
+    }
+    return 0;
+}
+
+int llg_set_staggered(llg_ctx* c, const double* hx, const double* hy, const double* hz) {
+    if (!c) SLLG_FAIL(LLG_E_INVALID, "NULL handle");
+    SLLG_HIP(hipSetDevice(c->device));
+    const uint32_t n = c->n;
+    std::vector<double>& s0 = c->stag0_host;
+    if (!hx && !hy && !hz) {
+        s0.clear();
+        c->stag_set = false;
+        SLLG_HIP(hipMemsetAsync(c->STG.p, 0, c->STG.bytes, c->stream));
+        SLLG_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    if (!hx || !hy || !hz) SLLG_FAIL(LLG_E_INVALID, "staggered field: give all three components or none");
+    s0.assign(3 * (size_t)n, 0.0);
+    std::memcpy(s0.data(), hx, n * 8);
+    std::memcpy(s0.data() + n, hy, n * 8);
+    std::memcpy(s0.data() + 2 * (size_t)n, hz, n * 8);
+    c->stag_set = true;
+    return upload_state(c, c->STG, hx, hy, hz);
+}
+
+// cullg::rampStag / cufields::CincStagFields (src/cuda.cu:38-41, src/cufields.cu:46-56): H = r H0
+int llg_scale_staggered(llg_ctx* c, double r) {
+    if (!c || !std::isfinite(r)) SLLG_FAIL(LLG_E_INVALID, "bad argument");
+    std::vector<double>& s0 = c->stag0_host;
+    if (s0.empty()) SLLG_FAIL(LLG_E_STATE, "no staggered field set");
+    const uint32_t n = c->n;
+    std::vector<double> v(3 * (size_t)n);
+    for (size_t k = 0; k < v.size(); ++k) v[k] = r * s0[k];
+    SLLG_HIP(hipSetDevice(c->device));
+    return upload_state(c, c->STG, v.data(), v.data() + n, v.data() + 2 * (size_t)n);
+}
+
+int llg_push_noise(llg_ctx* c, uint32_t nsteps, const double* xi) {
+    if (!c || (nsteps && !xi)) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    if (c->noise_mode != LLG_NOISE_INJECTED) SLLG_FAIL(LLG_E_STATE, "handle is not in injected-noise mode");
+    const size_t m = 3 * (size_t)c->n;
+    for (uint32_t s = 0; s < nsteps; ++s) c->noise_q.emplace_back(xi + s * m, xi + (s + 1) * m);
+    return 0;
+}
+
+int llg_step(llg_ctx* c, uint64_t t0, uint32_t nsteps) {
+    if (!c) SLLG_FAIL(LLG_E_INVALID, "NULL handle");
+    SLLG_HIP(hipSetDevice(c->device));
+    SLLG_TRY(do_steps(c, t0, nsteps));
+    if (c->flags & LLG_FLAG_CHECK_FINITE) {
+        int bad = 0;
+        SLLG_TRY(check_finite(c, &bad));
+        if (bad) SLLG_FAIL(LLG_E_NONFINITE, "non-finite spin after steps %llu..%llu", (unsigned long long)t0,
+                           (unsigned long long)(t0 + nsteps - 1));
+    }
+    return 0;
+}
+
+int llg_sync(llg_ctx* c) {
+    if (!c) SLLG_FAIL(LLG_E_INVALID, "NULL handle");
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int field_at(llg_ctx* c, const void* Sp, double* hx, double* hy, double* hz) {
+    DevBuf out;
+    SLLG_TRY(out.alloc(3 * (size_t)c->n * 8));
+    if (c->prec == LLG_FP64_PLAIN)
+        SLLG_TRY(plain_detfield(c, (const double*)Sp, out.as<double>()));
+    else
+        SLLG_TRY(fused_detfield(c, (const float*)Sp, out.as<double>()));
+    return download_state(c, out.p, true, hx, hy, hz);
+}
+
+int llg_get_det_field(llg_ctx* c, double* hx, double* hy, double* hz) {
+    if (!c || !hx || !hy || !hz) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    SLLG_HIP(hipSetDevice(c->device));
+    if (c->det_valid && c->prec == LLG_FP64_PLAIN) return download_state(c, c->DET.p, true, hx, hy, hz);
+    if (c->det_valid) return field_at(c, c->Y.p, hx, hy, hz);
+    return field_at(c, c->S.p, hx, hy, hz);
+}
+
+int llg_get_field(llg_ctx* c, double* hx, double* hy, double* hz) {
+    if (!c || !hx || !hy || !hz) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    SLLG_HIP(hipSetDevice(c->device));
+    return field_at(c, c->S.p, hx, hy, hz);
+}
+
+int llg_reduce(llg_ctx* c, double* m, double* energy) {
+    if (!c || !m) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    SLLG_HIP(hipSetDevice(c->device));
+    DevBuf out;
+    SLLG_TRY(out.alloc((c->nsub * 3 + 1) * sizeof(double)));
+    SLLG_TRY(reduce_mag(c, out.as<double>()));
+    if (energy) {
+        DevBuf dem;
+        if (c->dip) {
+            SLLG_TRY(dem.alloc(3 * (size_t)c->n * prec_bytes(c)));
+            SLLG_TRY(dipolar_refresh(c, c->S.p, dem.p));
+        }
+        SLLG_TRY(reduce_energy(c, c->dip ? dem.p : nullptr, out.as<double>() + c->nsub * 3));
+        SLLG_HIP(hipStreamSynchronize(c->stream));
+    }
+    std::vector<double> h(c->nsub * 3 + 1);
+    SLLG_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    std::memcpy(m, h.data(), c->nsub * 3 * 8);
+    if (energy) *energy = h[c->nsub * 3];
+    return 0;
+}
+
+int llg_run_sampled(llg_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t every, double* m_series) {
+    if (!c || !m_series || every == 0) SLLG_FAIL(LLG_E_INVALID, "bad argument");
+    SLLG_HIP(hipSetDevice(c->device));
+    const uint32_t nrec = (nsteps + every - 1) / every;
+    const size_t rec = (size_t)c->nsub * 3;
+    DevBuf ser;
+    SLLG_TRY(ser.alloc(std::max<size_t>(1, nrec) * rec * sizeof(double)));
+    uint32_t r = 0;
+    for (uint32_t s = 0; s < nsteps; s += every) {
+        SLLG_TRY(reduce_mag(c, ser.as<double>() + (size_t)r * rec));
+        ++r;
+        SLLG_TRY(do_steps(c, t0 + s, std::min(every, nsteps - s)));
+    }
+    SLLG_HIP(hipMemcpyAsync(m_series, ser.p, (size_t)nrec * rec * 8, hipMemcpyDeviceToHost, c->stream));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int llg_noise_sample(llg_ctx* c, uint64_t t, double* xi) {
+    if (!c || !xi) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    if (c->noise_mode != LLG_NOISE_COUNTER) SLLG_FAIL(LLG_E_STATE, "handle is not in counter-noise mode");
+    SLLG_HIP(hipSetDevice(c->device));
+    DevBuf out;
+    SLLG_TRY(out.alloc(3 * (size_t)c->n * 8));
+    SLLG_TRY(noise_sample(c, t, out.as<double>()));
+    SLLG_HIP(hipMemcpyAsync(xi, out.p, 3 * (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+void* llg_stream(llg_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int llg_profile_kernels(llg_ctx* c, uint64_t t0, uint32_t nsteps, int max_kinds, char* names, double* mean_ms,
+                        int* n_kinds) {
+    if (!c || !names || !mean_ms || !n_kinds || max_kinds <= 0) SLLG_FAIL(LLG_E_INVALID, "bad argument");
+    SLLG_HIP(hipSetDevice(c->device));
+    delete c->timer;
+    c->timer = new KernelTimer();
+    int rc = do_steps(c, t0, nsteps);
+    (void)hipStreamSynchronize(c->stream);
+    KernelTimer* tm = c->timer;
+    c->timer = nullptr;
+    if (rc) { delete tm; return rc; }
+    std::vector<std::string> order;
+    std::map<std::string, std::pair<double, int>> acc;
+    for (auto& r : tm->recs) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        if (!acc.count(r.name)) order.push_back(r.name);
+        acc[r.name].first += ms;
+        acc[r.name].second += 1;
+    }
+    delete tm;
+    int k = 0;
+    for (auto& nm : order) {
+        if (k >= max_kinds) break;
+        std::snprintf(names + 48 * k, 48, "%s", nm.c_str());
+        mean_ms[k] = acc[nm].first / acc[nm].second;
+        ++k;
+    }
+    *n_kinds = k;
+    return 0;
+}
+
+int llg_get_info(llg_ctx* c, llg_info* info) {
+    if (!c || !info) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    std::memset(info, 0, sizeof(*info));
+    size_t dev = 0;
+    DevBuf* all[] = {&c->S, &c->Y, &c->A, &c->H, &c->FN, &c->DET, &c->XI, &c->STG, &c->STG0, &c->DEM,
+                     &c->K[0], &c->K[1], &c->K[2], &c->K[3], &c->K[4], &c->K[5], &c->spec, &c->sub,
+                     &c->sp32, &c->sp64d, &c->p_sigma, &c->p_llgpf, &c->p_lambda, &c->p_k1u, &c->p_k1udir,
+                     &c->p_kc, &c->ex_a, &c->ex_b, &c->ex_c, &c->ex_d, &c->ex_e, &c->ex_dict64};
+    for (DevBuf* b : all) dev += b->bytes;
+    info->device_bytes = dev;
+    std::snprintf(info->exch_encoding, sizeof info->exch_encoding, "%s", c->exch_encoding.c_str());
+    const double n = (double)c->n;
+    const HostExch& h = c->hex;
+    const int nc = h.ncomp;
+    // coupling bytes per row per field evaluation (M)
+    double M = 0.0;
+    if (c->prec == LLG_FP64_PLAIN) {
+        if (h.fmt == LLG_EXCH_CSR) M = 4.0 + (double)h.nnz / n * (4.0 + 8.0 * nc);
+        else if (h.fmt == LLG_EXCH_DIA) M = (double)h.ndiag * 8.0 * nc;
+        else if (h.fmt == LLG_EXCH_STENCIL && h.per_row) M = (double)h.sk * 8.0 * nc;
+    } else {
+        const Exch<float>& x = c->ex32;
+        if (x.fmt == FMT_SELL_PACK) M = (double)x.nent / n * 4.0;
+        else if (x.fmt == FMT_SELL_RAW) M = (double)x.nent / n * (4.0 + 4.0 * nc);
+        else if (x.fmt == FMT_DIA) M = (double)h.ndiag * 4.0 * nc;
+        else if (x.fmt == FMT_STENCIL && h.per_row) M = (double)h.sk * 4.0 * nc;
+    }
+    const double P = (double)prec_bytes(c);
+    const double extra = (c->dip ? 3 * P : 0) + (c->stag_set ? 3 * P : 0) + (c->multi_species && c->prec != LLG_FP64_PLAIN ? 1 : 0) +
+                         (c->noise_mode == LLG_NOISE_INJECTED && c->T > 0 ? 24 : 0);
+    if (c->prec == LLG_FP64_PLAIN) {
+        const double params = 5 * 8.0 + 3 * 8.0;  // sigma, llgpf, lambda, k1u (+kc) and k1udir
+        const double noise = c->T > 0 ? 3 * 8.0 : 0.0;
+        const double spmv = M + 3 * P + (c->dip ? 3 * P : 0);
+        if (c->scheme == LLG_HEUN) {
+            const double h1 = 3 * P + 3 * P + params + noise + 6 * P + (c->stag_set ? 3 * P : 0);
+            const double h2 = 3 * P * 3 + params + noise + 6 * P + (c->stag_set ? 3 * P : 0);
+            info->bytes_per_spin_update = 2 * spmv + h1 + h2 + (c->T > 0 && c->noise_mode == LLG_NOISE_COUNTER ? noise : 0);
+            info->dominant_kernel_bytes_per_spin = spmv;
+            std::snprintf(info->dominant_kernel, sizeof info->dominant_kernel, "k_plain_spmv");
+        } else {
+            const int ns = c->scheme == LLG_RK4 ? 4 : 6;
+            info->bytes_per_spin_update = ns * (spmv + 6 * P + params + noise + 3 * P) + 6 * P + ns * 3 * P;
+            info->dominant_kernel_bytes_per_spin = spmv;
+            std::snprintf(info->dominant_kernel, sizeof info->dominant_kernel, "k_plain_spmv");
+        }
+    } else {
+        if (c->scheme == LLG_HEUN) {
+            info->dominant_kernel_bytes_per_spin = 9 * P + M + extra;
+            info->bytes_per_spin_update = 2 * info->dominant_kernel_bytes_per_spin;
+            std::snprintf(info->dominant_kernel, sizeof info->dominant_kernel, "k_heun_corr");
+        } else if (c->scheme == LLG_RK4) {
+            info->bytes_per_spin_update = 51 * P + 4 * (M + extra);
+            info->dominant_kernel_bytes_per_spin = 15 * P + M + extra;
+            std::snprintf(info->dominant_kernel, sizeof info->dominant_kernel, "k_rk4");
+        } else {
+            // stage m: read Yin, S (m>0), K_0..K_{m-1} (m>0), write K_m, Yout; last: read S,Yin,K0,K2,K3, write S
+            double b = 0;
+            for (int m = 0; m < 5; ++m) b += 3 * P * (1 + (m > 0) + m + 2);
+            b += 3 * P * 6;
+            info->bytes_per_spin_update = b + 6 * (M + extra);
+            info->dominant_kernel_bytes_per_spin = 3 * P * 7 + M + extra;
+            std::snprintf(info->dominant_kernel, sizeof info->dominant_kernel, "k_rk5");
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

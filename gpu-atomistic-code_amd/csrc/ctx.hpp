@@ -1,0 +1,136 @@
+// ctx.hpp - device-side description of one spin system and the host handle.
+//
+// HBM layout (all SoA, component-major: component c of spin i at [c*n + i]):
+//   FP64 plain path : S, Y (predicted / stage input), H (exchange + demag), FN (f(S_n)),
+//                     DET, XI (noise), per-spin sigma/llgpf/lambda/k1u/k1udir, CSR with
+//                     FP64 values per nnz - the reference GPU data structures, SoA.
+//   FP32 fused path : S, Y, A (= S + rdt/2 f(S)), per-species constants, a compact
+//                     exchange encoding (sliced-ELL with packed 16-bit column delta +
+//                     16-bit coupling-dictionary index, or raw u32/FP32 when it does not
+//                     fit; periodic stencil; DIA), noise regenerated in registers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace sllg {
+
+constexpr int kTPB = 256;       // threads per block of the per-spin kernels
+constexpr int kSlice = 64;      // sliced-ELL slice height = one wavefront
+constexpr int kMaxSub = 16;     // sublattices reduced on the device
+
+enum ExchFmt : int {
+    FMT_NONE = 0,
+    FMT_DIA = 1,
+    FMT_CSR = 2,        // FP64 plain: CSR u32 + FP64 values
+    FMT_STENCIL = 3,    // periodic stencil (shared or per-row values)
+    FMT_SELL_PACK = 4,  // FP32: sliced ELL, u32 = (int16 col delta << 16) | u16 dict index
+    FMT_SELL_RAW = 5    // FP32: sliced ELL, u32 col + FP32 values [ncomp][entries]
+};
+
+// Per-species constants, precision R.
+template <class R> struct Species {
+    R llgpf, lambda, sigma, k1u, kc;
+    R k1dir[3];
+};
+
+// Global anisotropy with prefactors 2K/(mu_0 mu_B) precomputed on the host in FP64.
+template <class R> struct AnisG {
+    R k2perp_pf, k2par_pf, k4perp_pf, k4par_pf;
+    R k2perp_dir[3], k2par_dir[3], k4perp_dirs[9], k4par_dirs[9];
+    int has_k2perp, has_k2par, has_k4perp, has_k4par;
+};
+
+// Exchange operands on the device (one struct for every format; unused fields null).
+template <class V> struct Exch {
+    int fmt = FMT_NONE, ncomp = 1;
+    uint32_t n = 0;
+    // CSR
+    const uint32_t* xadj = nullptr;
+    const uint32_t* adj = nullptr;
+    const V* vals = nullptr;  // [ncomp][nnz] (CSR), [ncomp][ndiag][n] (DIA), stencil values
+    uint64_t nnz = 0;
+    // DIA
+    const int32_t* doff = nullptr;
+    uint32_t ndiag = 0;
+    // stencil
+    uint32_t dims[3] = {0, 0, 0}, na = 1, pbc[3] = {0, 0, 0}, sk = 0, per_row = 0;
+    const int4* sent = nullptr;       // [na][sk] (dx, dy, dz, t')
+    const uint32_t* scount = nullptr; // [na]
+    // sliced ELL
+    const uint32_t* slice_off = nullptr;  // [nslices+1] offsets in entries/64
+    const uint32_t* ent = nullptr;        // packed entries or columns, [..][64]
+    const V* dict = nullptr;              // [ndict][ncomp] (SELL_PACK) or raw values
+    uint32_t ndict = 0;
+    uint64_t nent = 0;                    // total entries (SELL_RAW value stride)
+};
+
+struct DipolarPlan;  // k_dipolar.hip
+
+// Host-side snapshot of the exchange given at create time (FP64, reference layout).
+struct HostExch {
+    int fmt = 0, ncomp = 1;
+    std::vector<uint32_t> xadj, adj;
+    std::vector<double> vals;
+    uint64_t nnz = 0;
+    std::vector<int32_t> doff;
+    uint32_t ndiag = 0;
+    uint32_t dims[3] = {0, 0, 0}, na = 1, pbc[3] = {0, 0, 0}, sk = 0, per_row = 0;
+    std::vector<uint32_t> scount;
+    std::vector<int32_t> sent;
+};
+
+struct KernelTimer;  // capi.hip
+
+}  // namespace sllg
+
+struct llg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t n = 0, ns = 1, nsub = 1;
+    int scheme = LLG_HEUN, prec = LLG_FP32, noise_mode = LLG_NOISE_COUNTER;
+    uint64_t seed = 0, stream_id = 0;
+    uint32_t update = 1, flags = 0;
+    double dt = 0, rdt = 0, T = 0, B[3] = {0, 0, 0};
+    double basemm = 0;
+    bool stag_set = false, det_valid = false, dip = false;
+    int dip_every_stage = 0;
+    uint32_t dip_refresh = 1;
+    std::vector<double> mu_sp;      // per species (mu_B)
+    std::vector<uint32_t> sub_sp;   // per species
+    std::vector<uint64_t> sub_count;
+    llg_anisotropy anis_in{};
+    sllg::AnisG<double> an64{};
+    sllg::AnisG<float> an32{};
+    std::vector<sllg::Species<double>> sp64;
+    bool multi_species = false;
+
+    // device state
+    sllg::DevBuf S, Y, A, H, FN, DET, XI, STG, STG0, DEM;  // SoA 3n (type per path)
+    sllg::DevBuf K[6];                                     // RK stage increments
+    sllg::DevBuf spec;      // u8 species per spin (FP32 path, multi-species)
+    sllg::DevBuf sub;       // u8 sublattice per spin
+    sllg::DevBuf sp32, sp64d;  // Species tables
+    sllg::DevBuf p_sigma, p_llgpf, p_lambda, p_k1u, p_k1udir, p_kc;  // FP64 plain per-spin arrays
+    // exchange storage
+    sllg::DevBuf ex_a, ex_b, ex_c, ex_d, ex_e;
+    sllg::Exch<double> ex64;
+    sllg::Exch<float> ex32;
+    sllg::DevBuf ex_dict64;  // FP64 copy of the dictionary / values for energy on the FP32 path
+    sllg::HostExch hex;
+    std::string exch_encoding;
+    // reductions
+    sllg::DevBuf red_part, red_out, m_series;
+    // dipolar
+    sllg::DipolarPlan* dplan = nullptr;
+    // injected noise queue (host)
+    std::deque<std::vector<double>> noise_q;
+    // staggered field as last given (host, SoA 3n), scaled by llg_scale_staggered
+    std::vector<double> stag0_host;
+    // timing
+    sllg::KernelTimer* timer = nullptr;
+};

@@ -1,0 +1,342 @@
+// encode.hip - host-side validation of the exchange given in the descriptor and its
+// upload in the layout each path reads.
+//
+// FP64 plain path: the reference layouts as given (CSR u32 + FP64 [ncomp][nnz];
+// DIA [ncomp][ndiag][n]; periodic stencil).
+//
+// FP32 path: CSR becomes sliced ELL with 64-row slices (one wavefront) so that every
+// load of the coupling stream is a contiguous 256-byte wave access.  When every column
+// lies within +-32767 of its row (modulo n, i.e. across the periodic wrap) and the
+// distinct coupling tuples number at most 65536, an entry is one u32: int16 column
+// delta << 16 | u16 index into a coupling dictionary (M_CSR = 4 B per nnz, the
+// dictionary is a few KB and stays in cache).  Otherwise entries are a u32 column plus
+// ncomp FP32 values (M = 4 + 4 ncomp B per nnz).  Rows keep their stored order, so the
+// per-row summation order of the reference CSR (src/cufields.cu:192-200) is kept.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace sllg {
+
+namespace {
+
+template <class T>
+int upload(DevBuf& b, const T* h, size_t count, hipStream_t s) {
+    SLLG_TRY(b.alloc(count * sizeof(T)));
+    if (count) SLLG_HIP(hipMemcpyAsync(b.p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    SLLG_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int validate_and_copy(llg_ctx* c, const llg_exchange& e) {
+    HostExch& h = c->hex;
+    const uint32_t n = c->n;
+    h.fmt = e.format;
+    h.ncomp = e.ncomp ? e.ncomp : 1;
+    if (h.fmt == LLG_EXCH_NONE) return 0;
+    if (h.ncomp != 1 && h.ncomp != 3 && h.ncomp != 9) SLLG_FAIL(LLG_E_INVALID, "exchange ncomp must be 1, 3 or 9");
+    const int nc = h.ncomp;
+    if (h.fmt == LLG_EXCH_CSR) {
+        if (!e.xadj) SLLG_FAIL(LLG_E_INVALID, "CSR exchange without xadj");
+        if (e.nnz && (!e.adjncy || !e.csr_vals)) SLLG_FAIL(LLG_E_INVALID, "CSR exchange without adjncy/values");
+        if (e.nnz > 0xFFFFFFFFull) SLLG_FAIL(LLG_E_UNSUPPORTED, "nnz above 2^32-1");
+        if (e.xadj[0] != 0 || e.xadj[n] != e.nnz)
+            SLLG_FAIL(LLG_E_INVALID, "CSR row pointers: xadj[0]=%u xadj[n]=%u nnz=%llu", e.xadj[0], e.xadj[n],
+                      (unsigned long long)e.nnz);
+        for (uint32_t i = 0; i < n; ++i)
+            if (e.xadj[i + 1] < e.xadj[i]) SLLG_FAIL(LLG_E_INVALID, "CSR row pointers decrease at row %u", i);
+        for (uint64_t q = 0; q < e.nnz; ++q)
+            if (e.adjncy[q] >= n) SLLG_FAIL(LLG_E_INVALID, "CSR column %u out of range at nnz %llu", e.adjncy[q],
+                                            (unsigned long long)q);
+        h.nnz = e.nnz;
+        h.xadj.assign(e.xadj, e.xadj + n + 1);
+        h.adj.assign(e.adjncy, e.adjncy + e.nnz);
+        h.vals.assign(e.csr_vals, e.csr_vals + (size_t)nc * e.nnz);
+    } else if (h.fmt == LLG_EXCH_DIA) {
+        if (e.ndiag && (!e.dia_offsets || !e.dia_vals)) SLLG_FAIL(LLG_E_INVALID, "DIA exchange without offsets/values");
+        h.ndiag = e.ndiag;
+        h.doff.assign(e.dia_offsets, e.dia_offsets + e.ndiag);
+        h.vals.assign(e.dia_vals, e.dia_vals + (size_t)nc * e.ndiag * n);
+    } else if (h.fmt == LLG_EXCH_STENCIL) {
+        if ((uint64_t)e.dims[0] * e.dims[1] * e.dims[2] * e.n_atoms != n || e.n_atoms == 0)
+            SLLG_FAIL(LLG_E_INVALID, "stencil dims x n_atoms != n");
+        if (e.stencil_k && (!e.stencil_count || !e.stencil_entries || !e.stencil_vals))
+            SLLG_FAIL(LLG_E_INVALID, "stencil without counts/entries/values");
+        for (int a = 0; a < 3; ++a) { h.dims[a] = e.dims[a]; h.pbc[a] = e.pbc[a] ? 1 : 0; }
+        h.na = e.n_atoms;
+        h.sk = e.stencil_k;
+        h.per_row = e.stencil_vals_per_row ? 1 : 0;
+        h.scount.assign(e.stencil_count, e.stencil_count + h.na);
+        h.sent.assign(e.stencil_entries, e.stencil_entries + (size_t)h.na * h.sk * 4);
+        for (uint32_t t = 0; t < h.na; ++t) {
+            if (h.scount[t] > h.sk) SLLG_FAIL(LLG_E_INVALID, "stencil count above stencil_k");
+            for (uint32_t q = 0; q < h.scount[t]; ++q) {
+                const int32_t* en = &h.sent[((size_t)t * h.sk + q) * 4];
+                if (en[3] < 0 || (uint32_t)en[3] >= h.na) SLLG_FAIL(LLG_E_INVALID, "stencil target atom out of range");
+                for (int a = 0; a < 3; ++a)
+                    if (en[a] <= -(int32_t)h.dims[a] || en[a] >= (int32_t)h.dims[a])
+                        SLLG_FAIL(LLG_E_INVALID, "stencil offset reaches beyond one period");
+            }
+        }
+        const size_t nv = h.per_row ? (size_t)nc * h.sk * n : (size_t)nc * h.na * h.sk;
+        h.vals.assign(e.stencil_vals, e.stencil_vals + nv);
+    } else {
+        SLLG_FAIL(LLG_E_INVALID, "unknown exchange format %d", h.fmt);
+    }
+    return 0;
+}
+
+template <class V>
+void fill_common(Exch<V>& x, const HostExch& h, uint32_t n) {
+    x.fmt = h.fmt;
+    x.ncomp = h.ncomp;
+    x.n = n;
+    x.nnz = h.nnz;
+    x.ndiag = h.ndiag;
+    for (int a = 0; a < 3; ++a) { x.dims[a] = h.dims[a]; x.pbc[a] = h.pbc[a]; }
+    x.na = h.na;
+    x.sk = h.sk;
+    x.per_row = h.per_row;
+}
+
+int upload_stencil_tables(llg_ctx* c) {
+    const HostExch& h = c->hex;
+    std::vector<int4> ent((size_t)h.na * h.sk);
+    for (size_t q = 0; q < ent.size(); ++q)
+        ent[q] = make_int4(h.sent[q * 4], h.sent[q * 4 + 1], h.sent[q * 4 + 2], h.sent[q * 4 + 3]);
+    SLLG_TRY(upload(c->ex_a, ent.data(), ent.size(), c->stream));
+    SLLG_TRY(upload(c->ex_b, h.scount.data(), h.scount.size(), c->stream));
+    return 0;
+}
+
+}  // namespace
+
+int build_exchange_fp64(llg_ctx* c, const llg_exchange& e) {
+    SLLG_TRY(validate_and_copy(c, e));
+    const HostExch& h = c->hex;
+    Exch<double>& x = c->ex64;
+    fill_common(x, h, c->n);
+    c->exch_encoding = "none";
+    if (h.fmt == LLG_EXCH_CSR) {
+        SLLG_TRY(upload(c->ex_a, h.xadj.data(), h.xadj.size(), c->stream));
+        SLLG_TRY(upload(c->ex_b, h.adj.data(), h.adj.size(), c->stream));
+        SLLG_TRY(upload(c->ex_c, h.vals.data(), h.vals.size(), c->stream));
+        x.xadj = c->ex_a.as<uint32_t>();
+        x.adj = c->ex_b.as<uint32_t>();
+        x.vals = c->ex_c.as<double>();
+        c->exch_encoding = "csr-u32-f64";
+    } else if (h.fmt == LLG_EXCH_DIA) {
+        SLLG_TRY(upload(c->ex_a, h.doff.data(), h.doff.size(), c->stream));
+        SLLG_TRY(upload(c->ex_c, h.vals.data(), h.vals.size(), c->stream));
+        x.doff = c->ex_a.as<int32_t>();
+        x.vals = c->ex_c.as<double>();
+        c->exch_encoding = "dia-f64";
+    } else if (h.fmt == LLG_EXCH_STENCIL) {
+        SLLG_TRY(upload_stencil_tables(c));
+        SLLG_TRY(upload(c->ex_c, h.vals.data(), h.vals.size(), c->stream));
+        x.sent = c->ex_a.as<int4>();
+        x.scount = c->ex_b.as<uint32_t>();
+        x.vals = c->ex_c.as<double>();
+        c->exch_encoding = h.per_row ? "stencil-rowvals-f64" : "stencil-shared-f64";
+    }
+    return 0;
+}
+
+namespace {
+
+struct TupleKey {
+    double v[9];
+    int nc;
+    bool operator==(const TupleKey& o) const { return std::memcmp(v, o.v, sizeof(double) * nc) == 0; }
+};
+struct TupleHash {
+    size_t operator()(const TupleKey& k) const {
+        uint64_t h = 1469598103934665603ull;
+        const unsigned char* p = reinterpret_cast<const unsigned char*>(k.v);
+        for (size_t i = 0; i < sizeof(double) * (size_t)k.nc; ++i) h = (h ^ p[i]) * 1099511628211ull;
+        return (size_t)h;
+    }
+};
+
+// CSR -> sliced ELL (packed or raw).
+int build_sell(llg_ctx* c) {
+    const HostExch& h = c->hex;
+    const uint32_t n = c->n;
+    const int nc = h.ncomp;
+    const uint32_t nsl = (n + kSlice - 1) / kSlice;
+    std::vector<uint32_t> slice_off(nsl + 1, 0);
+    for (uint32_t s = 0; s < nsl; ++s) {
+        uint32_t mx = 0;
+        for (uint32_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r) mx = std::max(mx, h.xadj[r + 1] - h.xadj[r]);
+        slice_off[s + 1] = slice_off[s] + mx;
+    }
+    const uint64_t nent = (uint64_t)slice_off[nsl] * kSlice;
+    bool padded = false;
+    for (uint32_t r = 0; r < n && !padded; ++r) {
+        const uint32_t s = r / kSlice;
+        if (h.xadj[r + 1] - h.xadj[r] != slice_off[s + 1] - slice_off[s]) padded = true;
+    }
+    if (n % kSlice) padded = true;
+    // Try the packed form.
+    bool pack = true;
+    const int64_t half = (int64_t)n / 2;
+    std::vector<int16_t> delta(h.nnz);
+    for (uint32_t r = 0; r < n && pack; ++r)
+        for (uint32_t q = h.xadj[r]; q < h.xadj[r + 1]; ++q) {
+            int64_t d = (int64_t)h.adj[q] - (int64_t)r;
+            if (d > half) d -= n;
+            if (d < -half) d += n;
+            if (d < -32768 || d > 32767) { pack = false; break; }
+            delta[q] = (int16_t)d;
+        }
+    std::unordered_map<TupleKey, uint32_t, TupleHash> dict;
+    std::vector<TupleKey> tuples;
+    std::vector<uint32_t> idx;
+    if (pack) {
+        idx.resize(h.nnz);
+        TupleKey last{};
+        last.nc = -1;
+        uint32_t last_id = 0;
+        for (uint64_t q = 0; q < h.nnz && pack; ++q) {
+            TupleKey k{};
+            k.nc = nc;
+            for (int a = 0; a < nc; ++a) k.v[a] = h.vals[(size_t)a * h.nnz + q];
+            if (last.nc == nc && k == last) { idx[q] = last_id; continue; }
+            auto it = dict.find(k);
+            uint32_t id;
+            if (it == dict.end()) {
+                id = (uint32_t)tuples.size();
+                if (id >= 65535u) { pack = false; break; }
+                dict.emplace(k, id);
+                tuples.push_back(k);
+            } else {
+                id = it->second;
+            }
+            idx[q] = id;
+            last = k;
+            last_id = id;
+        }
+    }
+    Exch<float>& x = c->ex32;
+    Exch<double>& x64 = c->ex64;
+    SLLG_TRY(upload(c->ex_a, slice_off.data(), slice_off.size(), c->stream));
+    std::vector<uint32_t> ent(nent, 0);
+    if (pack) {
+        uint32_t zero_id = 0;
+        if (padded) {
+            TupleKey z{};
+            z.nc = nc;
+            auto it = dict.find(z);
+            if (it == dict.end()) { zero_id = (uint32_t)tuples.size(); tuples.push_back(z); }
+            else zero_id = it->second;
+        }
+        for (uint32_t s = 0; s < nsl; ++s) {
+            const uint32_t len = slice_off[s + 1] - slice_off[s];
+            for (uint32_t l = 0; l < kSlice; ++l) {
+                const uint32_t r = s * kSlice + l;
+                for (uint32_t q = 0; q < len; ++q) {
+                    uint32_t w = zero_id;
+                    if (r < n && q < h.xadj[r + 1] - h.xadj[r]) {
+                        const uint32_t qq = h.xadj[r] + q;
+                        w = ((uint32_t)(uint16_t)delta[qq] << 16) | idx[qq];
+                    }
+                    ent[((size_t)slice_off[s] + q) * kSlice + l] = w;
+                }
+            }
+        }
+        std::vector<float> d32(tuples.size() * nc);
+        std::vector<double> d64(tuples.size() * nc);
+        for (size_t t = 0; t < tuples.size(); ++t)
+            for (int a = 0; a < nc; ++a) {
+                d64[t * nc + a] = tuples[t].v[a];
+                d32[t * nc + a] = (float)tuples[t].v[a];
+            }
+        SLLG_TRY(upload(c->ex_b, ent.data(), ent.size(), c->stream));
+        SLLG_TRY(upload(c->ex_c, d32.data(), d32.size(), c->stream));
+        SLLG_TRY(upload(c->ex_dict64, d64.data(), d64.size(), c->stream));
+        x.fmt = FMT_SELL_PACK;
+        x.ndict = (uint32_t)tuples.size();
+        char buf[48];
+        std::snprintf(buf, sizeof buf, "sell64-pack16x16-dict%u", x.ndict);
+        c->exch_encoding = buf;
+    } else {
+        std::vector<float> v32((size_t)nc * nent, 0.0f);
+        std::vector<double> v64((size_t)nc * nent, 0.0);
+        for (uint32_t s = 0; s < nsl; ++s) {
+            const uint32_t len = slice_off[s + 1] - slice_off[s];
+            for (uint32_t l = 0; l < kSlice; ++l) {
+                const uint32_t r = s * kSlice + l;
+                for (uint32_t q = 0; q < len; ++q) {
+                    const size_t o = ((size_t)slice_off[s] + q) * kSlice + l;
+                    if (r < n && q < h.xadj[r + 1] - h.xadj[r]) {
+                        const uint32_t qq = h.xadj[r] + q;
+                        ent[o] = h.adj[qq];
+                        for (int a = 0; a < nc; ++a) {
+                            v64[(size_t)a * nent + o] = h.vals[(size_t)a * h.nnz + qq];
+                            v32[(size_t)a * nent + o] = (float)h.vals[(size_t)a * h.nnz + qq];
+                        }
+                    } else {
+                        ent[o] = r < n ? r : 0;
+                    }
+                }
+            }
+        }
+        SLLG_TRY(upload(c->ex_b, ent.data(), ent.size(), c->stream));
+        SLLG_TRY(upload(c->ex_c, v32.data(), v32.size(), c->stream));
+        SLLG_TRY(upload(c->ex_dict64, v64.data(), v64.size(), c->stream));
+        x.fmt = FMT_SELL_RAW;
+        x.ndict = 0;
+        c->exch_encoding = "sell64-u32-f32";
+    }
+    x.slice_off = c->ex_a.as<uint32_t>();
+    x.ent = c->ex_b.as<uint32_t>();
+    x.dict = c->ex_c.as<float>();
+    x.nent = nent;
+    x64 = Exch<double>();
+    fill_common(x64, h, n);
+    x64.fmt = x.fmt;
+    x64.slice_off = x.slice_off;
+    x64.ent = x.ent;
+    x64.dict = c->ex_dict64.as<double>();
+    x64.ndict = x.ndict;
+    x64.nent = nent;
+    return 0;
+}
+
+}  // namespace
+
+int build_exchange_fp32(llg_ctx* c, const llg_exchange& e) {
+    SLLG_TRY(validate_and_copy(c, e));
+    const HostExch& h = c->hex;
+    Exch<float>& x = c->ex32;
+    fill_common(x, h, c->n);
+    fill_common(c->ex64, h, c->n);
+    c->exch_encoding = "none";
+    if (h.fmt == LLG_EXCH_CSR) {
+        SLLG_TRY(build_sell(c));
+    } else if (h.fmt == LLG_EXCH_DIA || h.fmt == LLG_EXCH_STENCIL) {
+        std::vector<float> v32(h.vals.size());
+        for (size_t q = 0; q < v32.size(); ++q) v32[q] = (float)h.vals[q];
+        SLLG_TRY(upload(c->ex_c, v32.data(), v32.size(), c->stream));
+        SLLG_TRY(upload(c->ex_dict64, h.vals.data(), h.vals.size(), c->stream));
+        if (h.fmt == LLG_EXCH_DIA) {
+            SLLG_TRY(upload(c->ex_a, h.doff.data(), h.doff.size(), c->stream));
+            x.doff = c->ex64.doff = c->ex_a.as<int32_t>();
+            c->exch_encoding = "dia-f32";
+        } else {
+            SLLG_TRY(upload_stencil_tables(c));
+            x.sent = c->ex64.sent = c->ex_a.as<int4>();
+            x.scount = c->ex64.scount = c->ex_b.as<uint32_t>();
+            c->exch_encoding = h.per_row ? "stencil-rowvals-f32" : "stencil-shared-f32";
+        }
+        x.vals = c->ex_c.as<float>();
+        c->ex64.vals = c->ex_dict64.as<double>();
+    }
+    return 0;
+}
+
+}  // namespace sllg

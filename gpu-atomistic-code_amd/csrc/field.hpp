@@ -1,0 +1,159 @@
+// field.hpp - effective-field assembly shared by the FP64 plain kernels and the
+// FP32 fused kernels.  The order of every floating-point operation is the order of
+// the reference GPU kernels (src/cufields.cu:180-246 exchange, src/cuint.cu:72-150
+// local terms and torque), so that the FP64 path built with -ffp-contract=off
+// reproduces the FP64 oracle bit for bit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ctx.hpp"
+
+namespace sllg {
+
+// dot += J s for one neighbour: isotropic (NC 1), diagonal (NC 3), full tensor (NC 9,
+// val[co][ro] * s[ro] for ro = 0,1,2 in turn as in src/cufields.cu:233-238).
+template <int NC, class R>
+__device__ __forceinline__ void accum(const R v[NC], const R s[3], R dot[3]) {
+    if constexpr (NC == 1) {
+        dot[0] += v[0] * s[0];
+        dot[1] += v[0] * s[1];
+        dot[2] += v[0] * s[2];
+    } else if constexpr (NC == 3) {
+        dot[0] += v[0] * s[0];
+        dot[1] += v[1] * s[1];
+        dot[2] += v[2] * s[2];
+    } else {
+#pragma unroll
+        for (int co = 0; co < 3; ++co)
+#pragma unroll
+            for (int ro = 0; ro < 3; ++ro) dot[co] += v[co * 3 + ro] * s[ro];
+    }
+}
+
+// Neighbour index of stencil entry en for spin i (spin order ((x*d1+y)*d2+z)*na+t);
+// returns false across a non-periodic face.
+template <class V>
+__device__ __forceinline__ bool stencil_neighbour(const Exch<V>& e, uint32_t i, int4 en, uint32_t& j) {
+    const uint32_t t = i % e.na;
+    uint32_t r = i / e.na;
+    const int32_t z = (int32_t)(r % e.dims[2]);
+    r /= e.dims[2];
+    const int32_t y = (int32_t)(r % e.dims[1]);
+    const int32_t x = (int32_t)(r / e.dims[1]);
+    int32_t p[3] = {x + en.x, y + en.y, z + en.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int32_t D = (int32_t)e.dims[a];
+        if (p[a] < 0) {
+            if (!e.pbc[a]) return false;
+            p[a] += D;
+        } else if (p[a] >= D) {
+            if (!e.pbc[a]) return false;
+            p[a] -= D;
+        }
+    }
+    (void)t;
+    j = (((uint32_t)p[0] * e.dims[1] + (uint32_t)p[1]) * e.dims[2] + (uint32_t)p[2]) * e.na + (uint32_t)en.w;
+    return true;
+}
+
+// Exchange sum of row i in the reference formats (CSR, DIA) and the periodic stencil,
+// reading spins from S (SoA, precision SpinT) and couplings of precision V,
+// accumulating in R.
+template <int NC, class R, class V, class SpinT>
+__device__ __forceinline__ void exch_row_ref(const Exch<V>& e, uint32_t i, const SpinT* __restrict__ S,
+                                             R dot[3]) {
+    const uint32_t n = e.n;
+    dot[0] = dot[1] = dot[2] = R(0);
+    R v[NC];
+    if (e.fmt == FMT_CSR) {
+        const uint32_t q0 = e.xadj[i], q1 = e.xadj[i + 1];
+        for (uint32_t q = q0; q < q1; ++q) {
+            const uint32_t j = e.adj[q];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) v[k] = (R)e.vals[(size_t)k * e.nnz + q];
+            const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
+            accum<NC, R>(v, s, dot);
+        }
+    } else if (e.fmt == FMT_DIA) {
+        for (uint32_t q = 0; q < e.ndiag; ++q) {
+            const int64_t col = (int64_t)i + e.doff[q];
+            if (col >= 0 && col < (int64_t)n) {
+#pragma unroll
+                for (int k = 0; k < NC; ++k) v[k] = (R)e.vals[((size_t)k * e.ndiag + q) * n + i];
+                const R s[3] = {(R)S[col], (R)S[n + col], (R)S[2 * (size_t)n + col]};
+                accum<NC, R>(v, s, dot);
+            }
+        }
+    } else if (e.fmt == FMT_STENCIL) {
+        const uint32_t t = i % e.na;
+        const uint32_t cnt = e.scount[t];
+        for (uint32_t q = 0; q < cnt; ++q) {
+            const int4 en = e.sent[t * e.sk + q];
+            uint32_t j;
+            if (!stencil_neighbour(e, i, en, j)) continue;
+            if (e.per_row) {
+#pragma unroll
+                for (int k = 0; k < NC; ++k) v[k] = (R)e.vals[((size_t)k * e.sk + q) * n + i];
+            } else {
+#pragma unroll
+                for (int k = 0; k < NC; ++k) v[k] = (R)e.vals[((size_t)k * e.na + t) * e.sk + q];
+            }
+            const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
+            accum<NC, R>(v, s, dot);
+        }
+    }
+}
+
+// Local terms: uniaxial (src/cuint.cu:77-83), K2 perp/par (:98-110), K4 per axis perp then
+// par (:117-137), cubic (new).  k1u is in tesla (2K/(mu mu_B)), pf = 2K/(mu_0 mu_B).
+template <class R>
+__device__ __forceinline__ void local_terms(R k1u, const R d[3], R kc, const AnisG<R>& an, const R s[3], R h[3]) {
+    if (k1u != R(0)) {
+        const R sdn = s[0] * d[0] + s[1] * d[1] + s[2] * d[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) h[a] += (k1u * sdn * d[a]);
+    }
+    if (an.has_k2perp) {
+        const R sd = s[0] * an.k2perp_dir[0] + s[1] * an.k2perp_dir[1] + s[2] * an.k2perp_dir[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) h[a] += an.k2perp_pf * sd * an.k2perp_dir[a];
+    }
+    if (an.has_k2par) {
+        const R sd = s[0] * an.k2par_dir[0] + s[1] * an.k2par_dir[1] + s[2] * an.k2par_dir[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) h[a] += an.k2par_pf * sd * an.k2par_dir[a];
+    }
+    if (an.has_k4perp || an.has_k4par) {
+#pragma unroll
+        for (int dir = 0; dir < 3; ++dir) {
+            const R* dp = &an.k4perp_dirs[dir * 3];
+            const R* da = &an.k4par_dirs[dir * 3];
+            const R sdp = s[0] * dp[0] + s[1] * dp[1] + s[2] * dp[2];
+            const R sda = s[0] * da[0] + s[1] * da[1] + s[2] * da[2];
+            if (an.has_k4perp) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) h[a] += an.k4perp_pf * sdp * dp[a];
+            }
+            if (an.has_k4par) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) h[a] += an.k4par_pf * sda * da[a];
+            }
+        }
+    }
+    if (kc != R(0)) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) h[a] += kc * (s[a] * s[a] * s[a]);
+    }
+}
+
+// f = llgpf (S x h + lambda S x (S x h))  (src/cuint.cu:142-150)
+template <class R>
+__device__ __forceinline__ void llg_rhs(R llgpf, R lambda, const R s[3], const R h[3], R f[3]) {
+    const R sxh[3] = {s[1] * h[2] - s[2] * h[1], s[2] * h[0] - s[0] * h[2], s[0] * h[1] - s[1] * h[0]};
+    const R sxsxh[3] = {s[1] * sxh[2] - s[2] * sxh[1], s[2] * sxh[0] - s[0] * sxh[2], s[0] * sxh[1] - s[1] * sxh[0]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f[a] = llgpf * (sxh[a] + lambda * sxsxh[a]);
+}
+
+}  // namespace sllg

@@ -1,0 +1,244 @@
+// k_dipolar.hip - dipole-dipole field by zero-padded FFT convolution.
+//
+// Reference: kernel setup intmat::fillDipIntmat (src/intmat.cpp:263-331) + FP64 FFTW
+// (src/intmat.cpp:373-398) cast to float (src/cuda_util.cu:149-178); per refresh
+// CZero4DRSArrays, CdipCopySpin, cuFFT C2C forward, CdipFConv (9 complex MACs per
+// k-point), cuFFT C2C inverse, CdipCopyFields (/zps) (src/cuda.cu:65-90,
+// src/cufields.cu:294-358,409-462).
+//
+// Here: the padded grid G_a = 2 dim_a nk_a per axis; the kernel
+//   N_ab(r) = 1e-7 mu_B (3 e_a e_b - delta_ab) / |r|^3,  r = d * abc/nk,
+// with d_a = g_a for g_a < D_a and g_a - G_a for g_a > D_a (D_a = G_a/2), the origin
+// excluded, is built in FP64 on the host.  The plane g_a = D_a never couples two spins
+// (positions lie in [0, D_a)), so it is left zero; N is then even under r -> -r and its
+// transform is real: 6 real components (xx xy xz yy yz zz) are kept after an FP64 D2Z
+// transform on the device.  Each refresh: scatter mu_i S_i into the persistent real grid
+// (empty points stay zero, so no per-refresh zero fill), R2C (3 batched transforms),
+// H_k = N_k S_k (6 real x complex per k-point), C2R, gather / P.  FP32 path: FP32 grids
+// and transforms with the FP64-built kernel cast to FP32; FP64 plain path: FP64 throughout.
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace sllg {
+
+struct DipolarPlan {
+    bool f64 = false;
+    int G[3] = {0, 0, 0};
+    size_t P = 0, Ph = 0;
+    DevBuf lin;    // u32 per spin: linear index in the padded grid
+    DevBuf mu;     // per spin moment (mu_B), path precision
+    DevBuf Nk;     // [6][Ph] real, path precision
+    DevBuf grid;   // [3][P] real input grid
+    DevBuf spec;   // [3][Ph] complex
+    DevBuf hgrid;  // [3][P] real output grid
+    hipfftHandle fwd = 0, inv = 0;
+    bool have_fwd = false, have_inv = false;
+    ~DipolarPlan() {
+        if (have_fwd) hipfftDestroy(fwd);
+        if (have_inv) hipfftDestroy(inv);
+    }
+};
+
+#define SLLG_FFT(call)                                                                       \
+    do {                                                                                     \
+        hipfftResult r_ = (call);                                                            \
+        if (r_ != HIPFFT_SUCCESS) SLLG_FAIL(LLG_E_DEVICE, "%s failed: hipfft %d", #call, (int)r_); \
+    } while (0)
+
+template <class R>
+__global__ __launch_bounds__(kTPB) void k_dip_scatter(uint32_t n, size_t P, const uint32_t* __restrict__ lin,
+                                                      const R* __restrict__ mu, const R* __restrict__ S,
+                                                      R* __restrict__ grid) {
+    const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
+    if (i >= n) return;
+    const size_t g = lin[i];
+    const R m = mu[i];
+    grid[g] = m * S[i];
+    grid[P + g] = m * S[(size_t)n + i];
+    grid[2 * P + g] = m * S[2 * (size_t)n + i];
+}
+
+// H_a(k) = sum_b N_ab(k) S_b(k), N real symmetric: 6 comps (xx xy xz yy yz zz).
+template <class R, class C>
+__global__ __launch_bounds__(kTPB) void k_dip_mult(size_t Ph, const R* __restrict__ Nk, C* __restrict__ spec) {
+    const size_t k = (size_t)blockIdx.x * kTPB + threadIdx.x;
+    if (k >= Ph) return;
+    const C sx = spec[k], sy = spec[Ph + k], sz = spec[2 * Ph + k];
+    const R nxx = Nk[k], nxy = Nk[Ph + k], nxz = Nk[2 * Ph + k];
+    const R nyy = Nk[3 * Ph + k], nyz = Nk[4 * Ph + k], nzz = Nk[5 * Ph + k];
+    C hx, hy, hz;
+    hx.x = nxx * sx.x + nxy * sy.x + nxz * sz.x;
+    hx.y = nxx * sx.y + nxy * sy.y + nxz * sz.y;
+    hy.x = nxy * sx.x + nyy * sy.x + nyz * sz.x;
+    hy.y = nxy * sx.y + nyy * sy.y + nyz * sz.y;
+    hz.x = nxz * sx.x + nyz * sy.x + nzz * sz.x;
+    hz.y = nxz * sx.y + nyz * sy.y + nzz * sz.y;
+    spec[k] = hx;
+    spec[Ph + k] = hy;
+    spec[2 * Ph + k] = hz;
+}
+
+template <class R>
+__global__ __launch_bounds__(kTPB) void k_dip_gather(uint32_t n, size_t P, R invP, const uint32_t* __restrict__ lin,
+                                                     const R* __restrict__ hgrid, R* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
+    if (i >= n) return;
+    const size_t g = lin[i];
+    out[i] = hgrid[g] * invP;
+    out[(size_t)n + i] = hgrid[P + g] * invP;
+    out[2 * (size_t)n + i] = hgrid[2 * P + g] * invP;
+}
+
+template <class R>
+__global__ __launch_bounds__(kTPB) void k_take_real(size_t m, const hipfftDoubleComplex* __restrict__ in, R* __restrict__ out) {
+    const size_t k = (size_t)blockIdx.x * kTPB + threadIdx.x;
+    if (k >= m) return;
+    out[k] = (R)in[k].x;
+}
+
+// FP64 real-space kernel on the padded grid, component-major [6][P].
+static void fill_kernel(const int G[3], const uint32_t dim[3], const uint32_t nk[3], const double abc[3],
+                        std::vector<double>& N) {
+    const size_t P = (size_t)G[0] * G[1] * G[2];
+    N.assign(6 * P, 0.0);
+    const int D[3] = {G[0] / 2, G[1] / 2, G[2] / 2};
+    const int pairs[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+    for (int i = 0; i < G[0]; ++i)
+        for (int j = 0; j < G[1]; ++j)
+            for (int k = 0; k < G[2]; ++k) {
+                if (i == 0 && j == 0 && k == 0) continue;
+                const int g[3] = {i, j, k};
+                bool skip = false;
+                double r[3];
+                for (int a = 0; a < 3; ++a) {
+                    if (g[a] == D[a] && D[a] > 0) { skip = true; break; }
+                    const int d = g[a] < D[a] ? g[a] : g[a] - G[a];
+                    r[a] = static_cast<double>(d) * abc[a] / static_cast<double>(nk[a]);
+                }
+                if (skip) continue;
+                const double mr = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                const double oo = 1. / (mr * mr * mr);
+                const double e[3] = {r[0] / mr, r[1] / mr, r[2] / mr};
+                const size_t lin = ((size_t)i * G[1] + j) * G[2] + k;
+                for (int p = 0; p < 6; ++p) {
+                    const int a = pairs[p][0], b = pairs[p][1];
+                    N[p * P + lin] = 1e-7 * ((3.0 * e[a] * e[b]) - (a == b ? 1.0 : 0.0)) * oo * LLG_MU_B;
+                }
+            }
+    (void)dim;
+}
+
+int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_species) {
+    DipolarPlan* p = new DipolarPlan();
+    c->dplan = p;
+    p->f64 = (c->prec == LLG_FP64_PLAIN);
+    for (int a = 0; a < 3; ++a) {
+        if (d.dim[a] == 0 || d.nk[a] == 0 || !(d.abc[a] > 0)) SLLG_FAIL(LLG_E_INVALID, "dipolar dim/nk/abc must be positive");
+        p->G[a] = (int)(2 * d.dim[a] * d.nk[a]);
+    }
+    p->P = (size_t)p->G[0] * p->G[1] * p->G[2];
+    p->Ph = (size_t)p->G[0] * p->G[1] * (p->G[2] / 2 + 1);
+    const uint32_t n = c->n;
+    std::vector<uint32_t> lin(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t m[3];
+        for (int a = 0; a < 3; ++a) {
+            m[a] = d.mesh[(size_t)i * 3 + a];
+            if (m[a] >= d.dim[a] * d.nk[a]) SLLG_FAIL(LLG_E_INVALID, "mesh coordinate of spin %u outside dim*nk", i);
+        }
+        lin[i] = (uint32_t)(((size_t)m[0] * p->G[1] + m[1]) * p->G[2] + m[2]);
+    }
+    const size_t rs = p->f64 ? sizeof(double) : sizeof(float);
+    SLLG_TRY(p->lin.alloc(n * sizeof(uint32_t)));
+    SLLG_HIP(hipMemcpy(p->lin.p, lin.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    {
+        std::vector<double> mu64(n);
+        std::vector<float> mu32(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t s = spin_species ? spin_species[i] : (i % c->ns);
+            mu64[i] = c->mu_sp[s];
+            mu32[i] = (float)c->mu_sp[s];
+        }
+        SLLG_TRY(p->mu.alloc(n * rs));
+        SLLG_HIP(hipMemcpy(p->mu.p, p->f64 ? (void*)mu64.data() : (void*)mu32.data(), n * rs, hipMemcpyHostToDevice));
+    }
+    // kernel: FP64 real space -> FP64 D2Z on the device -> real part in path precision
+    {
+        std::vector<double> N;
+        fill_kernel(p->G, d.dim, d.nk, d.abc, N);
+        DevBuf dN, dNk;
+        SLLG_TRY(dN.alloc(6 * p->P * sizeof(double)));
+        SLLG_TRY(dNk.alloc(6 * p->Ph * sizeof(hipfftDoubleComplex)));
+        SLLG_HIP(hipMemcpy(dN.p, N.data(), 6 * p->P * sizeof(double), hipMemcpyHostToDevice));
+        hipfftHandle h;
+        int dims[3] = {p->G[0], p->G[1], p->G[2]};
+        SLLG_FFT(hipfftPlanMany(&h, 3, dims, nullptr, 1, (int)p->P, nullptr, 1, (int)p->Ph, HIPFFT_D2Z, 6));
+        hipfftResult r = hipfftExecD2Z(h, dN.as<double>(), dNk.as<hipfftDoubleComplex>());
+        hipfftDestroy(h);
+        if (r != HIPFFT_SUCCESS) SLLG_FAIL(LLG_E_DEVICE, "kernel D2Z failed: %d", (int)r);
+        SLLG_TRY(p->Nk.alloc(6 * p->Ph * rs));
+        const size_t m = 6 * p->Ph;
+        if (p->f64)
+            k_take_real<double><<<blocks_for(m, kTPB), kTPB>>>(m, dNk.as<hipfftDoubleComplex>(), p->Nk.as<double>());
+        else
+            k_take_real<float><<<blocks_for(m, kTPB), kTPB>>>(m, dNk.as<hipfftDoubleComplex>(), p->Nk.as<float>());
+        SLLG_HIP(hipGetLastError());
+        SLLG_HIP(hipDeviceSynchronize());
+    }
+    SLLG_TRY(p->grid.alloc(3 * p->P * rs));
+    SLLG_TRY(p->hgrid.alloc(3 * p->P * rs));
+    SLLG_TRY(p->spec.alloc(3 * p->Ph * 2 * rs));
+    SLLG_HIP(hipMemset(p->grid.p, 0, 3 * p->P * rs));
+    int dims[3] = {p->G[0], p->G[1], p->G[2]};
+    SLLG_FFT(hipfftPlanMany(&p->fwd, 3, dims, nullptr, 1, (int)p->P, nullptr, 1, (int)p->Ph,
+                            p->f64 ? HIPFFT_D2Z : HIPFFT_R2C, 3));
+    p->have_fwd = true;
+    SLLG_FFT(hipfftPlanMany(&p->inv, 3, dims, nullptr, 1, (int)p->Ph, nullptr, 1, (int)p->P,
+                            p->f64 ? HIPFFT_Z2D : HIPFFT_C2R, 3));
+    p->have_inv = true;
+    SLLG_FFT(hipfftSetStream(p->fwd, c->stream));
+    SLLG_FFT(hipfftSetStream(p->inv, c->stream));
+    return 0;
+}
+
+void dipolar_destroy(llg_ctx* c) {
+    delete c->dplan;
+    c->dplan = nullptr;
+}
+
+int dipolar_refresh(llg_ctx* c, const void* S, void* out) {
+    DipolarPlan* p = c->dplan;
+    if (!p) SLLG_FAIL(LLG_E_STATE, "dipolar plan missing");
+    const uint32_t n = c->n;
+    const dim3 gn(blocks_for(n, kTPB)), gk(blocks_for(p->Ph, kTPB)), b(kTPB);
+    if (p->f64) {
+        k_dip_scatter<double><<<gn, b, 0, c->stream>>>(n, p->P, p->lin.as<uint32_t>(), p->mu.as<double>(),
+                                                       (const double*)S, p->grid.as<double>());
+        SLLG_HIP(hipGetLastError());
+        SLLG_FFT(hipfftExecD2Z(p->fwd, p->grid.as<double>(), p->spec.as<hipfftDoubleComplex>()));
+        k_dip_mult<double, hipfftDoubleComplex><<<gk, b, 0, c->stream>>>(p->Ph, p->Nk.as<double>(),
+                                                                         p->spec.as<hipfftDoubleComplex>());
+        SLLG_HIP(hipGetLastError());
+        SLLG_FFT(hipfftExecZ2D(p->inv, p->spec.as<hipfftDoubleComplex>(), p->hgrid.as<double>()));
+        k_dip_gather<double><<<gn, b, 0, c->stream>>>(n, p->P, 1.0 / (double)p->P, p->lin.as<uint32_t>(),
+                                                      p->hgrid.as<double>(), (double*)out);
+    } else {
+        k_dip_scatter<float><<<gn, b, 0, c->stream>>>(n, p->P, p->lin.as<uint32_t>(), p->mu.as<float>(),
+                                                      (const float*)S, p->grid.as<float>());
+        SLLG_HIP(hipGetLastError());
+        SLLG_FFT(hipfftExecR2C(p->fwd, p->grid.as<float>(), p->spec.as<hipfftComplex>()));
+        k_dip_mult<float, hipfftComplex><<<gk, b, 0, c->stream>>>(p->Ph, p->Nk.as<float>(), p->spec.as<hipfftComplex>());
+        SLLG_HIP(hipGetLastError());
+        SLLG_FFT(hipfftExecC2R(p->inv, p->spec.as<hipfftComplex>(), p->hgrid.as<float>()));
+        k_dip_gather<float><<<gn, b, 0, c->stream>>>(n, p->P, 1.0f / (float)p->P, p->lin.as<uint32_t>(),
+                                                     p->hgrid.as<float>(), (float*)out);
+    }
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace sllg

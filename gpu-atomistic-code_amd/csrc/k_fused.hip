@@ -1,0 +1,434 @@
+// k_fused.hip - FP32 fused path: one kernel per field evaluation that gathers the
+// exchange field, adds dipolar/staggered/thermal/applied/anisotropy terms, forms the
+// LLG right-hand side and advances the integrator, with |S| = 1 renormalisation.
+//
+// Heun (the reference CHeun1 + SpMV + CHeun2 sequence, src/cuda.cu:92-233) becomes two
+// passes:  predictor  reads S, writes Y = N(S + rdt f(S)) and A = S + rdt/2 f(S);
+//          corrector  reads Y, A, writes S = N(A + rdt/2 f(Y)).
+// Bytes per spin-update: 18*4 (spins) + 2*M (couplings), see the per-format M in encode.cpp.
+// Thermal noise is regenerated from Philox in both passes (0 bytes), so the corrector
+// sees the same xi as the predictor (src/cuint.cu:69,174).
+//
+// RK4 / Cash-Karp RK5 stages (src/cuintRK4.cu, src/cuintRK5.cu) are one pass per stage
+// with ping-pong stage inputs, textbook stage formation.
+#include "field.hpp"
+#include "kernels.hpp"
+#include "philox.hpp"
+
+namespace sllg {
+
+// Sliced ELL row: slice = 64 consecutive rows (one wavefront), entries of slice-local
+// column q of all 64 rows are contiguous, so each load instruction reads 256 bytes.
+// Packed entry: high 16 bits = column delta (j - i, wrapped mod n into int16), low 16
+// bits = index into the coupling dictionary [ndict][NC].  Raw entry: u32 column, values
+// [NC][nent] in the same slice layout.
+template <int NC, class R, class V, class SpinT>
+__device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, const SpinT* __restrict__ S, R dot[3]) {
+    const uint32_t n = e.n;
+    dot[0] = dot[1] = dot[2] = R(0);
+    const uint32_t sl = i >> 6, lane = i & 63u;
+    const uint32_t q0 = e.slice_off[sl];
+    const uint32_t len = e.slice_off[sl + 1] - q0;
+    const uint32_t* __restrict__ ent = e.ent + (size_t)q0 * kSlice + lane;
+    R v[NC];
+    if (e.fmt == FMT_SELL_PACK) {
+#pragma unroll 6
+        for (uint32_t q = 0; q < len; ++q) {
+            const uint32_t w = ent[(size_t)q * kSlice];
+            int32_t j = (int32_t)i + (int32_t)(int16_t)(w >> 16);
+            j += (j < 0) ? (int32_t)n : 0;
+            j -= (j >= (int32_t)n) ? (int32_t)n : 0;
+            const V* dv = e.dict + (size_t)(w & 0xFFFFu) * NC;
+#pragma unroll
+            for (int k = 0; k < NC; ++k) v[k] = (R)dv[k];
+            const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
+            accum<NC, R>(v, s, dot);
+        }
+    } else {
+        const V* __restrict__ val = e.dict + (size_t)q0 * kSlice + lane;
+#pragma unroll 4
+        for (uint32_t q = 0; q < len; ++q) {
+            const uint32_t j = ent[(size_t)q * kSlice];
+#pragma unroll
+            for (int k = 0; k < NC; ++k) v[k] = (R)val[(size_t)k * e.nent + (size_t)q * kSlice];
+            const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
+            accum<NC, R>(v, s, dot);
+        }
+    }
+}
+
+template <int FMT, int NC, class R, class V, class SpinT>
+__device__ __forceinline__ void exch_any(const Exch<V>& e, uint32_t i, const SpinT* __restrict__ S, R dot[3]) {
+    if constexpr (FMT == FMT_SELL_PACK || FMT == FMT_SELL_RAW)
+        exch_row_sell<NC, R, V, SpinT>(e, i, S, dot);
+    else if constexpr (FMT == FMT_NONE)
+        dot[0] = dot[1] = dot[2] = R(0);
+    else
+        exch_row_ref<NC, R, V, SpinT>(e, i, S, dot);
+}
+
+struct FusedArgs {
+    uint32_t n;
+    int noise;  // 0 none (T = 0), 1 counter, 2 injected
+    int multi;  // per-spin species index present
+    int xcd;    // XCD-aware block remap
+    float rdt, half_rdt, sqrtT;
+    float B[3];
+    uint64_t seed, stream, t;
+    const double* xi;
+    const float* dem;
+    const float* stg;
+    const uint8_t* spec;
+    const Species<float>* sp;
+    Species<float> sp0;
+    AnisG<float> an;
+    Exch<float> ex;
+};
+
+// Block index -> row tile.  Consecutive workgroups are dispatched round-robin over the
+// 8 XCDs of the MI355X; the remap gives each XCD a contiguous range of rows so the
+// neighbour spins it gathers stay in its own L2.
+__device__ __forceinline__ uint32_t tile_index(int xcd) {
+    const uint32_t b = blockIdx.x;
+    if (!xcd) return b;
+    const uint32_t nb = gridDim.x, per = nb >> 3, rem = nb & 7u;
+    const uint32_t x = b & 7u, l = b >> 3;
+    return x * per + (x < rem ? x : rem) + l;
+}
+
+template <int FMT, int NC>
+__device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const float* __restrict__ src, float s[3],
+                                          float f[3]) {
+    const uint32_t n = a.n;
+    s[0] = src[i];
+    s[1] = src[n + i];
+    s[2] = src[2 * (size_t)n + i];
+    float h[3];
+    exch_any<FMT, NC, float, float, float>(a.ex, i, src, h);
+    if (a.dem) {
+        h[0] += a.dem[i];
+        h[1] += a.dem[n + i];
+        h[2] += a.dem[2 * (size_t)n + i];
+    }
+    if (a.stg) {
+        h[0] += a.stg[i];
+        h[1] += a.stg[n + i];
+        h[2] += a.stg[2 * (size_t)n + i];
+    }
+    const Species<float> sp = a.multi ? a.sp[a.spec[i]] : a.sp0;
+    if (a.noise == 1) {
+        float z[3];
+        normals3_f32(a.seed, a.stream, i, a.t, z);
+        const float TP = a.sqrtT * sp.sigma;
+        h[0] += z[0] * TP;
+        h[1] += z[1] * TP;
+        h[2] += z[2] * TP;
+    } else if (a.noise == 2) {
+        const float TP = a.sqrtT * sp.sigma;
+        h[0] += (float)a.xi[i] * TP;
+        h[1] += (float)a.xi[n + i] * TP;
+        h[2] += (float)a.xi[2 * (size_t)n + i] * TP;
+    }
+    h[0] += a.B[0];
+    h[1] += a.B[1];
+    h[2] += a.B[2];
+    local_terms<float>(sp.k1u, sp.k1dir, sp.kc, a.an, s, h);
+    llg_rhs<float>(sp.llgpf, sp.lambda, s, h, f);
+}
+
+template <int FMT, int NC>
+__global__ __launch_bounds__(kTPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S, float* __restrict__ Y,
+                                                    float* __restrict__ A) {
+    const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t n = a.n;
+    float s[3], f[3];
+    fused_rhs<FMT, NC>(a, i, S, s, f);
+    float y[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
+    const float r = rsqrtf(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Y[(size_t)c * n + i] = y[c] * r;
+        A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
+    }
+}
+
+template <int FMT, int NC>
+__global__ __launch_bounds__(kTPB) void k_heun_corr(FusedArgs a, const float* __restrict__ Y,
+                                                    const float* __restrict__ A, float* __restrict__ S) {
+    const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t n = a.n;
+    float s[3], f[3];
+    fused_rhs<FMT, NC>(a, i, Y, s, f);
+    float p[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p[c] = A[(size_t)c * n + i] + a.half_rdt * f[c];
+    const float r = rsqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) S[(size_t)c * n + i] = p[c] * r;
+}
+
+// RK4: stage m reads Yin (S for m = 0), k = rdt f(Yin);
+//   m0: ACC = k/6, Yout = S + k/2;  m1: ACC += k/3, Yout = S + k/2;  m2: ACC += k/3, Yout = S + k;
+//   m3: S = N(S + ACC + k/6).
+template <int FMT, int NC, int M>
+__global__ __launch_bounds__(kTPB) void k_rk4(FusedArgs a, const float* __restrict__ Yin, float* __restrict__ Yout,
+                                              float* __restrict__ ACC, float* __restrict__ S) {
+    const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t n = a.n;
+    float s[3], f[3];
+    fused_rhs<FMT, NC>(a, i, M == 0 ? S : Yin, s, f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t o = (size_t)c * n + i;
+        const float k = a.rdt * f[c];
+        const float s0 = (M == 0) ? s[c] : S[o];
+        if constexpr (M == 0) {
+            ACC[o] = k * (1.0f / 6.0f);
+            Yout[o] = s0 + 0.5f * k;
+        } else if constexpr (M == 1) {
+            ACC[o] += k * (1.0f / 3.0f);
+            Yout[o] = s0 + 0.5f * k;
+        } else if constexpr (M == 2) {
+            ACC[o] += k * (1.0f / 3.0f);
+            Yout[o] = s0 + k;
+        } else {
+            s[c] = s0 + (ACC[o] + k * (1.0f / 6.0f));
+        }
+    }
+    if constexpr (M == 3) {
+        const float r = rsqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) S[(size_t)c * n + i] = s[c] * r;
+    }
+}
+
+struct Rk5F {
+    float b[6][5];
+};
+__constant__ Rk5F c_rk5f;
+
+// Cash-Karp: stage m reads Yin (S for m = 0), k_m = rdt f(Yin), stores k_m (m <= 4),
+// Yout = S + sum_{j<=m} b_{m+1,j} k_j (m <= 4); m = 5: S = N(S + sum_j c_j k_j).
+template <int FMT, int NC, int M>
+__global__ __launch_bounds__(kTPB) void k_rk5(FusedArgs a, const float* __restrict__ Yin, float* __restrict__ Yout,
+                                              float* __restrict__ S, float* __restrict__ K0, float* __restrict__ K1,
+                                              float* __restrict__ K2, float* __restrict__ K3, float* __restrict__ K4) {
+    const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t n = a.n;
+    float s[3], f[3];
+    fused_rhs<FMT, NC>(a, i, M == 0 ? S : Yin, s, f);
+    float* Ks[5] = {K0, K1, K2, K3, K4};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const size_t o = (size_t)c * n + i;
+        const float k = a.rdt * f[c];
+        const float s0 = (M == 0) ? s[c] : S[o];
+        if constexpr (M < 5) {
+            Ks[M][o] = k;
+            float v = s0;
+#pragma unroll
+            for (int j = 0; j < M; ++j) v += c_rk5f.b[M + 1][j] * Ks[j][o];
+            v += c_rk5f.b[M + 1][M] * k;
+            Yout[o] = v;
+        } else {
+            s[c] = s0 + ((37.0f / 378.0f) * K0[o] + (250.0f / 621.0f) * K2[o] + (125.0f / 594.0f) * K3[o] +
+                         (512.0f / 1771.0f) * k);
+        }
+    }
+    if constexpr (M == 5) {
+        const float r = rsqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) S[(size_t)c * n + i] = s[c] * r;
+    }
+}
+
+// Deterministic field in FP64 at FP32 spins Sp (thermal part excluded).
+struct DetArgs {
+    uint32_t n;
+    int multi;
+    double B[3];
+    const float* dem;
+    const float* stg;
+    const uint8_t* spec;
+    const Species<double>* sp;
+    AnisG<double> an;
+    Exch<double> ex;  // same entries, FP64 dictionary / values
+};
+
+template <int FMT, int NC>
+__global__ __launch_bounds__(kTPB) void k_fused_detfield(DetArgs a, const float* __restrict__ Sp,
+                                                         double* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t n = a.n;
+    double h[3];
+    exch_any<FMT, NC, double, double, float>(a.ex, i, Sp, h);
+    const double s[3] = {(double)Sp[i], (double)Sp[n + i], (double)Sp[2 * (size_t)n + i]};
+    for (int c = 0; c < 3; ++c) {
+        if (a.dem) h[c] += (double)a.dem[(size_t)c * n + i];
+        if (a.stg) h[c] += (double)a.stg[(size_t)c * n + i];
+        h[c] += a.B[c];
+    }
+    const Species<double> sp = a.sp[a.multi ? a.spec[i] : 0];
+    local_terms<double>(sp.k1u, sp.k1dir, sp.kc, a.an, s, h);
+    for (int c = 0; c < 3; ++c) out[(size_t)c * n + i] = h[c];
+}
+
+// ---------------------------------------------------------------- host launchers
+
+static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
+    FusedArgs a{};
+    a.n = c->n;
+    if (c->T > 0.0) a.noise = (c->noise_mode == LLG_NOISE_INJECTED) ? 2 : 1;
+    a.multi = c->multi_species ? 1 : 0;
+    a.xcd = (c->flags & 0x100u) ? 0 : 1;
+    a.rdt = (float)c->rdt;
+    a.half_rdt = (float)(0.5 * c->rdt);
+    a.sqrtT = (float)std::sqrt(c->T);
+    for (int k = 0; k < 3; ++k) a.B[k] = (float)c->B[k];
+    a.seed = c->seed;
+    a.stream = c->stream_id;
+    a.t = s.t;
+    a.xi = s.xi;
+    a.dem = c->dip ? c->DEM.as<float>() : nullptr;
+    a.stg = c->stag_set ? c->STG.as<float>() : nullptr;
+    a.spec = c->spec.as<uint8_t>();
+    a.sp = c->sp32.as<Species<float>>();
+    const Species<double>& s0 = c->sp64[0];
+    a.sp0.llgpf = (float)s0.llgpf; a.sp0.lambda = (float)s0.lambda; a.sp0.sigma = (float)s0.sigma;
+    a.sp0.k1u = (float)s0.k1u; a.sp0.kc = (float)s0.kc;
+    for (int k = 0; k < 3; ++k) a.sp0.k1dir[k] = (float)s0.k1dir[k];
+    a.an = c->an32;
+    a.ex = c->ex32;
+    return a;
+}
+
+#define SLLG_DISPATCH_FMT_NC(FMT_VAL, NC_VAL, MACRO)                                 \
+    do {                                                                            \
+        if ((FMT_VAL) == FMT_NONE) { MACRO(FMT_NONE, 1); }                          \
+        else if ((FMT_VAL) == FMT_SELL_PACK) {                                      \
+            if ((NC_VAL) == 1) { MACRO(FMT_SELL_PACK, 1); }                         \
+            else if ((NC_VAL) == 3) { MACRO(FMT_SELL_PACK, 3); }                    \
+            else { MACRO(FMT_SELL_PACK, 9); }                                       \
+        } else if ((FMT_VAL) == FMT_SELL_RAW) {                                     \
+            if ((NC_VAL) == 1) { MACRO(FMT_SELL_RAW, 1); }                          \
+            else if ((NC_VAL) == 3) { MACRO(FMT_SELL_RAW, 3); }                     \
+            else { MACRO(FMT_SELL_RAW, 9); }                                        \
+        } else if ((FMT_VAL) == FMT_STENCIL) {                                      \
+            if ((NC_VAL) == 1) { MACRO(FMT_STENCIL, 1); }                           \
+            else if ((NC_VAL) == 3) { MACRO(FMT_STENCIL, 3); }                      \
+            else { MACRO(FMT_STENCIL, 9); }                                         \
+        } else {                                                                    \
+            if ((NC_VAL) == 1) { MACRO(FMT_DIA, 1); }                               \
+            else if ((NC_VAL) == 3) { MACRO(FMT_DIA, 3); }                          \
+            else { MACRO(FMT_DIA, 9); }                                             \
+        }                                                                           \
+    } while (0)
+
+int fused_heun_pred(llg_ctx* c, const StepSched& s) {
+    FusedArgs a = fused_args(c, s);
+    const dim3 g(blocks_for(c->n, kTPB)), b(kTPB);
+    float* S = c->S.as<float>();
+    float* Y = c->Y.as<float>();
+    float* A = c->A.as<float>();
+#define SLLG_L(F, N) k_heun_pred<F, N><<<g, b, 0, c->stream>>>(a, S, Y, A)
+    SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+#undef SLLG_L
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+int fused_heun_corr(llg_ctx* c, const StepSched& s) {
+    FusedArgs a = fused_args(c, s);
+    const dim3 g(blocks_for(c->n, kTPB)), b(kTPB);
+    float* S = c->S.as<float>();
+    float* Y = c->Y.as<float>();
+    float* A = c->A.as<float>();
+#define SLLG_L(F, N) k_heun_corr<F, N><<<g, b, 0, c->stream>>>(a, Y, A, S)
+    SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+#undef SLLG_L
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+int fused_rk_init_tables() {
+    static bool done = false;
+    if (done) return 0;
+    Rk5F t{};
+    const double b2[1] = {0.2};
+    const double b3[2] = {3.0 / 40.0, 9.0 / 40.0};
+    const double b4[3] = {0.3, -0.9, 1.2};
+    const double b5[4] = {-11.0 / 54.0, 2.5, -70.0 / 27.0, 35.0 / 27.0};
+    const double b6[5] = {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0};
+    const double* rows[6] = {nullptr, b2, b3, b4, b5, b6};
+    for (int m = 1; m < 6; ++m)
+        for (int j = 0; j < m; ++j) t.b[m][j] = (float)rows[m][j];
+    SLLG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_rk5f), &t, sizeof(t)));
+    done = true;
+    return 0;
+}
+
+// Stage input ping-pong: stage m reads buffer m%2 (Y, A) and writes the other.
+int fused_rk_stage(llg_ctx* c, const StepSched& s, int m) {
+    FusedArgs a = fused_args(c, s);
+    const dim3 g(blocks_for(c->n, kTPB)), b(kTPB);
+    float* S = c->S.as<float>();
+    float* P[2] = {c->Y.as<float>(), c->A.as<float>()};
+    float* Yin = P[m & 1];
+    float* Yout = P[(m + 1) & 1];
+    float* K[6];
+    for (int k = 0; k < 6; ++k) K[k] = c->K[k].as<float>();
+    if (c->scheme == LLG_RK4) {
+        float* ACC = K[0];
+#define SLLG_L(F, N)                                                                              \
+    switch (m) {                                                                                  \
+        case 0: k_rk4<F, N, 0><<<g, b, 0, c->stream>>>(a, Yin, Yout, ACC, S); break;              \
+        case 1: k_rk4<F, N, 1><<<g, b, 0, c->stream>>>(a, Yin, Yout, ACC, S); break;              \
+        case 2: k_rk4<F, N, 2><<<g, b, 0, c->stream>>>(a, Yin, Yout, ACC, S); break;              \
+        default: k_rk4<F, N, 3><<<g, b, 0, c->stream>>>(a, Yin, Yout, ACC, S); break;             \
+    }
+        SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+#undef SLLG_L
+    } else {
+#define SLLG_L(F, N)                                                                                    \
+    switch (m) {                                                                                        \
+        case 0: k_rk5<F, N, 0><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        case 1: k_rk5<F, N, 1><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        case 2: k_rk5<F, N, 2><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        case 3: k_rk5<F, N, 3><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        case 4: k_rk5<F, N, 4><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        default: k_rk5<F, N, 5><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+    }
+        SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+#undef SLLG_L
+    }
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+int fused_detfield(llg_ctx* c, const float* Sp, double* out) {
+    DetArgs a{};
+    a.n = c->n;
+    a.multi = c->multi_species ? 1 : 0;
+    for (int k = 0; k < 3; ++k) a.B[k] = c->B[k];
+    a.dem = c->dip ? c->DEM.as<float>() : nullptr;
+    a.stg = c->stag_set ? c->STG.as<float>() : nullptr;
+    a.spec = c->spec.as<uint8_t>();
+    a.sp = c->sp64d.as<Species<double>>();
+    a.an = c->an64;
+    a.ex = c->ex64;
+    const dim3 g(blocks_for(c->n, kTPB)), b(kTPB);
+#define SLLG_L(F, N) k_fused_detfield<F, N><<<g, b, 0, c->stream>>>(a, Sp, out)
+    SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+#undef SLLG_L
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace sllg

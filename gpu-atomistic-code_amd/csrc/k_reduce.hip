@@ -1,0 +1,301 @@
+// k_reduce.hip - on-device FP64 observables, replacing the reference's D2H copy of the
+// spins every spins::update steps and the host loop of util::calc_mag method 0
+// (src/cuda.cu:234-256, src/util_mag.cpp:44-63).
+//
+// Determinism: each block reduces its grid-stride share with 64-wide wavefront
+// shuffles and a fixed-order combine of its 4 waves; a single block then sums the
+// per-block partials in a fixed order.  No floating-point atomics, so the result is
+// bitwise reproducible for a given n and launch shape.
+#include "field.hpp"
+#include "kernels.hpp"
+#include "philox.hpp"
+
+namespace sllg {
+
+constexpr int kRedBlocks = 512;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sum of v in fixed order; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kTPB / 64; ++k) r += lds[k];
+    return r;
+}
+
+template <class SpinT>
+__global__ __launch_bounds__(kTPB) void k_mag_partial(uint32_t n, uint32_t nsub, const SpinT* __restrict__ S,
+                                                      const uint8_t* __restrict__ sub, double* __restrict__ part) {
+    __shared__ double lds[kTPB / 64];
+    const uint32_t stride = gridDim.x * kTPB;
+    for (uint32_t s = 0; s < nsub; ++s) {
+        double a[3] = {0.0, 0.0, 0.0};
+        for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < n; i += stride) {
+            if (sub && sub[i] != s) continue;
+            a[0] += (double)S[i];
+            a[1] += (double)S[(size_t)n + i];
+            a[2] += (double)S[2 * (size_t)n + i];
+        }
+        for (int c = 0; c < 3; ++c) {
+            const double r = block_sum(a[c], lds);
+            if (threadIdx.x == 0) part[(size_t)blockIdx.x * nsub * 3 + s * 3 + c] = r;
+        }
+    }
+}
+
+// out[v] = scale[v / 3] * sum_b part[b][v]
+__global__ __launch_bounds__(kTPB) void k_sum_partials(uint32_t nblocks, uint32_t nval, const double* __restrict__ part,
+                                                       const double* __restrict__ scale, double* __restrict__ out) {
+    __shared__ double lds[kTPB / 64];
+    for (uint32_t v = 0; v < nval; ++v) {
+        double a = 0.0;
+        for (uint32_t b = threadIdx.x; b < nblocks; b += kTPB) a += part[(size_t)b * nval + v];
+        const double r = block_sum(a, lds);
+        if (threadIdx.x == 0) out[v] = r * (scale ? scale[v / 3] : 1.0);
+    }
+}
+
+// Per-spin energy (J) in FP64:
+//   mu mu_B [ -1/2 S.(H_ex + H_dip + H_quad) - 1/4 S.H_cubic - S.(B + H_stag) ]
+// where H_quad collects the terms linear in S (uniaxial, K2, K4).
+struct EnergyArgs {
+    uint32_t n;
+    int multi;
+    double B[3];
+    const void* dem;  // float (FP32 path) or double (plain), current dipolar field or null
+    const void* stg;
+    int f64;
+    const uint8_t* spec;
+    const Species<double>* sp;
+    const double* mu;  // per species, mu_B units
+    AnisG<double> an;
+    Exch<double> ex;
+};
+
+template <int FMT, int NC, class SpinT>
+__device__ __forceinline__ void exch_dispatch(const Exch<double>& e, uint32_t i, const SpinT* S, double dot[3]);
+
+template <int FMT, int NC, class SpinT>
+__global__ __launch_bounds__(kTPB) void k_energy_partial(EnergyArgs a, const SpinT* __restrict__ S,
+                                                         double* __restrict__ part);
+
+}  // namespace sllg
+
+// The sliced-ELL row reader lives in k_fused.hip; the energy kernel needs the same
+// traversal in FP64, so it is restated here for the reduction's own translation unit.
+namespace sllg {
+
+template <int NC, class SpinT>
+__device__ __forceinline__ void exch_row_sell64(const Exch<double>& e, uint32_t i, const SpinT* __restrict__ S,
+                                                double dot[3]) {
+    const uint32_t n = e.n;
+    dot[0] = dot[1] = dot[2] = 0.0;
+    const uint32_t sl = i >> 6, lane = i & 63u;
+    const uint32_t q0 = e.slice_off[sl];
+    const uint32_t len = e.slice_off[sl + 1] - q0;
+    const uint32_t* ent = e.ent + (size_t)q0 * kSlice + lane;
+    double v[NC];
+    for (uint32_t q = 0; q < len; ++q) {
+        const uint32_t w = ent[(size_t)q * kSlice];
+        uint32_t j;
+        if (e.fmt == FMT_SELL_PACK) {
+            int32_t jj = (int32_t)i + (int32_t)(int16_t)(w >> 16);
+            jj += (jj < 0) ? (int32_t)n : 0;
+            jj -= (jj >= (int32_t)n) ? (int32_t)n : 0;
+            j = (uint32_t)jj;
+            const double* dv = e.dict + (size_t)(w & 0xFFFFu) * NC;
+            for (int k = 0; k < NC; ++k) v[k] = dv[k];
+        } else {
+            j = w;
+            for (int k = 0; k < NC; ++k) v[k] = e.dict[(size_t)k * e.nent + ((size_t)q0 + q) * kSlice + lane];
+        }
+        const double s[3] = {(double)S[j], (double)S[n + j], (double)S[2 * (size_t)n + j]};
+        accum<NC, double>(v, s, dot);
+    }
+}
+
+template <int FMT, int NC, class SpinT>
+__device__ __forceinline__ void exch_dispatch(const Exch<double>& e, uint32_t i, const SpinT* S, double dot[3]) {
+    if constexpr (FMT == FMT_SELL_PACK || FMT == FMT_SELL_RAW)
+        exch_row_sell64<NC, SpinT>(e, i, S, dot);
+    else if constexpr (FMT == FMT_NONE)
+        dot[0] = dot[1] = dot[2] = 0.0;
+    else
+        exch_row_ref<NC, double, double, SpinT>(e, i, S, dot);
+}
+
+template <int FMT, int NC, class SpinT>
+__global__ __launch_bounds__(kTPB) void k_energy_partial(EnergyArgs a, const SpinT* __restrict__ S,
+                                                         double* __restrict__ part) {
+    __shared__ double lds[kTPB / 64];
+    const uint32_t n = a.n;
+    const uint32_t stride = gridDim.x * kTPB;
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * kTPB + threadIdx.x; i < n; i += stride) {
+        double dot[3];
+        exch_dispatch<FMT, NC, SpinT>(a.ex, i, S, dot);
+        const double s[3] = {(double)S[i], (double)S[(size_t)n + i], (double)S[2 * (size_t)n + i]};
+        const uint32_t spx = a.multi ? a.spec[i] : 0;
+        const Species<double> sp = a.sp[spx];
+        double hp[3] = {dot[0], dot[1], dot[2]};
+        if (a.dem) {
+            for (int c = 0; c < 3; ++c)
+                hp[c] += a.f64 ? ((const double*)a.dem)[(size_t)c * n + i] : (double)((const float*)a.dem)[(size_t)c * n + i];
+        }
+        double hq[3] = {0.0, 0.0, 0.0};
+        local_terms<double>(sp.k1u, sp.k1dir, 0.0, a.an, s, hq);
+        double hz[3];
+        for (int c = 0; c < 3; ++c) {
+            hz[c] = a.B[c];
+            if (a.stg)
+                hz[c] += a.f64 ? ((const double*)a.stg)[(size_t)c * n + i] : (double)((const float*)a.stg)[(size_t)c * n + i];
+        }
+        const double ep = s[0] * hp[0] + s[1] * hp[1] + s[2] * hp[2];
+        const double eq = s[0] * hq[0] + s[1] * hq[1] + s[2] * hq[2];
+        const double ec = sp.kc * (s[0] * s[0] * s[0] * s[0] + s[1] * s[1] * s[1] * s[1] + s[2] * s[2] * s[2] * s[2]);
+        const double ez = s[0] * hz[0] + s[1] * hz[1] + s[2] * hz[2];
+        acc += a.mu[spx] * LLG_MU_B * (-0.5 * ep - 0.5 * eq - 0.25 * ec - ez);
+    }
+    const double r = block_sum(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kTPB) void k_noise_f64(uint32_t n, uint64_t seed, uint64_t stream, uint64_t t, double* out) {
+    const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
+    if (i >= n) return;
+    double z[3];
+    normals3_f64(seed, stream, i, t, z);
+    for (int c = 0; c < 3; ++c) out[(size_t)c * n + i] = z[c];
+}
+
+__global__ __launch_bounds__(kTPB) void k_noise_f32(uint32_t n, uint64_t seed, uint64_t stream, uint64_t t, double* out) {
+    const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
+    if (i >= n) return;
+    float z[3];
+    normals3_f32(seed, stream, i, t, z);
+    for (int c = 0; c < 3; ++c) out[(size_t)c * n + i] = (double)z[c];
+}
+
+template <class SpinT>
+__global__ __launch_bounds__(kTPB) void k_check_finite(uint32_t n3, const SpinT* __restrict__ S, int* flag) {
+    const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
+    if (i >= n3) return;
+    const double v = (double)S[i];
+    if (!(v == v) || v > 1e300 || v < -1e300) atomicOr(flag, 1);
+}
+
+// ---------------------------------------------------------------- host launchers
+
+static int ensure_red(llg_ctx* c) {
+    const size_t need = (size_t)kRedBlocks * (size_t)std::max<uint32_t>(c->nsub * 3, 1) * sizeof(double);
+    if (c->red_part.bytes < need) SLLG_TRY(c->red_part.alloc(need));
+    if (c->red_out.bytes < 64 * sizeof(double) + kMaxSub * sizeof(double) + 64 * sizeof(double))
+        SLLG_TRY(c->red_out.alloc(64 * sizeof(double) + kMaxSub * sizeof(double) + 64 * sizeof(double)));
+    return 0;
+}
+
+static unsigned red_blocks(uint32_t n) { return std::min<unsigned>(kRedBlocks, std::max<unsigned>(1, blocks_for(n, kTPB))); }
+
+int reduce_mag(llg_ctx* c, double* dev_out) {
+    SLLG_TRY(ensure_red(c));
+    const unsigned nb = red_blocks(c->n);
+    const uint8_t* sub = (c->nsub > 1) ? c->sub.as<uint8_t>() : nullptr;
+    // inverse sublattice counts live after the first 64 doubles of red_out
+    double* inv = c->red_out.as<double>() + 64;
+    std::vector<double> hinv(c->nsub);
+    for (uint32_t s = 0; s < c->nsub; ++s) hinv[s] = c->sub_count[s] ? 1. / (double)c->sub_count[s] : 0.0;
+    SLLG_HIP(hipMemcpyAsync(inv, hinv.data(), c->nsub * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (c->prec == LLG_FP64_PLAIN)
+        k_mag_partial<double><<<nb, kTPB, 0, c->stream>>>(c->n, c->nsub, c->S.as<double>(), sub, c->red_part.as<double>());
+    else
+        k_mag_partial<float><<<nb, kTPB, 0, c->stream>>>(c->n, c->nsub, c->S.as<float>(), sub, c->red_part.as<double>());
+    SLLG_HIP(hipGetLastError());
+    k_sum_partials<<<1, kTPB, 0, c->stream>>>(nb, c->nsub * 3, c->red_part.as<double>(), inv, dev_out);
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+int reduce_energy(llg_ctx* c, const void* dem_now, double* dev_out) {
+    SLLG_TRY(ensure_red(c));
+    const unsigned nb = red_blocks(c->n);
+    EnergyArgs a{};
+    a.n = c->n;
+    a.multi = c->multi_species ? 1 : 0;
+    for (int k = 0; k < 3; ++k) a.B[k] = c->B[k];
+    a.f64 = c->prec == LLG_FP64_PLAIN;
+    a.dem = dem_now;
+    a.stg = c->stag_set ? c->STG.p : nullptr;
+    a.spec = c->spec.as<uint8_t>();
+    a.sp = c->sp64d.as<Species<double>>();
+    a.mu = c->red_out.as<double>() + 64 + kMaxSub;
+    SLLG_HIP(hipMemcpyAsync((void*)a.mu, c->mu_sp.data(), c->mu_sp.size() * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+    a.an = c->an64;
+    a.ex = c->ex64;
+    double* part = c->red_part.as<double>();
+    const int fmt = a.f64 ? c->ex64.fmt : c->ex32.fmt;
+    const int nc = c->ex64.ncomp;
+#define SLLG_E(F, N)                                                                                 \
+    do {                                                                                             \
+        if (a.f64)                                                                                   \
+            k_energy_partial<F, N, double><<<nb, kTPB, 0, c->stream>>>(a, c->S.as<double>(), part);  \
+        else                                                                                         \
+            k_energy_partial<F, N, float><<<nb, kTPB, 0, c->stream>>>(a, c->S.as<float>(), part);    \
+    } while (0)
+#define SLLG_E_NC(F)                         \
+    do {                                     \
+        if (nc == 1) SLLG_E(F, 1);           \
+        else if (nc == 3) SLLG_E(F, 3);      \
+        else SLLG_E(F, 9);                   \
+    } while (0)
+    switch (fmt) {
+        case FMT_NONE: SLLG_E(FMT_NONE, 1); break;
+        case FMT_CSR: SLLG_E_NC(FMT_CSR); break;
+        case FMT_DIA: SLLG_E_NC(FMT_DIA); break;
+        case FMT_STENCIL: SLLG_E_NC(FMT_STENCIL); break;
+        case FMT_SELL_PACK: SLLG_E_NC(FMT_SELL_PACK); break;
+        default: SLLG_E_NC(FMT_SELL_RAW); break;
+    }
+#undef SLLG_E_NC
+#undef SLLG_E
+    SLLG_HIP(hipGetLastError());
+    k_sum_partials<<<1, kTPB, 0, c->stream>>>(nb, 1, part, nullptr, dev_out);
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+int noise_sample(llg_ctx* c, uint64_t t, double* dev_out) {
+    if (c->prec == LLG_FP64_PLAIN)
+        k_noise_f64<<<blocks_for(c->n, kTPB), kTPB, 0, c->stream>>>(c->n, c->seed, c->stream_id, t, dev_out);
+    else
+        k_noise_f32<<<blocks_for(c->n, kTPB), kTPB, 0, c->stream>>>(c->n, c->seed, c->stream_id, t, dev_out);
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+int check_finite(llg_ctx* c, int* bad) {
+    SLLG_TRY(ensure_red(c));
+    int* flag = reinterpret_cast<int*>(c->red_out.as<double>() + 63);
+    SLLG_HIP(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
+    const uint32_t n3 = 3 * c->n;
+    if (c->prec == LLG_FP64_PLAIN)
+        k_check_finite<double><<<blocks_for(n3, kTPB), kTPB, 0, c->stream>>>(n3, c->S.as<double>(), flag);
+    else
+        k_check_finite<float><<<blocks_for(n3, kTPB), kTPB, 0, c->stream>>>(n3, c->S.as<float>(), flag);
+    SLLG_HIP(hipGetLastError());
+    SLLG_HIP(hipMemcpyAsync(bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // namespace sllg

@@ -1,0 +1,44 @@
+// kernels.hpp - host launchers of the device kernels (each returns 0 or an llg_status).
+#pragma once
+#include "ctx.hpp"
+
+namespace sllg {
+
+// FP64 plain path (k_plain.hip)
+int plain_spmv(llg_ctx* c, const double* S, double* H);
+int plain_noise(llg_ctx* c, uint64_t t);
+int plain_heun1(llg_ctx* c);
+int plain_heun2(llg_ctx* c);
+int plain_rk_init_tables();
+int plain_rk_stage(llg_ctx* c, int m);
+int plain_rk_comb(llg_ctx* c);
+int plain_detfield(llg_ctx* c, const double* Sp, double* out);
+
+// FP32 fused path (k_fused.hip)
+struct StepSched {
+    uint64_t t;        // step number (counter noise, dipolar refresh)
+    const double* xi;  // injected noise for this step (device, 3n doubles) or null
+};
+int fused_heun_pred(llg_ctx* c, const StepSched& s);
+int fused_heun_corr(llg_ctx* c, const StepSched& s);
+int fused_rk_stage(llg_ctx* c, const StepSched& s, int m);
+int fused_detfield(llg_ctx* c, const float* Sp, double* out);
+int fused_rk_init_tables();
+
+// Reductions (k_reduce.hip): FP64 sublattice magnetisation and energy.
+int reduce_mag(llg_ctx* c, double* dev_out /* nsub*3 */);
+int reduce_energy(llg_ctx* c, const void* dem_now /* dipolar field at the current spins or null */,
+                  double* dev_out /* 1 */);
+int noise_sample(llg_ctx* c, uint64_t t, double* dev_out);
+int check_finite(llg_ctx* c, int* bad);
+
+// Dipolar convolution (k_dipolar.hip)
+int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_species);
+void dipolar_destroy(llg_ctx* c);
+int dipolar_refresh(llg_ctx* c, const void* S, void* out);  // spins of the path's precision -> out (3n)
+
+// Encodings (encode.cpp)
+int build_exchange_fp64(llg_ctx* c, const llg_exchange& e);
+int build_exchange_fp32(llg_ctx* c, const llg_exchange& e);
+
+}  // namespace sllg

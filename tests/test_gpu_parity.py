@@ -1,0 +1,190 @@
+"""GPU parity: the HIP paths against the FP64 oracle, through the C ABI.
+
+Bars (written into each assertion):
+  FP64 plain path  - max |dS| <= 1e-10 per spin component (T = 0, and T > 0 with the same
+                     injected noise); the two are built to give identical bits.
+  FP32 path        - error against the oracle stated per test (float rounding, 2 evaluations
+                     per step).
+"""
+import numpy as np
+import pytest
+
+import sllg
+from sllg import workloads as W
+
+from helpers import run_pair, tensor_csr_system
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+
+
+def _maxdiff(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
+
+
+def test_plain_heun_sc_T0(gpu):
+    sysd, s0, _ = W.sc_ferro(L=12, precision="fp64")
+    g, o = run_pair(sysd, s0, 2000, T=0.0)
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print("plain heun T=0 max|dS| =", d)
+    assert d <= TOL64
+    assert _maxdiff(g.det_field(), o.det_field()) <= 1e-6  # tesla, |H| ~ 1e3
+
+
+def test_plain_heun_sc_injected_T300(gpu):
+    sysd, s0, _ = W.sc_ferro(L=10, precision="fp64", noise="injected")
+    rng = np.random.default_rng(1)
+    xi = rng.standard_normal((300, 3, sysd.n))
+    g, o = run_pair(sysd, s0, 300, T=300.0, xi=xi)
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print("plain heun T=300 injected max|dS| =", d)
+    assert d <= TOL64
+
+
+@pytest.mark.parametrize("fmt,ncomp", [("csr", 3), ("csr", 9), ("dia", 9), ("stencil", 9), ("stencil", 3)])
+def test_plain_heun_formats_full_field(gpu, fmt, ncomp):
+    sysd, s0 = tensor_csr_system(L=6, precision="fp64", ncomp=ncomp, fmt=fmt)
+    rng = np.random.default_rng(2)
+    stag = rng.normal(0, 5.0, (3, sysd.n))
+    g, o = run_pair(sysd, s0, 300, T=0.0, B=(0.3, -0.2, 1.0), stag=stag)
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print(fmt, ncomp, "max|dS| =", d)
+    assert d <= TOL64
+
+
+@pytest.mark.parametrize("scheme", ["rk4", "rk5"])
+def test_plain_rk_parity(gpu, scheme):
+    sysd, s0 = tensor_csr_system(L=6, precision="fp64", ncomp=9, fmt="csr", scheme=scheme)
+    g, o = run_pair(sysd, s0, 300, T=0.0, B=(0.0, 0.0, 0.5))
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print(scheme, "max|dS| =", d)
+    assert d <= TOL64
+
+
+def test_fp32_heun_vs_oracle(gpu):
+    sysd, s0, _ = W.sc_ferro(L=12, precision="fp32")
+    g, o = run_pair(sysd, s0, 200, T=0.0)
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print("fp32 heun 200 steps max|dS| =", d)
+    assert d <= 1e-4
+    assert "pack" in g.info()["exch_encoding"]
+
+
+@pytest.mark.parametrize("fmt,ncomp", [("csr", 9), ("dia", 9), ("stencil", 3)])
+def test_fp32_formats_full_field(gpu, fmt, ncomp):
+    sysd, s0 = tensor_csr_system(L=6, precision="fp32", ncomp=ncomp, fmt=fmt)
+    rng = np.random.default_rng(2)
+    stag = rng.normal(0, 5.0, (3, sysd.n))
+    g, o = run_pair(sysd, s0, 100, T=0.0, B=(0.3, -0.2, 1.0), stag=stag)
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print("fp32", fmt, ncomp, "max|dS| =", d)
+    assert d <= 1e-4
+    h = g.field()
+    ho = o.field()
+    assert _maxdiff(h, ho) <= 1e-4 * np.abs(ho).max()
+
+
+@pytest.mark.parametrize("scheme", ["rk4", "rk5"])
+def test_fp32_rk_vs_oracle(gpu, scheme):
+    sysd, s0 = tensor_csr_system(L=6, precision="fp32", ncomp=9, fmt="csr", scheme=scheme)
+    g, o = run_pair(sysd, s0, 100, T=0.0, B=(0.0, 0.0, 0.5))
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print("fp32", scheme, "max|dS| =", d)
+    assert d <= 1e-4
+
+
+def test_fp32_injected_T300(gpu):
+    sysd, s0, _ = W.sc_ferro(L=8, precision="fp32", noise="injected")
+    xi = np.random.default_rng(3).standard_normal((50, 3, sysd.n))
+    g, o = run_pair(sysd, s0, 50, T=300.0, xi=xi)
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print("fp32 injected T=300 50 steps max|dS| =", d)
+    assert d <= 1e-4
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-5)])
+def test_counter_noise_matches_oracle(gpu, prec, tol):
+    sysd, s0, _ = W.sc_ferro(L=8, precision=prec, seed=1234)
+    sysd.stream = 77
+    import pyoracle
+
+    g = sllg.Context(sysd)
+    o = pyoracle.OracleContext(sysd)
+    for t in (0, 1, 123456789012):
+        a, b = g.noise_sample(t), o.noise_sample(t)
+        d = _maxdiff(a, b)
+        print(prec, t, d)
+        assert d <= tol * max(1.0, np.abs(b).max())
+
+
+def test_counter_noise_trajectory_fp64(gpu):
+    """Counter mode: GPU FP64 plain vs oracle at T = 300 K; the noise differs only by libm
+    rounding (1e-16 relative), so the trajectories agree to 1e-9 over 100 steps."""
+    sysd, s0, _ = W.sc_ferro(L=8, precision="fp64")
+    g, o = run_pair(sysd, s0, 100, T=300.0)
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print("counter T=300 100 steps max|dS| =", d)
+    assert d <= 1e-9
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_reduce_mag_energy(gpu, prec):
+    sysd, s0 = tensor_csr_system(L=6, precision=prec, ncomp=9, fmt="csr")
+    rng = np.random.default_rng(2)
+    stag = rng.normal(0, 5.0, (3, sysd.n))
+    g, o = run_pair(sysd, s0, 0, T=0.0, B=(0.3, -0.2, 1.0), stag=stag)
+    mg, eg = g.reduce()
+    mo, eo = o.reduce()
+    tol = 1e-12 if prec == "fp64" else 1e-6
+    print(prec, mg, mo, eg, eo)
+    assert _maxdiff(mg, mo) <= tol
+    assert abs(eg - eo) <= tol * abs(eo) * 10
+
+
+def test_run_sampled_matches_oracle(gpu):
+    sysd, s0, _ = W.sc_ferro(L=8, precision="fp64")
+    import pyoracle
+
+    g = sllg.Context(sysd)
+    o = pyoracle.OracleContext(sysd)
+    for c in (g, o):
+        c.set_spins(s0)
+        c.set_temperature(0.0)
+    a = g.run_sampled(0, 250, 50)
+    b = o.run_sampled(0, 250, 50)
+    assert a.shape == (5, 1, 3)
+    assert _maxdiff(a, b) <= 1e-12
+
+
+def test_dipolar_matches_bruteforce(gpu):
+    for prec, tol in (("fp64", 1e-9), ("fp32", 2e-5)):
+        sysd, s0 = W.sc_film_dip(Lx=8, Ly=6, Lz=4, precision=prec)
+        import pyoracle
+
+        g = sllg.Context(sysd)
+        o = pyoracle.OracleContext(sysd)
+        for c in (g, o):
+            c.set_spins(s0)
+        g.step(0, 1)  # refresh at t = 0 computes the dipolar field of s0
+        o.step(0, 1)
+        d = _maxdiff(g.get_spins(), o.get_spins())
+        print(prec, "dipolar step max|dS| =", d)
+        assert d <= (1e-10 if prec == "fp64" else 1e-5)
+        # field at the current spins includes the stale dipolar field of s0 on both sides
+        hg, ho = g.field(), o.field()
+        rel = _maxdiff(hg, ho) / np.abs(ho).max()
+        print(prec, "field rel err", rel)
+        assert rel <= tol
+
+
+def test_invalid_descriptor_rejected(gpu):
+    sysd, s0, _ = W.sc_ferro(L=4, precision="fp32")
+    bad = dict(sysd.exch)
+    adj = bad["adjncy"].copy()
+    adj[5] = sysd.n + 3
+    bad["adjncy"] = adj
+    sysd.exch = bad
+    with pytest.raises(sllg.LLGError) as e:
+        sllg.Context(sysd)
+    assert e.value.code == -1
