@@ -22,6 +22,7 @@ namespace sllg {
 constexpr int kTPB = 256;       // threads per block of the per-spin kernels
 constexpr int kSlice = 64;      // sliced-ELL slice height = one wavefront
 constexpr int kMaxSub = 16;     // sublattices reduced on the device
+constexpr int kDictLds = 2048;  // floats of coupling dictionary staged in LDS (8 KB)
 
 enum ExchFmt : int {
     FMT_NONE = 0,

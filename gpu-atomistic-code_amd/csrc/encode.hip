@@ -5,11 +5,11 @@
 // DIA [ncomp][ndiag][n]; periodic stencil).
 //
 // FP32 path: CSR becomes sliced ELL with 64-row slices (one wavefront) so that every
-// load of the coupling stream is a contiguous 256-byte wave access.  When every column
+// load of the coupling stream is a contiguous wave access (1 KB per uint4 load).  When every column
 // lies within +-32767 of its row (modulo n, i.e. across the periodic wrap) and the
-// distinct coupling tuples number at most 65536, an entry is one u32: int16 column
-// delta << 16 | u16 index into a coupling dictionary (M_CSR = 4 B per nnz, the
-// dictionary is a few KB and stays in cache).  Otherwise entries are a u32 column plus
+// distinct coupling tuples fit the LDS dictionary (kDictLds floats), an entry is one u32:
+// int16 column delta << 16 | u16 index into the coupling dictionary (M_CSR = 4 B per
+// nnz; the dictionary is staged in LDS by every block).  Otherwise entries are a u32 column plus
 // ncomp FP32 values (M = 4 + 4 ncomp B per nnz).  Rows keep their stored order, so the
 // per-row summation order of the reference CSR (src/cufields.cu:192-200) is kept.
 #include <algorithm>
@@ -162,26 +162,21 @@ struct TupleHash {
     }
 };
 
-// CSR -> sliced ELL (packed or raw).
+// CSR -> sliced ELL.  Packed layout: per slice, entries in chunks of 4 per lane,
+// [chunk][lane][4], so one uint4 load per lane reads 4 entries and a wave reads 1 KB
+// contiguous; slice lengths are rounded up to a multiple of 4 with zero-coupling entries
+// (delta 0, dictionary tuple 0.0), which add exactly 0 to the field.  Raw layout:
+// [entry][lane] u32 columns and [ncomp][entry][lane] FP32 values.
 int build_sell(llg_ctx* c) {
     const HostExch& h = c->hex;
     const uint32_t n = c->n;
     const int nc = h.ncomp;
     const uint32_t nsl = (n + kSlice - 1) / kSlice;
-    std::vector<uint32_t> slice_off(nsl + 1, 0);
-    for (uint32_t s = 0; s < nsl; ++s) {
-        uint32_t mx = 0;
-        for (uint32_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r) mx = std::max(mx, h.xadj[r + 1] - h.xadj[r]);
-        slice_off[s + 1] = slice_off[s] + mx;
-    }
-    const uint64_t nent = (uint64_t)slice_off[nsl] * kSlice;
-    bool padded = false;
-    for (uint32_t r = 0; r < n && !padded; ++r) {
-        const uint32_t s = r / kSlice;
-        if (h.xadj[r + 1] - h.xadj[r] != slice_off[s + 1] - slice_off[s]) padded = true;
-    }
-    if (n % kSlice) padded = true;
-    // Try the packed form.
+    std::vector<uint32_t> mx(nsl, 0);
+    for (uint32_t s = 0; s < nsl; ++s)
+        for (uint32_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r)
+            mx[s] = std::max(mx[s], h.xadj[r + 1] - h.xadj[r]);
+    // Packed form: every column within +-32767 of its row modulo n, few distinct tuples.
     bool pack = true;
     const int64_t half = (int64_t)n / 2;
     std::vector<int16_t> delta(h.nnz);
@@ -197,6 +192,10 @@ int build_sell(llg_ctx* c) {
     std::vector<TupleKey> tuples;
     std::vector<uint32_t> idx;
     if (pack) {
+        TupleKey z{};
+        z.nc = nc;
+        dict.emplace(z, 0u);  // tuple 0 is the zero coupling used for padding
+        tuples.push_back(z);
         idx.resize(h.nnz);
         TupleKey last{};
         last.nc = -1;
@@ -210,7 +209,7 @@ int build_sell(llg_ctx* c) {
             uint32_t id;
             if (it == dict.end()) {
                 id = (uint32_t)tuples.size();
-                if (id >= 65535u) { pack = false; break; }
+                if ((uint64_t)(id + 1) * nc > (uint64_t)kDictLds) { pack = false; break; }
                 dict.emplace(k, id);
                 tuples.push_back(k);
             } else {
@@ -221,30 +220,25 @@ int build_sell(llg_ctx* c) {
             last_id = id;
         }
     }
+    std::vector<uint32_t> slice_off(nsl + 1, 0);
+    for (uint32_t s = 0; s < nsl; ++s) slice_off[s + 1] = slice_off[s] + (pack ? (mx[s] + 3u) / 4u * 4u : mx[s]);
+    const uint64_t nent = (uint64_t)slice_off[nsl] * kSlice;
     Exch<float>& x = c->ex32;
     Exch<double>& x64 = c->ex64;
     SLLG_TRY(upload(c->ex_a, slice_off.data(), slice_off.size(), c->stream));
     std::vector<uint32_t> ent(nent, 0);
     if (pack) {
-        uint32_t zero_id = 0;
-        if (padded) {
-            TupleKey z{};
-            z.nc = nc;
-            auto it = dict.find(z);
-            if (it == dict.end()) { zero_id = (uint32_t)tuples.size(); tuples.push_back(z); }
-            else zero_id = it->second;
-        }
         for (uint32_t s = 0; s < nsl; ++s) {
             const uint32_t len = slice_off[s + 1] - slice_off[s];
             for (uint32_t l = 0; l < kSlice; ++l) {
                 const uint32_t r = s * kSlice + l;
                 for (uint32_t q = 0; q < len; ++q) {
-                    uint32_t w = zero_id;
+                    uint32_t w = 0;  // delta 0, tuple 0
                     if (r < n && q < h.xadj[r + 1] - h.xadj[r]) {
                         const uint32_t qq = h.xadj[r] + q;
                         w = ((uint32_t)(uint16_t)delta[qq] << 16) | idx[qq];
                     }
-                    ent[((size_t)slice_off[s] + q) * kSlice + l] = w;
+                    ent[(size_t)slice_off[s] * kSlice + (size_t)(q >> 2) * (kSlice * 4) + l * 4 + (q & 3)] = w;
                 }
             }
         }
@@ -261,7 +255,7 @@ int build_sell(llg_ctx* c) {
         x.fmt = FMT_SELL_PACK;
         x.ndict = (uint32_t)tuples.size();
         char buf[48];
-        std::snprintf(buf, sizeof buf, "sell64-pack16x16-dict%u", x.ndict);
+        std::snprintf(buf, sizeof buf, "sell64x4-pack16x16-dict%u", x.ndict);
         c->exch_encoding = buf;
     } else {
         std::vector<float> v32((size_t)nc * nent, 0.0f);

@@ -17,11 +17,11 @@
 
 namespace sllg {
 
-// Sliced ELL row: slice = 64 consecutive rows (one wavefront), entries of slice-local
-// column q of all 64 rows are contiguous, so each load instruction reads 256 bytes.
-// Packed entry: high 16 bits = column delta (j - i, wrapped mod n into int16), low 16
-// bits = index into the coupling dictionary [ndict][NC].  Raw entry: u32 column, values
-// [NC][nent] in the same slice layout.
+// Sliced ELL, scalar reader (any precision; dictionary read from global memory).  Slice
+// = 64 consecutive rows (one wavefront).  Packed entry: high 16 bits = column delta
+// (j - i, wrapped mod n into int16), low 16 bits = index into the coupling dictionary
+// [ndict][NC]; entries of a slice are stored [chunk of 4][lane][4].  Raw entry: u32
+// column, values [NC][nent], both [entry][lane].
 template <int NC, class R, class V, class SpinT>
 __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, const SpinT* __restrict__ S, R dot[3]) {
     const uint32_t n = e.n;
@@ -29,12 +29,11 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
     const uint32_t sl = i >> 6, lane = i & 63u;
     const uint32_t q0 = e.slice_off[sl];
     const uint32_t len = e.slice_off[sl + 1] - q0;
-    const uint32_t* __restrict__ ent = e.ent + (size_t)q0 * kSlice + lane;
     R v[NC];
     if (e.fmt == FMT_SELL_PACK) {
-#pragma unroll 6
+        const uint32_t* __restrict__ ent = e.ent + (size_t)q0 * kSlice + lane * 4;
         for (uint32_t q = 0; q < len; ++q) {
-            const uint32_t w = ent[(size_t)q * kSlice];
+            const uint32_t w = ent[(size_t)(q >> 2) * (kSlice * 4) + (q & 3)];
             int32_t j = (int32_t)i + (int32_t)(int16_t)(w >> 16);
             j += (j < 0) ? (int32_t)n : 0;
             j -= (j >= (int32_t)n) ? (int32_t)n : 0;
@@ -45,6 +44,7 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
             accum<NC, R>(v, s, dot);
         }
     } else {
+        const uint32_t* __restrict__ ent = e.ent + (size_t)q0 * kSlice + lane;
         const V* __restrict__ val = e.dict + (size_t)q0 * kSlice + lane;
 #pragma unroll 4
         for (uint32_t q = 0; q < len; ++q) {
@@ -54,6 +54,45 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
             const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
             accum<NC, R>(v, s, dot);
         }
+    }
+}
+
+// One packed entry: decode, gather the neighbour, read its coupling from the LDS dictionary.
+template <int NC>
+__device__ __forceinline__ void pack_entry(uint32_t w, uint32_t i, uint32_t n, const float* __restrict__ S,
+                                           const float* sdict, float dot[3]) {
+    int32_t j = (int32_t)i + (int32_t)(int16_t)(w >> 16);
+    j += (j < 0) ? (int32_t)n : 0;
+    j -= (j >= (int32_t)n) ? (int32_t)n : 0;
+    const float s[3] = {S[j], S[n + j], S[2 * (size_t)n + j]};
+    float v[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) v[k] = sdict[(w & 0xFFFFu) * NC + k];
+    accum<NC, float>(v, s, dot);
+}
+
+// Fast packed reader of the fused kernels: one uint4 (4 entries) per lane per load, the
+// next chunk prefetched while the current one's neighbours are gathered, couplings from
+// the block's LDS copy of the dictionary.
+template <int NC>
+__device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_t i, const float* __restrict__ S,
+                                                   const float* sdict, float dot[3]) {
+    const uint32_t n = e.n;
+    dot[0] = dot[1] = dot[2] = 0.0f;
+    const uint32_t sl = i >> 6, lane = i & 63u;
+    const uint32_t q0 = e.slice_off[sl];
+    const uint32_t nch = (e.slice_off[sl + 1] - q0) >> 2;
+    if (nch == 0) return;
+    const uint4* __restrict__ p = reinterpret_cast<const uint4*>(e.ent + (size_t)q0 * kSlice) + lane;
+    uint4 cur = p[0];
+#pragma unroll 2
+    for (uint32_t c = 0; c < nch; ++c) {
+        const uint4 nxt = (c + 1 < nch) ? p[(size_t)(c + 1) * kSlice] : cur;
+        pack_entry<NC>(cur.x, i, n, S, sdict, dot);
+        pack_entry<NC>(cur.y, i, n, S, sdict, dot);
+        pack_entry<NC>(cur.z, i, n, S, sdict, dot);
+        pack_entry<NC>(cur.w, i, n, S, sdict, dot);
+        cur = nxt;
     }
 }
 
@@ -97,14 +136,17 @@ __device__ __forceinline__ uint32_t tile_index(int xcd) {
 }
 
 template <int FMT, int NC>
-__device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const float* __restrict__ src, float s[3],
-                                          float f[3]) {
+__device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const float* __restrict__ src,
+                                          const float* sdict, float s[3], float f[3]) {
     const uint32_t n = a.n;
     s[0] = src[i];
     s[1] = src[n + i];
     s[2] = src[2 * (size_t)n + i];
     float h[3];
-    exch_any<FMT, NC, float, float, float>(a.ex, i, src, h);
+    if constexpr (FMT == FMT_SELL_PACK)
+        exch_row_pack_fast<NC>(a.ex, i, src, sdict, h);
+    else
+        exch_any<FMT, NC, float, float, float>(a.ex, i, src, h);
     if (a.dem) {
         h[0] += a.dem[i];
         h[1] += a.dem[n + i];
@@ -136,14 +178,24 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const 
     llg_rhs<float>(sp.llgpf, sp.lambda, s, h, f);
 }
 
+// Stage the coupling dictionary of the packed encoding in LDS (before any early exit).
+#define SLLG_STAGE_DICT(FMT_, NC_, a_)                                              \
+    __shared__ float sdict[FMT_ == FMT_SELL_PACK ? kDictLds : 1];                 \
+    if constexpr (FMT_ == FMT_SELL_PACK) {                                        \
+        const uint32_t m_ = (a_).ex.ndict * NC_;                                  \
+        for (uint32_t k_ = threadIdx.x; k_ < m_; k_ += kTPB) sdict[k_] = (a_).ex.dict[k_]; \
+        __syncthreads();                                                          \
+    }
+
 template <int FMT, int NC>
 __global__ __launch_bounds__(kTPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S, float* __restrict__ Y,
                                                     float* __restrict__ A) {
+    SLLG_STAGE_DICT(FMT, NC, a)
     const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, S, s, f);
+    fused_rhs<FMT, NC>(a, i, S, sdict, s, f);
     float y[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -158,11 +210,12 @@ __global__ __launch_bounds__(kTPB) void k_heun_pred(FusedArgs a, const float* __
 template <int FMT, int NC>
 __global__ __launch_bounds__(kTPB) void k_heun_corr(FusedArgs a, const float* __restrict__ Y,
                                                     const float* __restrict__ A, float* __restrict__ S) {
+    SLLG_STAGE_DICT(FMT, NC, a)
     const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, Y, s, f);
+    fused_rhs<FMT, NC>(a, i, Y, sdict, s, f);
     float p[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = A[(size_t)c * n + i] + a.half_rdt * f[c];
@@ -177,11 +230,12 @@ __global__ __launch_bounds__(kTPB) void k_heun_corr(FusedArgs a, const float* __
 template <int FMT, int NC, int M>
 __global__ __launch_bounds__(kTPB) void k_rk4(FusedArgs a, const float* __restrict__ Yin, float* __restrict__ Yout,
                                               float* __restrict__ ACC, float* __restrict__ S) {
+    SLLG_STAGE_DICT(FMT, NC, a)
     const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, M == 0 ? S : Yin, s, f);
+    fused_rhs<FMT, NC>(a, i, M == 0 ? S : Yin, sdict, s, f);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const size_t o = (size_t)c * n + i;
@@ -218,11 +272,12 @@ template <int FMT, int NC, int M>
 __global__ __launch_bounds__(kTPB) void k_rk5(FusedArgs a, const float* __restrict__ Yin, float* __restrict__ Yout,
                                               float* __restrict__ S, float* __restrict__ K0, float* __restrict__ K1,
                                               float* __restrict__ K2, float* __restrict__ K3, float* __restrict__ K4) {
+    SLLG_STAGE_DICT(FMT, NC, a)
     const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, M == 0 ? S : Yin, s, f);
+    fused_rhs<FMT, NC>(a, i, M == 0 ? S : Yin, sdict, s, f);
     float* Ks[5] = {K0, K1, K2, K3, K4};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {

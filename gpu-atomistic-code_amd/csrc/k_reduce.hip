@@ -103,11 +103,13 @@ __device__ __forceinline__ void exch_row_sell64(const Exch<double>& e, uint32_t 
     const uint32_t sl = i >> 6, lane = i & 63u;
     const uint32_t q0 = e.slice_off[sl];
     const uint32_t len = e.slice_off[sl + 1] - q0;
-    const uint32_t* ent = e.ent + (size_t)q0 * kSlice + lane;
     double v[NC];
     for (uint32_t q = 0; q < len; ++q) {
-        const uint32_t w = ent[(size_t)q * kSlice];
-        uint32_t j;
+        uint32_t w, j;
+        if (e.fmt == FMT_SELL_PACK)
+            w = e.ent[(size_t)q0 * kSlice + (size_t)(q >> 2) * (kSlice * 4) + lane * 4 + (q & 3)];
+        else
+            w = e.ent[((size_t)q0 + q) * kSlice + lane];
         if (e.fmt == FMT_SELL_PACK) {
             int32_t jj = (int32_t)i + (int32_t)(int16_t)(w >> 16);
             jj += (jj < 0) ? (int32_t)n : 0;

@@ -1,0 +1,218 @@
+"""CPU tests of the host-side builders and of the C ABI surface (no GPU compute)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import pyoracle
+import sllg
+from sllg import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ----------------------------------------------------------------------------- lattices
+
+@pytest.mark.parametrize("kind,counts", [("sc", [6, 12, 8]), ("bcc", [8, 6, 12]), ("fcc", [12, 6, 24]),
+                                         ("bct", [8, 4, 2])])
+def test_shell_counts(kind, counts):
+    uc = sllg.unit_cell(kind, 1.2)
+    ints, dist = sllg.shells(uc, [1.0, 2.0, 3.0])
+    na = uc.n_atoms
+    for a in range(na):
+        per = [sum(1 for it in ints if it.atom_i == a and it.shell == s) for s in range(3)]
+        assert per == counts, (kind, a, per, dist)
+
+
+def _bruteforce_neighbours(uc, dims, pbc, rmax):
+    """Neighbour sets by distance with explicit periodic images (numpy, O(n^2))."""
+    na = uc.n_atoms
+    cells = np.stack(np.unravel_index(np.arange(int(np.prod(dims))), dims), axis=1)
+    frac = (cells[:, None, :] + np.asarray(uc.xred)[None]).reshape(-1, 3)
+    n = len(frac)
+    out = []
+    D = np.asarray(dims, float)
+    for i in range(n):
+        d = frac - frac[i]
+        for a in range(3):
+            if pbc[a]:
+                d[:, a] -= D[a] * np.round(d[:, a] / D[a])
+        r = np.linalg.norm(d @ np.asarray(uc.rprim), axis=1)
+        out.append(set(np.nonzero((r > 1e-9) & (r < rmax))[0].tolist()))
+    return out
+
+
+@pytest.mark.parametrize("kind,dims,pbc", [("sc", (5, 6, 7), (1, 1, 1)), ("fcc", (5, 5, 5), (1, 1, 1)),
+                                           ("bcc", (6, 5, 5), (1, 0, 1)), ("bct", (5, 5, 6), (1, 1, 0))])
+def test_csr_builder_against_oracle_and_bruteforce(kind, dims, pbc):
+    uc = sllg.unit_cell(kind, 1.2)
+    ints, dist = sllg.shells(uc, [1.0, 0.5])
+    mu = [2.22] * uc.n_atoms
+    xa, adj, v = sllg.build_csr(uc, dims, ints, mu, ncomp=9, pbc=pbc)
+    xo, ao, vo = pyoracle.build_csr(uc, dims, ints, mu, ncomp=9, pbc=pbc)
+    assert np.array_equal(xa, xo) and np.array_equal(adj, ao) and np.array_equal(v, vo)
+    nb = _bruteforce_neighbours(uc, dims, pbc, dist[1] + 1e-6)
+    for i in range(len(xa) - 1):
+        assert set(adj[xa[i]:xa[i + 1]].tolist()) == nb[i]
+        assert len(adj[xa[i]:xa[i + 1]]) == len(nb[i])  # no duplicates
+    # value = J / (mu mu_B) on the diagonal, 0 off it
+    J0 = v[0] * 2.22 * sllg.MU_B
+    assert set(np.round(J0 / 0.5, 9).tolist()) <= {1.0, 2.0}
+    assert np.abs(v[[1, 2, 3, 5, 6, 7]]).max() == 0.0
+
+
+def test_stencil_and_dia_equal_csr_field():
+    uc = sllg.unit_cell("fcc")
+    ints, _ = sllg.shells(uc, [5e-21, 1e-21, -4e-22])
+    dims = (4, 4, 4)
+    xadj, adj, vals = sllg.build_csr(uc, dims, ints, [2.22] * 4, ncomp=3)
+    cnt, ent, sv = sllg.build_stencil(uc, ints, [2.22] * 4, ncomp=3)
+    n = 256
+    offs, dv = sllg.csr_to_dia(n, xadj, adj, vals)
+    s = sllg.random_spins(n, 3, 4)
+    fields = []
+    for exch in (dict(format=sllg.EXCH_CSR, xadj=xadj, adjncy=adj, vals=vals),
+                 dict(format=sllg.EXCH_STENCIL, count=cnt, entries=ent, vals=sv, dims=dims, per_row=0),
+                 dict(format=sllg.EXCH_DIA, offsets=offs, vals=dv)):
+        sysd = sllg.System(n=n, species=[dict(mu=2.22)], exch=exch, precision="fp64")
+        o = pyoracle.OracleContext(sysd)
+        o.set_spins(s)
+        fields.append(o.field())
+    assert np.array_equal(fields[0], fields[1])  # same per-row order: bit-identical
+    assert np.abs(fields[0] - fields[2]).max() < 1e-12 * np.abs(fields[0]).max()
+
+
+def test_mesh_coordinates():
+    uc = sllg.unit_cell("fcc")
+    m = sllg.mesh(uc, (2, 3, 4))
+    assert m.shape == (96, 3)
+    assert m.max(axis=0).tolist() == [3, 5, 7]
+    assert len({tuple(r) for r in m}) == 96
+
+
+def test_workload_sizes():
+    sysd, s, T = W.sc_ferro(L=32, precision="fp64")
+    assert sysd.n == 32768 and len(sysd.exch["adjncy"]) == 6 * 32768
+    assert abs(sysd.exch["vals"][0][0] - 3e-21 / (2.22 * sllg.MU_B)) < 1e-12
+    sysd, s = W.fcc_3shell(L=8)
+    assert sysd.n == 2048 and len(sysd.exch["adjncy"]) == 42 * 2048
+    sysd, s = W.bct_afm_dmi(Lx=4, Ly=4, Lz=2)
+    v = sysd.exch["vals"]  # [9][k][n]
+    assert v.shape == (9, 8, 64)
+    assert np.abs(np.linalg.norm(s, axis=0) - 1).max() < 1e-12
+    # J_ji = J_ij^T: the field of a Neel pair is reciprocal - check via the dense matrix
+    import pyoracle as po
+    o = po.OracleContext(sysd)
+    n = sysd.n
+    dense = np.zeros((3 * n, 3 * n))
+    for j in range(n):
+        for a in range(3):
+            e = np.zeros((3, n))
+            e[a, j] = 1.0
+            o.set_spins(e + 1e-300)
+            dense[:, 3 * j + a] = o.field().T.reshape(-1)
+    # subtract the on-site (anisotropy) blocks, which are symmetric on their own
+    assert np.allclose(dense, dense.T, atol=1e-9 * np.abs(dense).max())
+
+
+# ----------------------------------------------------------------------------- ABI
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"\b(llg_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = sllg.lib()
+    decl = _declared("llg_capi.h") + _declared("llg_lattice.h")
+    assert len(decl) > 25
+    missing = [s for s in decl if not hasattr(L, s)]
+    assert not missing, missing
+    assert L.llg_abi_version() == sllg.ABI_VERSION
+
+
+def test_struct_layout_matches_c():
+    """sizeof/offsetof of the ABI structs from the C compiler equal the ctypes mirror."""
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "llg_capi.h"
+#include "llg_lattice.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(llg_desc), sizeof(llg_species), sizeof(llg_anisotropy),
+         sizeof(llg_exchange), sizeof(llg_dipolar), sizeof(llg_info), sizeof(llg_unitcell), sizeof(llg_interaction));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", offsetof(llg_desc, anis), offsetof(llg_desc, exch), offsetof(llg_desc, dip),
+         offsetof(llg_desc, dt), offsetof(llg_desc, seed), offsetof(llg_desc, device), offsetof(llg_exchange, stencil_vals),
+         offsetof(llg_dipolar, mesh));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "l")
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        a, b = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")[:2]
+    sizes = [int(x) for x in a.split()]
+    offs = [int(x) for x in b.split()]
+    assert sizes == [C.sizeof(t) for t in (sllg.Desc, sllg.Species, sllg.Anisotropy, sllg.Exchange, sllg.Dipolar,
+                                             sllg.Info, sllg.UnitCell, sllg.Interaction)]
+    assert offs == [sllg.Desc.anis.offset, sllg.Desc.exch.offset, sllg.Desc.dip.offset, sllg.Desc.dt.offset,
+                    sllg.Desc.seed.offset, sllg.Desc.device.offset, sllg.Exchange.stencil_vals.offset,
+                    sllg.Dipolar.mesh.offset]
+
+
+def _create_rc(sysd):
+    L = sllg.lib()
+    d = sysd.desc()
+    h = C.c_void_p()
+    rc = L.llg_create(C.byref(d), C.byref(h))
+    if rc == 0:
+        L.llg_destroy(h)
+    return rc, L.llg_last_error().decode()
+
+
+def test_descriptor_validation_without_device():
+    """Malformed descriptors are rejected with LLG_E_INVALID before any device work."""
+    sysd, _, _ = W.sc_ferro(L=4, precision="fp32")
+    bad = []
+    s1 = W.sc_ferro(L=4)[0]; s1.dt = -1.0; bad.append(s1)
+    s2 = W.sc_ferro(L=4)[0]; s2.update = 0; bad.append(s2)
+    s3 = W.sc_ferro(L=4)[0]; s3.species = [dict(mu=-1.0)]; bad.append(s3)
+    s4 = W.sc_ferro(L=4)[0]; s4.species = [dict(mu=2.22, k1u=1e-23, k1u_dir=(1.0, 1.0, 0.0))]; bad.append(s4)
+    s5 = W.sc_ferro(L=4)[0]; s5.scheme = 7; bad.append(s5)
+    s6 = W.sc_ferro(L=4)[0]; s6.species = [dict(mu=2.22, sublattice=3)]; bad.append(s6)
+    for s in bad:
+        rc, msg = _create_rc(s)
+        assert rc == -1, (rc, msg)
+        assert msg
+
+
+def test_create_without_gpu_reports_nodevice():
+    if sllg.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    sysd, _, _ = W.sc_ferro(L=4, precision="fp32")
+    rc, msg = _create_rc(sysd)
+    assert rc == -5 and "device" in msg.lower()
+
+
+def test_lattice_builders_reject_bad_input():
+    L = sllg.lib()
+    cnt = C.c_uint32(0)
+    assert L.llg_lattice_shells(None, 1, None, None, 0, C.byref(cnt), None) == -1
+    assert L.llg_random_spins(0, 1, 4, None, None, None) == -1
+    uc = sllg.unit_cell("sc")
+    ints, _ = sllg.shells(uc, [1.0])
+    ucc = uc.c()
+    dims = np.array([4, 4, 4], np.uint32)
+    pbc = np.array([1, 1, 1], np.uint32)
+    mu = np.array([2.22])
+    nnz = C.c_uint64(0)
+    up = C.POINTER(C.c_uint32)
+    assert L.llg_lattice_build_csr(C.byref(ucc), dims.ctypes.data_as(up), pbc.ctypes.data_as(up), len(ints), ints,
+                                   mu.ctypes.data_as(C.POINTER(C.c_double)), 2, None, None, None,
+                                   C.byref(nnz)) == -1  # ncomp 2 is not a layout
