@@ -74,45 +74,47 @@ struct Scope {
 
 size_t prec_bytes(const llg_ctx* c) { return c->prec == LLG_FP64_PLAIN ? sizeof(double) : sizeof(float); }
 
-// Upload a host SoA double[3n] into a state buffer of the path's precision.
+// Upload a host SoA double[3n] (reference spin order) into a state buffer of the path's
+// precision, in the device order.
 int upload_state(llg_ctx* c, DevBuf& dst, const double* x, const double* y, const double* z) {
     const uint32_t n = c->n;
+    const double* src[3] = {x, y, z};
     if (c->prec == LLG_FP64_PLAIN) {
-        SLLG_HIP(hipMemcpyAsync(dst.as<double>(), x, n * 8, hipMemcpyHostToDevice, c->stream));
-        SLLG_HIP(hipMemcpyAsync(dst.as<double>() + n, y, n * 8, hipMemcpyHostToDevice, c->stream));
-        SLLG_HIP(hipMemcpyAsync(dst.as<double>() + 2 * (size_t)n, z, n * 8, hipMemcpyHostToDevice, c->stream));
+        for (int a = 0; a < 3; ++a)
+            SLLG_HIP(hipMemcpyAsync(dst.as<double>() + (size_t)a * n, src[a], n * 8, hipMemcpyHostToDevice, c->stream));
     } else {
         std::vector<float> t(3 * (size_t)n);
         for (uint32_t i = 0; i < n; ++i) {
-            t[i] = (float)x[i];
-            t[n + i] = (float)y[i];
-            t[2 * (size_t)n + i] = (float)z[i];
+            const uint32_t d = c->perm.to_dev(i);
+            for (int a = 0; a < 3; ++a) t[(size_t)a * n + d] = (float)src[a][i];
         }
         SLLG_HIP(hipMemcpyAsync(dst.p, t.data(), t.size() * 4, hipMemcpyHostToDevice, c->stream));
-        SLLG_HIP(hipStreamSynchronize(c->stream));
-        return 0;
     }
     SLLG_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
 
+// Download a device SoA vector (FP64 or FP32, device order) into host SoA doubles in the
+// reference spin order.
 int download_state(llg_ctx* c, const void* src_dev, bool src_f64, double* x, double* y, double* z) {
     const uint32_t n = c->n;
-    if (src_f64) {
+    double* dst[3] = {x, y, z};
+    if (src_f64 && !c->permuted()) {
         const double* s = (const double*)src_dev;
-        SLLG_HIP(hipMemcpyAsync(x, s, n * 8, hipMemcpyDeviceToHost, c->stream));
-        SLLG_HIP(hipMemcpyAsync(y, s + n, n * 8, hipMemcpyDeviceToHost, c->stream));
-        SLLG_HIP(hipMemcpyAsync(z, s + 2 * (size_t)n, n * 8, hipMemcpyDeviceToHost, c->stream));
+        for (int a = 0; a < 3; ++a)
+            SLLG_HIP(hipMemcpyAsync(dst[a], s + (size_t)a * n, n * 8, hipMemcpyDeviceToHost, c->stream));
         SLLG_HIP(hipStreamSynchronize(c->stream));
-    } else {
-        std::vector<float> t(3 * (size_t)n);
-        SLLG_HIP(hipMemcpyAsync(t.data(), src_dev, t.size() * 4, hipMemcpyDeviceToHost, c->stream));
-        SLLG_HIP(hipStreamSynchronize(c->stream));
-        for (uint32_t i = 0; i < n; ++i) {
-            x[i] = t[i];
-            y[i] = t[n + i];
-            z[i] = t[2 * (size_t)n + i];
-        }
+        return 0;
+    }
+    const size_t eb = src_f64 ? 8 : 4;
+    std::vector<unsigned char> raw(3 * (size_t)n * eb);
+    SLLG_HIP(hipMemcpyAsync(raw.data(), src_dev, raw.size(), hipMemcpyDeviceToHost, c->stream));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    const double* r64 = reinterpret_cast<const double*>(raw.data());
+    const float* r32 = reinterpret_cast<const float*>(raw.data());
+    for (uint32_t i = 0; i < n; ++i) {
+        const size_t d = c->perm.to_dev(i);
+        for (int a = 0; a < 3; ++a) dst[a][i] = src_f64 ? r64[(size_t)a * n + d] : (double)r32[(size_t)a * n + d];
     }
     return 0;
 }
@@ -144,6 +146,15 @@ int take_noise(llg_ctx* c, const double** dev_xi) {
     if (c->noise_q.empty()) SLLG_FAIL(LLG_E_STATE, "injected noise exhausted: push more with llg_push_noise");
     std::vector<double> v = std::move(c->noise_q.front());
     c->noise_q.pop_front();
+    if (c->permuted()) {  // [3][n] reference order -> device order
+        const uint32_t n = c->n;
+        std::vector<double> w(v.size());
+        for (uint32_t i = 0; i < n; ++i) {
+            const size_t d = c->perm.to_dev(i);
+            for (int a = 0; a < 3; ++a) w[(size_t)a * n + d] = v[(size_t)a * n + i];
+        }
+        v.swap(w);
+    }
     SLLG_HIP(hipMemcpyAsync(c->XI.p, v.data(), v.size() * 8, hipMemcpyHostToDevice, c->stream));
     SLLG_HIP(hipStreamSynchronize(c->stream));
     *dev_xi = c->XI.as<double>();
@@ -291,6 +302,8 @@ int llg_create(const llg_desc* d, llg_ctx** out) {
     c->anis_in = d->anis;
     c->basemm = d->species[0].mu * LLG_MU_B;  // src/cuint.cu:56
     const uint32_t n = c->n;
+    c->perm.na = 1;
+    c->perm.ncell = n;
 
     // per-species constants: src/llg.cpp:591-592, src/geom.cpp:244
     c->sp64.resize(c->ns);
@@ -332,10 +345,6 @@ int llg_create(const llg_desc* d, llg_ctx** out) {
     SLLG_HIP(hipMemcpy(c->sp32.p, sp32.data(), sp32.size() * sizeof(Species<float>), hipMemcpyHostToDevice));
     SLLG_TRY(c->sp64d.alloc(c->sp64.size() * sizeof(Species<double>)));
     SLLG_HIP(hipMemcpy(c->sp64d.p, c->sp64.data(), c->sp64.size() * sizeof(Species<double>), hipMemcpyHostToDevice));
-    SLLG_TRY(c->spec.alloc(n));
-    SLLG_HIP(hipMemcpy(c->spec.p, spec8.data(), n, hipMemcpyHostToDevice));
-    SLLG_TRY(c->sub.alloc(n));
-    SLLG_HIP(hipMemcpy(c->sub.p, sub8.data(), n, hipMemcpyHostToDevice));
 
     const size_t pb = prec_bytes(c);
     const size_t v3 = 3 * (size_t)n * pb;
@@ -378,6 +387,19 @@ int llg_create(const llg_desc* d, llg_ctx** out) {
             for (int m = 0; m < 5; ++m) SLLG_TRY(c->K[m].alloc(v3));
         }
         SLLG_TRY(build_exchange_fp32(c, d->exch));
+    }
+    // per-spin species / sublattice bytes in the device order chosen by the encoder
+    {
+        std::vector<uint8_t> s8(n), u8(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t dv = c->perm.to_dev(i);
+            s8[dv] = spec8[i];
+            u8[dv] = sub8[i];
+        }
+        SLLG_TRY(c->spec.alloc(n));
+        SLLG_HIP(hipMemcpy(c->spec.p, s8.data(), n, hipMemcpyHostToDevice));
+        SLLG_TRY(c->sub.alloc(n));
+        SLLG_HIP(hipMemcpy(c->sub.p, u8.data(), n, hipMemcpyHostToDevice));
     }
     if (d->dip.enabled) {
         if (!d->dip.mesh) SLLG_FAIL(LLG_E_INVALID, "dipolar enabled without mesh coordinates");
@@ -578,9 +600,7 @@ int llg_noise_sample(llg_ctx* c, uint64_t t, double* xi) {
     DevBuf out;
     SLLG_TRY(out.alloc(3 * (size_t)c->n * 8));
     SLLG_TRY(noise_sample(c, t, out.as<double>()));
-    SLLG_HIP(hipMemcpyAsync(xi, out.p, 3 * (size_t)c->n * 8, hipMemcpyDeviceToHost, c->stream));
-    SLLG_HIP(hipStreamSynchronize(c->stream));
-    return 0;
+    return download_state(c, out.p, true, xi, xi + c->n, xi + 2 * (size_t)c->n);
 }
 
 void* llg_stream(llg_ctx* c) { return c ? (void*)c->stream : nullptr; }

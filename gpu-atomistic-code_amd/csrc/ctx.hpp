@@ -68,7 +68,33 @@ template <class V> struct Exch {
     const V* dict = nullptr;              // [ndict][ncomp] (SELL_PACK) or raw values
     uint32_t ndict = 0;
     uint64_t nent = 0;                    // total entries (SELL_RAW value stride)
+    uint32_t ncell = 0;                   // rows per sublattice block of the device order
 };
+
+// Device order of the FP32 sliced-ELL path.  The reference orders spins basis-atom
+// fastest, i = cell * na + t (src/geom.cpp:214-266); on the device spin t of cell c sits
+// at d = t * ncell + c ("sublattice-major").  Rows of one slice then share t, and the
+// neighbours a wavefront gathers for one coupling entry are consecutive cells of one
+// sublattice: contiguous loads instead of na-strided ones.
+struct Perm {
+    uint32_t na = 1, ncell = 0;
+    __host__ __device__ uint32_t to_dev(uint32_t i) const { return (i % na) * ncell + i / na; }
+    __host__ __device__ uint32_t to_ref(uint32_t d) const { return (d % ncell) * na + d / ncell; }
+};
+
+// Packed sliced-ELL entry: bits 31..28 target sublattice t', 27..16 dictionary index,
+// 15..0 int16 cell delta (wrapped modulo ncell).
+__host__ __device__ __forceinline__ uint32_t pack_word(uint32_t t2, uint32_t idx, int32_t dcell) {
+    return (t2 << 28) | ((idx & 0xFFFu) << 16) | (uint32_t)(uint16_t)(int16_t)dcell;
+}
+__host__ __device__ __forceinline__ uint32_t pack_dict(uint32_t w) { return (w >> 16) & 0xFFFu; }
+// Neighbour device row of entry w for a row in cell `cell`.
+__host__ __device__ __forceinline__ uint32_t pack_nb(uint32_t w, uint32_t cell, uint32_t ncell) {
+    int32_t cj = (int32_t)cell + (int32_t)(int16_t)(w & 0xFFFFu);
+    cj += (cj < 0) ? (int32_t)ncell : 0;
+    cj -= (cj >= (int32_t)ncell) ? (int32_t)ncell : 0;
+    return (w >> 28) * ncell + (uint32_t)cj;
+}
 
 struct DipolarPlan;  // k_dipolar.hip
 
@@ -109,6 +135,8 @@ struct llg_ctx {
     sllg::AnisG<float> an32{};
     std::vector<sllg::Species<double>> sp64;
     bool multi_species = false;
+    sllg::Perm perm;  // device order of the FP32 sliced-ELL path (identity elsewhere)
+    bool permuted() const { return perm.na > 1; }
 
     // device state
     sllg::DevBuf S, Y, A, H, FN, DET, XI, STG, STG0, DEM;  // SoA 3n (type per path)

@@ -171,23 +171,35 @@ int build_sell(llg_ctx* c) {
     const HostExch& h = c->hex;
     const uint32_t n = c->n;
     const int nc = h.ncomp;
+    const Perm P = c->perm;
+    const uint32_t ncell = P.ncell;
     const uint32_t nsl = (n + kSlice - 1) / kSlice;
+    // device row d holds reference row P.to_ref(d)
+    std::vector<uint32_t> ref_of(n);
+    for (uint32_t d = 0; d < n; ++d) ref_of[d] = P.to_ref(d);
     std::vector<uint32_t> mx(nsl, 0);
     for (uint32_t s = 0; s < nsl; ++s)
-        for (uint32_t r = s * kSlice; r < std::min(n, (s + 1) * kSlice); ++r)
+        for (uint32_t d = s * kSlice; d < std::min(n, (s + 1) * kSlice); ++d) {
+            const uint32_t r = ref_of[d];
             mx[s] = std::max(mx[s], h.xadj[r + 1] - h.xadj[r]);
-    // Packed form: every column within +-32767 of its row modulo n, few distinct tuples.
-    bool pack = true;
-    const int64_t half = (int64_t)n / 2;
+        }
+    // Packed form: target sublattice, cell delta within +-32767 (modulo ncell), few tuples.
+    bool pack = P.na <= 16;
     std::vector<int16_t> delta(h.nnz);
-    for (uint32_t r = 0; r < n && pack; ++r)
+    std::vector<uint8_t> tgt(h.nnz);
+    const int64_t half = (int64_t)ncell / 2;
+    for (uint32_t r = 0; r < n && pack; ++r) {
+        const uint32_t cr = P.to_dev(r) % ncell;
         for (uint32_t q = h.xadj[r]; q < h.xadj[r + 1]; ++q) {
-            int64_t d = (int64_t)h.adj[q] - (int64_t)r;
-            if (d > half) d -= n;
-            if (d < -half) d += n;
+            const uint32_t dj = P.to_dev(h.adj[q]);
+            int64_t d = (int64_t)(dj % ncell) - (int64_t)cr;
+            if (d > half) d -= ncell;
+            if (d < -half) d += ncell;
             if (d < -32768 || d > 32767) { pack = false; break; }
             delta[q] = (int16_t)d;
+            tgt[q] = (uint8_t)(dj / ncell);
         }
+    }
     std::unordered_map<TupleKey, uint32_t, TupleHash> dict;
     std::vector<TupleKey> tuples;
     std::vector<uint32_t> idx;
@@ -209,7 +221,7 @@ int build_sell(llg_ctx* c) {
             uint32_t id;
             if (it == dict.end()) {
                 id = (uint32_t)tuples.size();
-                if ((uint64_t)(id + 1) * nc > (uint64_t)kDictLds) { pack = false; break; }
+                if ((uint64_t)(id + 1) * nc > (uint64_t)kDictLds || id > 0xFFFu) { pack = false; break; }
                 dict.emplace(k, id);
                 tuples.push_back(k);
             } else {
@@ -231,12 +243,13 @@ int build_sell(llg_ctx* c) {
         for (uint32_t s = 0; s < nsl; ++s) {
             const uint32_t len = slice_off[s + 1] - slice_off[s];
             for (uint32_t l = 0; l < kSlice; ++l) {
-                const uint32_t r = s * kSlice + l;
+                const uint32_t d = s * kSlice + l;
+                const uint32_t r = d < n ? ref_of[d] : 0;
                 for (uint32_t q = 0; q < len; ++q) {
-                    uint32_t w = 0;  // delta 0, tuple 0
-                    if (r < n && q < h.xadj[r + 1] - h.xadj[r]) {
+                    uint32_t w = pack_word(d < n ? d / ncell : 0, 0, 0);  // zero coupling on itself
+                    if (d < n && q < h.xadj[r + 1] - h.xadj[r]) {
                         const uint32_t qq = h.xadj[r] + q;
-                        w = ((uint32_t)(uint16_t)delta[qq] << 16) | idx[qq];
+                        w = pack_word(tgt[qq], idx[qq], delta[qq]);
                     }
                     ent[(size_t)slice_off[s] * kSlice + (size_t)(q >> 2) * (kSlice * 4) + l * 4 + (q & 3)] = w;
                 }
@@ -255,7 +268,7 @@ int build_sell(llg_ctx* c) {
         x.fmt = FMT_SELL_PACK;
         x.ndict = (uint32_t)tuples.size();
         char buf[48];
-        std::snprintf(buf, sizeof buf, "sell64x4-pack16x16-dict%u", x.ndict);
+        std::snprintf(buf, sizeof buf, "sell64x4-pack-t%u-dict%u", P.na, x.ndict);
         c->exch_encoding = buf;
     } else {
         std::vector<float> v32((size_t)nc * nent, 0.0f);
@@ -263,18 +276,19 @@ int build_sell(llg_ctx* c) {
         for (uint32_t s = 0; s < nsl; ++s) {
             const uint32_t len = slice_off[s + 1] - slice_off[s];
             for (uint32_t l = 0; l < kSlice; ++l) {
-                const uint32_t r = s * kSlice + l;
+                const uint32_t d = s * kSlice + l;
+                const uint32_t r = d < n ? ref_of[d] : 0;
                 for (uint32_t q = 0; q < len; ++q) {
                     const size_t o = ((size_t)slice_off[s] + q) * kSlice + l;
-                    if (r < n && q < h.xadj[r + 1] - h.xadj[r]) {
+                    if (d < n && q < h.xadj[r + 1] - h.xadj[r]) {
                         const uint32_t qq = h.xadj[r] + q;
-                        ent[o] = h.adj[qq];
+                        ent[o] = P.to_dev(h.adj[qq]);
                         for (int a = 0; a < nc; ++a) {
                             v64[(size_t)a * nent + o] = h.vals[(size_t)a * h.nnz + qq];
                             v32[(size_t)a * nent + o] = (float)h.vals[(size_t)a * h.nnz + qq];
                         }
                     } else {
-                        ent[o] = r < n ? r : 0;
+                        ent[o] = d < n ? d : 0;
                     }
                 }
             }
@@ -284,12 +298,13 @@ int build_sell(llg_ctx* c) {
         SLLG_TRY(upload(c->ex_dict64, v64.data(), v64.size(), c->stream));
         x.fmt = FMT_SELL_RAW;
         x.ndict = 0;
-        c->exch_encoding = "sell64-u32-f32";
+        c->exch_encoding = P.na > 1 ? "sell64-u32-f32-sublattice-major" : "sell64-u32-f32";
     }
     x.slice_off = c->ex_a.as<uint32_t>();
     x.ent = c->ex_b.as<uint32_t>();
     x.dict = c->ex_c.as<float>();
     x.nent = nent;
+    x.ncell = ncell;
     x64 = Exch<double>();
     fill_common(x64, h, n);
     x64.fmt = x.fmt;
@@ -298,6 +313,7 @@ int build_sell(llg_ctx* c) {
     x64.dict = c->ex_dict64.as<double>();
     x64.ndict = x.ndict;
     x64.nent = nent;
+    x64.ncell = ncell;
     return 0;
 }
 
@@ -311,6 +327,16 @@ int build_exchange_fp32(llg_ctx* c, const llg_exchange& e) {
     fill_common(c->ex64, h, c->n);
     c->exch_encoding = "none";
     if (h.fmt == LLG_EXCH_CSR) {
+        // Sublattice-major device order when the caller declares the basis size (spins
+        // ordered cell * n_atoms + t, as the reference orders them).
+        const uint32_t na = e.n_atoms;
+        if (na > 1 && na <= 16 && c->n % na == 0 && !(c->flags & 0x200u)) {
+            c->perm.na = na;
+            c->perm.ncell = c->n / na;
+        } else {
+            c->perm.na = 1;
+            c->perm.ncell = c->n;
+        }
         SLLG_TRY(build_sell(c));
     } else if (h.fmt == LLG_EXCH_DIA || h.fmt == LLG_EXCH_STENCIL) {
         std::vector<float> v32(h.vals.size());

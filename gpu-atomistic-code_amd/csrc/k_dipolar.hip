@@ -143,6 +143,7 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
     p->P = (size_t)p->G[0] * p->G[1] * p->G[2];
     p->Ph = (size_t)p->G[0] * p->G[1] * (p->G[2] / 2 + 1);
     const uint32_t n = c->n;
+    // grid point of each spin, indexed in the handle's device order
     std::vector<uint32_t> lin(n);
     for (uint32_t i = 0; i < n; ++i) {
         uint32_t m[3];
@@ -150,7 +151,7 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
             m[a] = d.mesh[(size_t)i * 3 + a];
             if (m[a] >= d.dim[a] * d.nk[a]) SLLG_FAIL(LLG_E_INVALID, "mesh coordinate of spin %u outside dim*nk", i);
         }
-        lin[i] = (uint32_t)(((size_t)m[0] * p->G[1] + m[1]) * p->G[2] + m[2]);
+        lin[c->perm.to_dev(i)] = (uint32_t)(((size_t)m[0] * p->G[1] + m[1]) * p->G[2] + m[2]);
     }
     const size_t rs = p->f64 ? sizeof(double) : sizeof(float);
     SLLG_TRY(p->lin.alloc(n * sizeof(uint32_t)));
@@ -160,8 +161,9 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
         std::vector<float> mu32(n);
         for (uint32_t i = 0; i < n; ++i) {
             const uint32_t s = spin_species ? spin_species[i] : (i % c->ns);
-            mu64[i] = c->mu_sp[s];
-            mu32[i] = (float)c->mu_sp[s];
+            const uint32_t dv = c->perm.to_dev(i);
+            mu64[dv] = c->mu_sp[s];
+            mu32[dv] = (float)c->mu_sp[s];
         }
         SLLG_TRY(p->mu.alloc(n * rs));
         SLLG_HIP(hipMemcpy(p->mu.p, p->f64 ? (void*)mu64.data() : (void*)mu32.data(), n * rs, hipMemcpyHostToDevice));

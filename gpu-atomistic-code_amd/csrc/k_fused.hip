@@ -32,12 +32,11 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
     R v[NC];
     if (e.fmt == FMT_SELL_PACK) {
         const uint32_t* __restrict__ ent = e.ent + (size_t)q0 * kSlice + lane * 4;
+        const uint32_t cell = i % e.ncell;
         for (uint32_t q = 0; q < len; ++q) {
             const uint32_t w = ent[(size_t)(q >> 2) * (kSlice * 4) + (q & 3)];
-            int32_t j = (int32_t)i + (int32_t)(int16_t)(w >> 16);
-            j += (j < 0) ? (int32_t)n : 0;
-            j -= (j >= (int32_t)n) ? (int32_t)n : 0;
-            const V* dv = e.dict + (size_t)(w & 0xFFFFu) * NC;
+            const uint32_t j = pack_nb(w, cell, e.ncell);
+            const V* dv = e.dict + (size_t)pack_dict(w) * NC;
 #pragma unroll
             for (int k = 0; k < NC; ++k) v[k] = (R)dv[k];
             const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
@@ -59,21 +58,18 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
 
 // One packed entry: decode, gather the neighbour, read its coupling from the LDS dictionary.
 template <int NC>
-__device__ __forceinline__ void pack_entry(uint32_t w, uint32_t i, uint32_t n, const float* __restrict__ S,
-                                           const float* sdict, float dot[3]) {
-    int32_t j = (int32_t)i + (int32_t)(int16_t)(w >> 16);
-    j += (j < 0) ? (int32_t)n : 0;
-    j -= (j >= (int32_t)n) ? (int32_t)n : 0;
+__device__ __forceinline__ void pack_entry(uint32_t w, uint32_t cell, uint32_t ncell, uint32_t n,
+                                           const float* __restrict__ S, const float* sdict, float dot[3]) {
+    const uint32_t j = pack_nb(w, cell, ncell);
     const float s[3] = {S[j], S[n + j], S[2 * (size_t)n + j]};
     float v[NC];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) v[k] = sdict[(w & 0xFFFFu) * NC + k];
+    for (int k = 0; k < NC; ++k) v[k] = sdict[pack_dict(w) * NC + k];
     accum<NC, float>(v, s, dot);
 }
 
-// Fast packed reader of the fused kernels: one uint4 (4 entries) per lane per load, the
-// next chunk prefetched while the current one's neighbours are gathered, couplings from
-// the block's LDS copy of the dictionary.
+// Fast packed reader of the fused kernels: one uint4 (4 entries) per lane per load
+// (1 KB per wavefront), couplings from the block's LDS copy of the dictionary.
 template <int NC>
 __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_t i, const float* __restrict__ S,
                                                    const float* sdict, float dot[3]) {
@@ -84,15 +80,14 @@ __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_
     const uint32_t nch = (e.slice_off[sl + 1] - q0) >> 2;
     if (nch == 0) return;
     const uint4* __restrict__ p = reinterpret_cast<const uint4*>(e.ent + (size_t)q0 * kSlice) + lane;
-    uint4 cur = p[0];
+    const uint32_t ncell = e.ncell, cell = i % ncell;
 #pragma unroll 2
     for (uint32_t c = 0; c < nch; ++c) {
-        const uint4 nxt = (c + 1 < nch) ? p[(size_t)(c + 1) * kSlice] : cur;
-        pack_entry<NC>(cur.x, i, n, S, sdict, dot);
-        pack_entry<NC>(cur.y, i, n, S, sdict, dot);
-        pack_entry<NC>(cur.z, i, n, S, sdict, dot);
-        pack_entry<NC>(cur.w, i, n, S, sdict, dot);
-        cur = nxt;
+        const uint4 cur = p[(size_t)c * kSlice];
+        pack_entry<NC>(cur.x, cell, ncell, n, S, sdict, dot);
+        pack_entry<NC>(cur.y, cell, ncell, n, S, sdict, dot);
+        pack_entry<NC>(cur.z, cell, ncell, n, S, sdict, dot);
+        pack_entry<NC>(cur.w, cell, ncell, n, S, sdict, dot);
     }
 }
 
@@ -108,6 +103,7 @@ __device__ __forceinline__ void exch_any(const Exch<V>& e, uint32_t i, const Spi
 
 struct FusedArgs {
     uint32_t n;
+    Perm perm;  // device row -> reference spin index (counter noise is keyed by the latter)
     int noise;  // 0 none (T = 0), 1 counter, 2 injected
     int multi;  // per-spin species index present
     int xcd;    // XCD-aware block remap
@@ -160,7 +156,7 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const 
     const Species<float> sp = a.multi ? a.sp[a.spec[i]] : a.sp0;
     if (a.noise == 1) {
         float z[3];
-        normals3_f32(a.seed, a.stream, i, a.t, z);
+        normals3_f32(a.seed, a.stream, a.perm.to_ref(i), a.t, z);
         const float TP = a.sqrtT * sp.sigma;
         h[0] += z[0] * TP;
         h[1] += z[1] * TP;
@@ -340,6 +336,7 @@ __global__ __launch_bounds__(kTPB) void k_fused_detfield(DetArgs a, const float*
 static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
     FusedArgs a{};
     a.n = c->n;
+    a.perm = c->perm;
     if (c->T > 0.0) a.noise = (c->noise_mode == LLG_NOISE_INJECTED) ? 2 : 1;
     a.multi = c->multi_species ? 1 : 0;
     a.xcd = (c->flags & 0x100u) ? 0 : 1;

@@ -111,11 +111,8 @@ __device__ __forceinline__ void exch_row_sell64(const Exch<double>& e, uint32_t 
         else
             w = e.ent[((size_t)q0 + q) * kSlice + lane];
         if (e.fmt == FMT_SELL_PACK) {
-            int32_t jj = (int32_t)i + (int32_t)(int16_t)(w >> 16);
-            jj += (jj < 0) ? (int32_t)n : 0;
-            jj -= (jj >= (int32_t)n) ? (int32_t)n : 0;
-            j = (uint32_t)jj;
-            const double* dv = e.dict + (size_t)(w & 0xFFFFu) * NC;
+            j = pack_nb(w, i % e.ncell, e.ncell);
+            const double* dv = e.dict + (size_t)pack_dict(w) * NC;
             for (int k = 0; k < NC; ++k) v[k] = dv[k];
         } else {
             j = w;
@@ -180,11 +177,12 @@ __global__ __launch_bounds__(kTPB) void k_noise_f64(uint32_t n, uint64_t seed, u
     for (int c = 0; c < 3; ++c) out[(size_t)c * n + i] = z[c];
 }
 
-__global__ __launch_bounds__(kTPB) void k_noise_f32(uint32_t n, uint64_t seed, uint64_t stream, uint64_t t, double* out) {
+__global__ __launch_bounds__(kTPB) void k_noise_f32(uint32_t n, Perm perm, uint64_t seed, uint64_t stream, uint64_t t,
+                                                    double* out) {
     const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
     if (i >= n) return;
     float z[3];
-    normals3_f32(seed, stream, i, t, z);
+    normals3_f32(seed, stream, perm.to_ref(i), t, z);
     for (int c = 0; c < 3; ++c) out[(size_t)c * n + i] = (double)z[c];
 }
 
@@ -280,7 +278,7 @@ int noise_sample(llg_ctx* c, uint64_t t, double* dev_out) {
     if (c->prec == LLG_FP64_PLAIN)
         k_noise_f64<<<blocks_for(c->n, kTPB), kTPB, 0, c->stream>>>(c->n, c->seed, c->stream_id, t, dev_out);
     else
-        k_noise_f32<<<blocks_for(c->n, kTPB), kTPB, 0, c->stream>>>(c->n, c->seed, c->stream_id, t, dev_out);
+        k_noise_f32<<<blocks_for(c->n, kTPB), kTPB, 0, c->stream>>>(c->n, c->perm, c->seed, c->stream_id, t, dev_out);
     SLLG_HIP(hipGetLastError());
     return 0;
 }

@@ -28,6 +28,7 @@ EXCH_NONE, EXCH_DIA, EXCH_CSR, EXCH_STENCIL = 0, 1, 2, 3
 NOISE_COUNTER, NOISE_INJECTED = 0, 1
 FLAG_CHECK_FINITE = 0x1
 FLAG_NO_XCD_REMAP = 0x100
+FLAG_NO_SUBLATTICE_ORDER = 0x200
 
 STATUS = {0: "OK", -1: "INVALID", -2: "NOMEM", -3: "DEVICE", -4: "UNSUPPORTED", -5: "NODEVICE",
           -6: "STATE", -7: "NONFINITE"}
@@ -358,6 +359,7 @@ class System:
             self._keep += [vals, xadj, adj]
             d.exch.nnz = len(adj)
             d.exch.xadj, d.exch.adjncy, d.exch.csr_vals = _p(xadj, C.c_uint32), _p(adj, C.c_uint32), _p(vals, C.c_double)
+            d.exch.n_atoms = int(e.get("n_atoms", 0))  # basis size: enables the sublattice-major device order
         elif d.exch.format == EXCH_DIA:
             dv = np.ascontiguousarray(e["vals"], dtype=np.float64)
             offs = np.ascontiguousarray(e["offsets"], dtype=np.int32)

@@ -49,7 +49,7 @@ def fcc_3shell(L=64, precision="fp32", lam=0.05, dt=1e-16, Kc=5e-24, seed=42, up
     xadj, adj, vals = build_csr(uc, dims, ints, [MU] * 4, ncomp=ncomp)
     n = L ** 3 * 4
     sysd = System(n=n, species=[_species(lam=lam)], anis=dict(kc=Kc),
-                  exch=dict(format=EXCH_CSR, xadj=xadj, adjncy=adj, vals=vals), dt=dt, scheme=scheme,
+                  exch=dict(format=EXCH_CSR, xadj=xadj, adjncy=adj, vals=vals, n_atoms=4), dt=dt, scheme=scheme,
                   precision=precision, noise=noise, update=update, seed=seed)
     return sysd, random_spins(n, seed, seed + 100)
 

@@ -123,7 +123,10 @@ typedef struct llg_exchange {
     const double* dia_vals;     /* [ncomp][ndiag][n] */
     /* Periodic stencil on the spin order idx = ((x*dims1 + y)*dims2 + z)*n_atoms + t
      * (src/geom.cpp:214-266).  Basis atom t has stencil_count[t] entries
-     * (dx, dy, dz, t'); neighbours across a non-periodic face are skipped. */
+     * (dx, dy, dz, t'); neighbours across a non-periodic face are skipped.
+     * With LLG_EXCH_CSR, n_atoms > 1 (<= 16) declares that spins follow that order
+     * (basis atom fastest); the FP32 path then stores spins sublattice-major on the
+     * device.  Results are returned in the caller's order either way. */
     uint32_t dims[3];
     uint32_t n_atoms;
     uint32_t pbc[3];

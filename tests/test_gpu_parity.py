@@ -188,3 +188,81 @@ def test_invalid_descriptor_rejected(gpu):
     with pytest.raises(sllg.LLGError) as e:
         sllg.Context(sysd)
     assert e.value.code == -1
+
+
+# ----------------------------------------------------------------------------- sublattice-major order
+
+def _fcc_multi(L=4, precision="fp32", noise="counter", dip=False):
+    """fcc with 4 species (one per basis atom, 4 sublattices), every local term, optional dipolar;
+    the CSR declares n_atoms = 4 so the FP32 path stores spins sublattice-major."""
+    sysd, s0 = W.fcc_3shell(L=L, precision=precision, noise=noise)
+    sysd.species = [dict(mu=2.22 if t != 2 else 1.5, lambda_=0.05 + 0.02 * t, gamma_rel=1.0 + 0.1 * t,
+                         k1u=(1e-23 if t % 2 == 0 else -3e-24), k1u_dir=(0.0, 0.6, 0.8), sublattice=t)
+                    for t in range(4)]
+    sysd.spin_species = (np.arange(sysd.n) % 4).astype(np.uint32)
+    sysd.n_sublattice = 4
+    if dip:
+        uc = sllg.unit_cell("fcc")
+        sysd.dip = dict(enabled=1, mesh=sllg.mesh(uc, (L, L, L)), dim=(L, L, L), nk=(2, 2, 2),
+                        abc=(3.6e-10,) * 3)
+    return sysd, s0
+
+
+def test_sublattice_order_is_used_and_matches_oracle(gpu):
+    sysd, s0 = _fcc_multi(L=4)
+    import pyoracle
+
+    g = sllg.Context(sysd)
+    assert "-t4-" in g.info()["exch_encoding"]
+    rng = np.random.default_rng(4)
+    stag = rng.normal(0, 3.0, (3, sysd.n))
+    g2, o = run_pair(sysd, s0, 100, T=0.0, B=(0.1, 0.2, 0.7), stag=stag)
+    d = _maxdiff(g2.get_spins(), o.get_spins())
+    print("fcc sublattice-major 100 steps max|dS| =", d)
+    assert d <= 1e-4
+    assert _maxdiff(g2.field(), o.field()) <= 1e-4 * np.abs(o.field()).max()
+    mg, eg = g2.reduce()
+    mo, eo = o.reduce()
+    assert _maxdiff(mg, mo) <= 1e-5 and abs(eg - eo) <= 1e-5 * abs(eo)
+    # counter noise is keyed by the reference spin index: identical to the oracle's draws
+    a, b = g2.noise_sample(17), o.noise_sample(17)
+    assert _maxdiff(a, b) <= 2e-5 * np.abs(b).max()
+
+
+def test_sublattice_order_equals_identity_order(gpu):
+    """The device order is invisible: same results (to FP32 rounding of the same sums) with
+    the permutation switched off, at T > 0 with counter noise."""
+    sysd, s0 = _fcc_multi(L=4)
+    res = []
+    for flags in (0, sllg.FLAG_NO_SUBLATTICE_ORDER):
+        sysd.flags = flags
+        g = sllg.Context(sysd)
+        g.set_spins(s0)
+        g.set_temperature(200.0)
+        g.step(0, 50)
+        res.append(g.get_spins())
+    assert _maxdiff(res[0], res[1]) <= 2e-5
+
+
+def test_sublattice_order_injected_noise(gpu):
+    sysd, s0 = _fcc_multi(L=4, noise="injected")
+    xi = np.random.default_rng(9).standard_normal((30, 3, sysd.n))
+    g, o = run_pair(sysd, s0, 30, T=300.0, xi=xi)
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print("fcc sublattice-major injected max|dS| =", d)
+    assert d <= 1e-4
+
+
+def test_sublattice_order_dipolar(gpu):
+    sysd, s0 = _fcc_multi(L=4, dip=True)
+    import pyoracle
+
+    g = sllg.Context(sysd)
+    o = pyoracle.OracleContext(sysd)
+    for c in (g, o):
+        c.set_spins(s0)
+        c.step(0, 1)
+    hg, ho = g.field(), o.field()
+    rel = _maxdiff(hg, ho) / np.abs(ho).max()
+    print("fcc dipolar field rel err", rel)
+    assert rel <= 2e-5
