@@ -149,6 +149,146 @@ def headline(args):
     print(json.dumps(line), flush=True)
 
 
+def accuracy(args):
+    """FP32 fused Heun vs the FP64 plain path (bit-identical to the FP64 oracle) on the
+    1M-spin fcc case at T = 0 from the same random state, and FP64 vs FP64 with a 1e-15
+    perturbation of one component (the Lyapunov growth the dynamics imposes)."""
+    import sllg
+    from sllg import workloads as W
+
+    sysd, s0 = W.fcc_3shell(L=args.L, precision="fp32")
+    checkpoints = [c for c in (100, 1000, 3000, 10000, 30000) if c <= args.steps]
+
+    def run(prec, s, ncomp=1):
+        sd, _ = W.fcc_3shell(L=args.L, precision=prec, ncomp=ncomp)
+        g = sllg.Context(sd)
+        g.set_spins(s)
+        g.set_temperature(0.0)
+        out, t = {}, 0
+        for c in checkpoints:
+            g.step(t, c - t)
+            t = c
+            out[c] = (g.get_spins(), g.reduce(energy=False)[0][0])
+        g.close()
+        return out
+
+    ref = run("fp64", s0)
+    f32 = run("fp32", s0)
+    sp = s0.copy()
+    sp[0] += 1e-15
+    sp /= np.linalg.norm(sp, axis=0)
+    lya = run("fp64", sp)
+    rows = []
+    for c in checkpoints:
+        rows.append({"step": c,
+                     "fp32_max_abs_dS": float(np.abs(f32[c][0] - ref[c][0]).max()),
+                     "fp32_abs_dM": float(np.abs(f32[c][1] - ref[c][1]).max()),
+                     "fp64_perturbed_max_abs_dS": float(np.abs(lya[c][0] - ref[c][0]).max()),
+                     "fp64_perturbed_abs_dM": float(np.abs(lya[c][1] - ref[c][1]).max()),
+                     "M_fp64": [float(x) for x in ref[c][1]]})
+    print(json.dumps({"mode": "accuracy", "n_spins": sysd.n, "T": 0.0, "rows": rows}), flush=True)
+
+
+def rk4_dmi(args):
+    """bct 96x96x48 x 2 antiferromagnet, 3x3 tensors with per-bond DMI as a periodic stencil
+    with per-row tensors, easy-plane anisotropy, T = 0, RK4, Neel + 1e-3 rad tilt: FP32
+    spin-updates/s and the FP32-vs-FP64 trajectory error."""
+    import torch
+    import sllg
+    from sllg import workloads as W
+
+    out = {"mode": "rk4_dmi"}
+    res = {}
+    for prec in ("fp32", "fp64"):
+        sysd, s0 = W.bct_afm_dmi(precision=prec)
+        g = sllg.Context(sysd)
+        g.set_spins(s0)
+        g.step(0, 5)
+        steps = args.steps if prec == "fp32" else max(20, args.steps // 10)
+        sec, _ = time_steps(g, torch, 5, steps)
+        out[prec] = {"spin_updates_per_s": sysd.n * steps / sec, "ms_per_step": sec * 1e3 / steps,
+                     "kernels_ms": g.profile_kernels(5 + steps, 5), "info": g.info()}
+        g.close()
+    n_err = args.err_steps
+    for prec in ("fp32", "fp64"):
+        sysd, s0 = W.bct_afm_dmi(precision=prec)
+        g = sllg.Context(sysd)
+        g.set_spins(s0)
+        g.step(0, n_err)
+        res[prec] = (g.get_spins(), g.reduce(energy=False)[0])
+        g.close()
+    out["n_spins"] = sysd.n
+    out["error_after_steps"] = n_err
+    out["fp32_max_abs_dS"] = float(np.abs(res["fp32"][0] - res["fp64"][0]).max())
+    out["fp32_abs_dM"] = float(np.abs(res["fp32"][1] - res["fp64"][1]).max())
+    out["fp32_over_fp64"] = out["fp32"]["spin_updates_per_s"] / out["fp64"]["spin_updates_per_s"]
+    print(json.dumps(out), flush=True)
+
+
+def batched(args):
+    """64 replicas of the 128x128x16 film (NN exchange + FFT dipolar on the 256x256x32 padded
+    grid) at T_r = 900 r/63 K, Heun FP32, one batched launch per half step (hipGraph when
+    no dipolar refresh falls in the chunk), dipolar refreshed every `update` = 100 steps."""
+    import sllg
+    from sllg import batch as SB
+    from sllg import workloads as W
+
+    R = args.replicas
+    B = SB.Batch(flags=SB.BATCH_GRAPH)
+    t = time.perf_counter()
+    ids = []
+    for r in range(R):
+        sysd, s0 = W.sc_film_dip(Lx=args.film[0], Ly=args.film[1], Lz=args.film[2], precision="fp32", seed=100 + r,
+                                 stream=r)
+        ids.append(B.add(sysd, s0, T=900.0 * r / max(R - 1, 1)))
+    setup = time.perf_counter() - t
+    B.step(10)
+    B.reduce(ids)
+    steps = args.steps
+    t = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(100, steps - done)
+        B.step(k)
+        done += k
+    m = B.reduce(ids)  # synchronises
+    sec = time.perf_counter() - t
+    info = B.info()
+    print(json.dumps({"mode": "batched", "replicas": R, "spins_total": info["spins"], "steps": steps,
+                      "spin_updates_per_s": info["spins"] * steps / sec, "replica_steps_per_s": R * steps / sec,
+                      "ms_per_step": sec * 1e3 / steps, "setup_s": setup, "device_bytes": info["device_bytes"],
+                      "m_z_lowest_T": float(m[0][2]), "m_z_highest_T": float(m[-1][2])}), flush=True)
+
+
+def serve_mode(args):
+    """Poisson stream of simple-cubic M(T) and field-ramp requests, L log-uniform in
+    [10, 100], T uniform in [0, 1.5 Tc], steps uniform in [smin, smax]; batched FP32 loop vs
+    sequential FP64 plain solves, p50/p99 latency."""
+    import sllg
+    from sllg import batch as SB
+
+    rng = np.random.default_rng(args.seed)
+    Tc = 1.44 * 3e-21 / sllg.K_B
+    specs = []
+    t = 0.0
+    for k in range(args.requests):
+        t += rng.exponential(1.0 / args.rate)
+        L = int(round(10 ** rng.uniform(np.log10(args.lmin), np.log10(args.lmax))))
+        typ = SB.REQ_RAMP if rng.random() < 0.5 else SB.REQ_MT
+        specs.append(dict(type=typ, L=L, steps=int(rng.integers(args.smin, args.smax + 1)) // 100 * 100,
+                          update=100, T=float(rng.uniform(0, 1.5 * Tc)), arrival_s=t, seed=1000 + k,
+                          B0=(0, 0, -0.5) if typ == SB.REQ_RAMP else (0, 0, 0), B1=(0, 0, 0.5) if typ == SB.REQ_RAMP else (0, 0, 0)))
+    reqs = SB.make_requests(specs)
+    _, st, _ = SB.serve(reqs, budget_bytes=int(args.budget_gb * 2**30), graph=False)
+    out = {"mode": "serve", "requests": args.requests, "rate_per_s": args.rate,
+           "batched_fp32": st}
+    if args.sequential:
+        _, sq, _ = SB.serve(reqs, sequential=True, precision=sllg.FP64_PLAIN)
+        out["sequential_fp64_plain"] = sq
+        out["solves_per_s_ratio"] = st["solves_per_s"] / sq["solves_per_s"]
+    print(json.dumps(out), flush=True)
+
+
 def copy_probe(args):
     """Device copy of a known size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -168,20 +308,32 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--mode", default="headline", choices=["headline", "copy-probe"])
+    ap.add_argument("--mode", default="headline",
+                    choices=["headline", "copy-probe", "accuracy", "rk4", "batched", "serve"])
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--T", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--err-steps", type=int, default=2000, help="rk4: steps for the FP32-vs-FP64 error")
+    ap.add_argument("--replicas", type=int, default=64)
+    ap.add_argument("--film", type=int, nargs=3, default=[128, 128, 16])
+    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--rate", type=float, default=200.0)
+    ap.add_argument("--lmin", type=int, default=10)
+    ap.add_argument("--lmax", type=int, default=100)
+    ap.add_argument("--smin", type=int, default=2000)
+    ap.add_argument("--smax", type=int, default=20000)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--sequential", action="store_true", help="serve: also run the sequential FP64 plain comparator")
+    ap.add_argument("--seed", type=int, default=7)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.mode == "copy-probe":
-        copy_probe(args)
-    else:
-        headline(args)
+    modes = {"copy-probe": copy_probe, "accuracy": accuracy, "rk4": rk4_dmi, "batched": batched,
+             "serve": serve_mode, "headline": headline}
+    modes[args.mode](args)
 
 
 if __name__ == "__main__":

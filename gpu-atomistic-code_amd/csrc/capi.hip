@@ -252,9 +252,12 @@ void llg_desc_init(llg_desc* d) {
     d->exch.ncomp = 1;
 }
 
-int llg_create(const llg_desc* d, llg_ctx** out) {
+static int create_impl(const llg_desc* d, const std::shared_ptr<ExchStore>& share, llg_ctx** out) {
     if (!d || !out) SLLG_FAIL(LLG_E_INVALID, "descriptor or output pointer is NULL");
     *out = nullptr;
+    if (share && (share->n != d->n || share->prec != d->precision || share->device != d->device))
+        SLLG_FAIL(LLG_E_INVALID, "shared couplings differ in n, precision or device");
+    const bool proto = (bool)share;
     if (d->abi_version != LLG_ABI_VERSION) SLLG_FAIL(LLG_E_INVALID, "abi_version %u != %u", d->abi_version, LLG_ABI_VERSION);
     if (d->n == 0) SLLG_FAIL(LLG_E_INVALID, "n must be positive");
     if (d->n >= 0x7FFFFFFFu) SLLG_FAIL(LLG_E_UNSUPPORTED, "n above 2^31-1");
@@ -304,6 +307,15 @@ int llg_create(const llg_desc* d, llg_ctx** out) {
     const uint32_t n = c->n;
     c->perm.na = 1;
     c->perm.ncell = n;
+    if (proto) {
+        c->xs = share;  // shared, immutable couplings
+    } else {
+        c->xs = std::make_shared<ExchStore>();
+        c->xs->device = d->device;
+        c->xs->prec = d->precision;
+        c->xs->n = n;
+        c->xs->perm = c->perm;
+    }
 
     // per-species constants: src/llg.cpp:591-592, src/geom.cpp:244
     c->sp64.resize(c->ns);
@@ -351,7 +363,8 @@ int llg_create(const llg_desc* d, llg_ctx** out) {
     SLLG_TRY(c->S.alloc(v3));
     SLLG_TRY(c->Y.alloc(v3));
     SLLG_TRY(c->STG.alloc(v3));
-    SLLG_TRY(c->XI.alloc(3 * (size_t)n * sizeof(double)));
+    if (c->prec == LLG_FP64_PLAIN || c->noise_mode == LLG_NOISE_INJECTED)
+        SLLG_TRY(c->XI.alloc(3 * (size_t)n * sizeof(double)));
     if (c->prec == LLG_FP64_PLAIN) {
         SLLG_TRY(c->H.alloc(v3));
         SLLG_TRY(c->FN.alloc(v3));
@@ -378,7 +391,7 @@ int llg_create(const llg_desc* d, llg_ctx** out) {
             SLLG_TRY(plain_rk_init_tables());
             for (int m = 0; m < 6; ++m) SLLG_TRY(c->K[m].alloc(v3));
         }
-        SLLG_TRY(build_exchange_fp64(c, d->exch));
+        if (!proto) SLLG_TRY(build_exchange_fp64(c, d->exch));
     } else {
         SLLG_TRY(c->A.alloc(v3));
         if (c->scheme == LLG_RK4) SLLG_TRY(c->K[0].alloc(v3));
@@ -386,8 +399,11 @@ int llg_create(const llg_desc* d, llg_ctx** out) {
             SLLG_TRY(fused_rk_init_tables());
             for (int m = 0; m < 5; ++m) SLLG_TRY(c->K[m].alloc(v3));
         }
-        SLLG_TRY(build_exchange_fp32(c, d->exch));
+        if (!proto) SLLG_TRY(build_exchange_fp32(c, d->exch));
     }
+    c->ex32 = c->xs->ex32;
+    c->ex64 = c->xs->ex64;
+    c->perm = c->xs->perm;
     // per-spin species / sublattice bytes in the device order chosen by the encoder
     {
         std::vector<uint8_t> s8(n), u8(n);
@@ -423,6 +439,13 @@ int llg_create(const llg_desc* d, llg_ctx** out) {
     return 0;
 }
 
+int llg_create(const llg_desc* d, llg_ctx** out) { return create_impl(d, nullptr, out); }
+
+int llg_create_like(const llg_desc* d, llg_ctx* proto, llg_ctx** out) {
+    if (!proto || !proto->xs) SLLG_FAIL(LLG_E_INVALID, "NULL prototype");
+    return create_impl(d, proto->xs, out);
+}
+
 void llg_destroy(llg_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
@@ -430,7 +453,7 @@ void llg_destroy(llg_ctx* c) {
     dipolar_destroy(c);
     delete c->timer;
     c->timer = nullptr;
-    hipStream_t s = c->stream;
+    hipStream_t s = c->own_stream ? c->stream : nullptr;
     delete c;
     if (s) (void)hipStreamDestroy(s);
 }
@@ -644,12 +667,13 @@ int llg_get_info(llg_ctx* c, llg_info* info) {
     DevBuf* all[] = {&c->S, &c->Y, &c->A, &c->H, &c->FN, &c->DET, &c->XI, &c->STG, &c->STG0, &c->DEM,
                      &c->K[0], &c->K[1], &c->K[2], &c->K[3], &c->K[4], &c->K[5], &c->spec, &c->sub,
                      &c->sp32, &c->sp64d, &c->p_sigma, &c->p_llgpf, &c->p_lambda, &c->p_k1u, &c->p_k1udir,
-                     &c->p_kc, &c->ex_a, &c->ex_b, &c->ex_c, &c->ex_d, &c->ex_e, &c->ex_dict64};
+                     &c->p_kc, &c->red_part, &c->red_out};
     for (DevBuf* b : all) dev += b->bytes;
-    info->device_bytes = dev;
-    std::snprintf(info->exch_encoding, sizeof info->exch_encoding, "%s", c->exch_encoding.c_str());
+    info->device_bytes = dev + (c->xs ? c->xs->bytes() : 0);
+    std::snprintf(info->exch_encoding, sizeof info->exch_encoding, "%s", c->xs ? c->xs->encoding.c_str() : "none");
     const double n = (double)c->n;
-    const HostExch& h = c->hex;
+    static const HostExch empty;
+    const HostExch& h = c->xs ? c->xs->h : empty;
     const int nc = h.ncomp;
     // coupling bytes per row per field evaluation (M)
     double M = 0.0;
@@ -706,3 +730,10 @@ int llg_get_info(llg_ctx* c, llg_info* info) {
 }
 
 }  // extern "C"
+
+namespace sllg {
+// Internal: create a handle over an existing coupling store (batch.hip, serving).
+int create_with_store(const llg_desc* d, const std::shared_ptr<ExchStore>& store, llg_ctx** out) {
+    return create_impl(d, store, out);
+}
+}  // namespace sllg

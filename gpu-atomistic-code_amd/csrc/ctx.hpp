@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <deque>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -113,11 +114,27 @@ struct HostExch {
 
 struct KernelTimer;  // capi.hip
 
+// Device-resident exchange encoding, immutable after llg_create and shared by every
+// handle created from it (llg_create_like): replicas and serving requests of one lattice
+// hold one copy of the couplings.
+struct ExchStore {
+    int device = 0, prec = 0;
+    uint32_t n = 0;
+    DevBuf a, b, c, d, e, dict64;
+    Exch<float> ex32;
+    Exch<double> ex64;
+    Perm perm;
+    std::string encoding;
+    HostExch h;  // scalars kept; the large host vectors are released after upload
+    uint64_t bytes() const { return a.bytes + b.bytes + c.bytes + d.bytes + e.bytes + dict64.bytes; }
+};
+
 }  // namespace sllg
 
 struct llg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false while the handle is a member of a batch (batch stream)
     uint32_t n = 0, ns = 1, nsub = 1;
     int scheme = LLG_HEUN, prec = LLG_FP32, noise_mode = LLG_NOISE_COUNTER;
     uint64_t seed = 0, stream_id = 0;
@@ -145,13 +162,10 @@ struct llg_ctx {
     sllg::DevBuf sub;       // u8 sublattice per spin
     sllg::DevBuf sp32, sp64d;  // Species tables
     sllg::DevBuf p_sigma, p_llgpf, p_lambda, p_k1u, p_k1udir, p_kc;  // FP64 plain per-spin arrays
-    // exchange storage
-    sllg::DevBuf ex_a, ex_b, ex_c, ex_d, ex_e;
-    sllg::Exch<double> ex64;
+    // exchange: shared encoded store + by-value views of it for the kernels
+    std::shared_ptr<sllg::ExchStore> xs;
+    sllg::Exch<double> ex64;  // FP64 plain couplings, or FP64 dictionary/values of the FP32 path
     sllg::Exch<float> ex32;
-    sllg::DevBuf ex_dict64;  // FP64 copy of the dictionary / values for energy on the FP32 path
-    sllg::HostExch hex;
-    std::string exch_encoding;
     // reductions
     sllg::DevBuf red_part, red_out, m_series;
     // dipolar

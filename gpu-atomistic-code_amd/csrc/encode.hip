@@ -33,7 +33,8 @@ int upload(DevBuf& b, const T* h, size_t count, hipStream_t s) {
 }
 
 int validate_and_copy(llg_ctx* c, const llg_exchange& e) {
-    HostExch& h = c->hex;
+    ExchStore& X = *c->xs;
+    HostExch& h = X.h;
     const uint32_t n = c->n;
     h.fmt = e.format;
     h.ncomp = e.ncomp ? e.ncomp : 1;
@@ -104,45 +105,58 @@ void fill_common(Exch<V>& x, const HostExch& h, uint32_t n) {
 }
 
 int upload_stencil_tables(llg_ctx* c) {
-    const HostExch& h = c->hex;
+    ExchStore& X = *c->xs;
+    const HostExch& h = X.h;
     std::vector<int4> ent((size_t)h.na * h.sk);
     for (size_t q = 0; q < ent.size(); ++q)
         ent[q] = make_int4(h.sent[q * 4], h.sent[q * 4 + 1], h.sent[q * 4 + 2], h.sent[q * 4 + 3]);
-    SLLG_TRY(upload(c->ex_a, ent.data(), ent.size(), c->stream));
-    SLLG_TRY(upload(c->ex_b, h.scount.data(), h.scount.size(), c->stream));
+    SLLG_TRY(upload(X.a, ent.data(), ent.size(), c->stream));
+    SLLG_TRY(upload(X.b, h.scount.data(), h.scount.size(), c->stream));
     return 0;
+}
+
+// Drop the host copies once the device holds the encoding (scalars stay for llg_get_info).
+void release_host(HostExch& h) {
+    std::vector<uint32_t>().swap(h.xadj);
+    std::vector<uint32_t>().swap(h.adj);
+    std::vector<double>().swap(h.vals);
+    std::vector<int32_t>().swap(h.doff);
+    std::vector<uint32_t>().swap(h.scount);
+    std::vector<int32_t>().swap(h.sent);
 }
 
 }  // namespace
 
 int build_exchange_fp64(llg_ctx* c, const llg_exchange& e) {
+    ExchStore& X = *c->xs;
     SLLG_TRY(validate_and_copy(c, e));
-    const HostExch& h = c->hex;
-    Exch<double>& x = c->ex64;
+    const HostExch& h = X.h;
+    Exch<double>& x = X.ex64;
     fill_common(x, h, c->n);
-    c->exch_encoding = "none";
+    X.encoding = "none";
     if (h.fmt == LLG_EXCH_CSR) {
-        SLLG_TRY(upload(c->ex_a, h.xadj.data(), h.xadj.size(), c->stream));
-        SLLG_TRY(upload(c->ex_b, h.adj.data(), h.adj.size(), c->stream));
-        SLLG_TRY(upload(c->ex_c, h.vals.data(), h.vals.size(), c->stream));
-        x.xadj = c->ex_a.as<uint32_t>();
-        x.adj = c->ex_b.as<uint32_t>();
-        x.vals = c->ex_c.as<double>();
-        c->exch_encoding = "csr-u32-f64";
+        SLLG_TRY(upload(X.a, h.xadj.data(), h.xadj.size(), c->stream));
+        SLLG_TRY(upload(X.b, h.adj.data(), h.adj.size(), c->stream));
+        SLLG_TRY(upload(X.c, h.vals.data(), h.vals.size(), c->stream));
+        x.xadj = X.a.as<uint32_t>();
+        x.adj = X.b.as<uint32_t>();
+        x.vals = X.c.as<double>();
+        X.encoding = "csr-u32-f64";
     } else if (h.fmt == LLG_EXCH_DIA) {
-        SLLG_TRY(upload(c->ex_a, h.doff.data(), h.doff.size(), c->stream));
-        SLLG_TRY(upload(c->ex_c, h.vals.data(), h.vals.size(), c->stream));
-        x.doff = c->ex_a.as<int32_t>();
-        x.vals = c->ex_c.as<double>();
-        c->exch_encoding = "dia-f64";
+        SLLG_TRY(upload(X.a, h.doff.data(), h.doff.size(), c->stream));
+        SLLG_TRY(upload(X.c, h.vals.data(), h.vals.size(), c->stream));
+        x.doff = X.a.as<int32_t>();
+        x.vals = X.c.as<double>();
+        X.encoding = "dia-f64";
     } else if (h.fmt == LLG_EXCH_STENCIL) {
         SLLG_TRY(upload_stencil_tables(c));
-        SLLG_TRY(upload(c->ex_c, h.vals.data(), h.vals.size(), c->stream));
-        x.sent = c->ex_a.as<int4>();
-        x.scount = c->ex_b.as<uint32_t>();
-        x.vals = c->ex_c.as<double>();
-        c->exch_encoding = h.per_row ? "stencil-rowvals-f64" : "stencil-shared-f64";
+        SLLG_TRY(upload(X.c, h.vals.data(), h.vals.size(), c->stream));
+        x.sent = X.a.as<int4>();
+        x.scount = X.b.as<uint32_t>();
+        x.vals = X.c.as<double>();
+        X.encoding = h.per_row ? "stencil-rowvals-f64" : "stencil-shared-f64";
     }
+    release_host(X.h);
     return 0;
 }
 
@@ -168,10 +182,11 @@ struct TupleHash {
 // (delta 0, dictionary tuple 0.0), which add exactly 0 to the field.  Raw layout:
 // [entry][lane] u32 columns and [ncomp][entry][lane] FP32 values.
 int build_sell(llg_ctx* c) {
-    const HostExch& h = c->hex;
+    ExchStore& X = *c->xs;
+    const HostExch& h = X.h;
     const uint32_t n = c->n;
     const int nc = h.ncomp;
-    const Perm P = c->perm;
+    const Perm P = X.perm;
     const uint32_t ncell = P.ncell;
     const uint32_t nsl = (n + kSlice - 1) / kSlice;
     // device row d holds reference row P.to_ref(d)
@@ -235,9 +250,9 @@ int build_sell(llg_ctx* c) {
     std::vector<uint32_t> slice_off(nsl + 1, 0);
     for (uint32_t s = 0; s < nsl; ++s) slice_off[s + 1] = slice_off[s] + (pack ? (mx[s] + 3u) / 4u * 4u : mx[s]);
     const uint64_t nent = (uint64_t)slice_off[nsl] * kSlice;
-    Exch<float>& x = c->ex32;
-    Exch<double>& x64 = c->ex64;
-    SLLG_TRY(upload(c->ex_a, slice_off.data(), slice_off.size(), c->stream));
+    Exch<float>& x = X.ex32;
+    Exch<double>& x64 = X.ex64;
+    SLLG_TRY(upload(X.a, slice_off.data(), slice_off.size(), c->stream));
     std::vector<uint32_t> ent(nent, 0);
     if (pack) {
         for (uint32_t s = 0; s < nsl; ++s) {
@@ -262,14 +277,14 @@ int build_sell(llg_ctx* c) {
                 d64[t * nc + a] = tuples[t].v[a];
                 d32[t * nc + a] = (float)tuples[t].v[a];
             }
-        SLLG_TRY(upload(c->ex_b, ent.data(), ent.size(), c->stream));
-        SLLG_TRY(upload(c->ex_c, d32.data(), d32.size(), c->stream));
-        SLLG_TRY(upload(c->ex_dict64, d64.data(), d64.size(), c->stream));
+        SLLG_TRY(upload(X.b, ent.data(), ent.size(), c->stream));
+        SLLG_TRY(upload(X.c, d32.data(), d32.size(), c->stream));
+        SLLG_TRY(upload(X.dict64, d64.data(), d64.size(), c->stream));
         x.fmt = FMT_SELL_PACK;
         x.ndict = (uint32_t)tuples.size();
         char buf[48];
         std::snprintf(buf, sizeof buf, "sell64x4-pack-t%u-dict%u", P.na, x.ndict);
-        c->exch_encoding = buf;
+        X.encoding = buf;
     } else {
         std::vector<float> v32((size_t)nc * nent, 0.0f);
         std::vector<double> v64((size_t)nc * nent, 0.0);
@@ -293,16 +308,16 @@ int build_sell(llg_ctx* c) {
                 }
             }
         }
-        SLLG_TRY(upload(c->ex_b, ent.data(), ent.size(), c->stream));
-        SLLG_TRY(upload(c->ex_c, v32.data(), v32.size(), c->stream));
-        SLLG_TRY(upload(c->ex_dict64, v64.data(), v64.size(), c->stream));
+        SLLG_TRY(upload(X.b, ent.data(), ent.size(), c->stream));
+        SLLG_TRY(upload(X.c, v32.data(), v32.size(), c->stream));
+        SLLG_TRY(upload(X.dict64, v64.data(), v64.size(), c->stream));
         x.fmt = FMT_SELL_RAW;
         x.ndict = 0;
-        c->exch_encoding = P.na > 1 ? "sell64-u32-f32-sublattice-major" : "sell64-u32-f32";
+        X.encoding = P.na > 1 ? "sell64-u32-f32-sublattice-major" : "sell64-u32-f32";
     }
-    x.slice_off = c->ex_a.as<uint32_t>();
-    x.ent = c->ex_b.as<uint32_t>();
-    x.dict = c->ex_c.as<float>();
+    x.slice_off = X.a.as<uint32_t>();
+    x.ent = X.b.as<uint32_t>();
+    x.dict = X.c.as<float>();
     x.nent = nent;
     x.ncell = ncell;
     x64 = Exch<double>();
@@ -310,7 +325,7 @@ int build_sell(llg_ctx* c) {
     x64.fmt = x.fmt;
     x64.slice_off = x.slice_off;
     x64.ent = x.ent;
-    x64.dict = c->ex_dict64.as<double>();
+    x64.dict = X.dict64.as<double>();
     x64.ndict = x.ndict;
     x64.nent = nent;
     x64.ncell = ncell;
@@ -320,42 +335,44 @@ int build_sell(llg_ctx* c) {
 }  // namespace
 
 int build_exchange_fp32(llg_ctx* c, const llg_exchange& e) {
+    ExchStore& X = *c->xs;
     SLLG_TRY(validate_and_copy(c, e));
-    const HostExch& h = c->hex;
-    Exch<float>& x = c->ex32;
+    const HostExch& h = X.h;
+    Exch<float>& x = X.ex32;
     fill_common(x, h, c->n);
-    fill_common(c->ex64, h, c->n);
-    c->exch_encoding = "none";
+    fill_common(X.ex64, h, c->n);
+    X.encoding = "none";
     if (h.fmt == LLG_EXCH_CSR) {
         // Sublattice-major device order when the caller declares the basis size (spins
         // ordered cell * n_atoms + t, as the reference orders them).
         const uint32_t na = e.n_atoms;
         if (na > 1 && na <= 16 && c->n % na == 0 && !(c->flags & 0x200u)) {
-            c->perm.na = na;
-            c->perm.ncell = c->n / na;
+            X.perm.na = na;
+            X.perm.ncell = c->n / na;
         } else {
-            c->perm.na = 1;
-            c->perm.ncell = c->n;
+            X.perm.na = 1;
+            X.perm.ncell = c->n;
         }
         SLLG_TRY(build_sell(c));
     } else if (h.fmt == LLG_EXCH_DIA || h.fmt == LLG_EXCH_STENCIL) {
         std::vector<float> v32(h.vals.size());
         for (size_t q = 0; q < v32.size(); ++q) v32[q] = (float)h.vals[q];
-        SLLG_TRY(upload(c->ex_c, v32.data(), v32.size(), c->stream));
-        SLLG_TRY(upload(c->ex_dict64, h.vals.data(), h.vals.size(), c->stream));
+        SLLG_TRY(upload(X.c, v32.data(), v32.size(), c->stream));
+        SLLG_TRY(upload(X.dict64, h.vals.data(), h.vals.size(), c->stream));
         if (h.fmt == LLG_EXCH_DIA) {
-            SLLG_TRY(upload(c->ex_a, h.doff.data(), h.doff.size(), c->stream));
-            x.doff = c->ex64.doff = c->ex_a.as<int32_t>();
-            c->exch_encoding = "dia-f32";
+            SLLG_TRY(upload(X.a, h.doff.data(), h.doff.size(), c->stream));
+            x.doff = X.ex64.doff = X.a.as<int32_t>();
+            X.encoding = "dia-f32";
         } else {
             SLLG_TRY(upload_stencil_tables(c));
-            x.sent = c->ex64.sent = c->ex_a.as<int4>();
-            x.scount = c->ex64.scount = c->ex_b.as<uint32_t>();
-            c->exch_encoding = h.per_row ? "stencil-rowvals-f32" : "stencil-shared-f32";
+            x.sent = X.ex64.sent = X.a.as<int4>();
+            x.scount = X.ex64.scount = X.b.as<uint32_t>();
+            X.encoding = h.per_row ? "stencil-rowvals-f32" : "stencil-shared-f32";
         }
-        x.vals = c->ex_c.as<float>();
-        c->ex64.vals = c->ex_dict64.as<double>();
+        x.vals = X.c.as<float>();
+        X.ex64.vals = X.dict64.as<double>();
     }
+    release_host(X.h);
     return 0;
 }
 

@@ -19,6 +19,10 @@
 #include <hipfft/hipfft.h>
 
 #include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <vector>
 
 #include "kernels.hpp"
@@ -31,7 +35,7 @@ struct DipolarPlan {
     size_t P = 0, Ph = 0;
     DevBuf lin;    // u32 per spin: linear index in the padded grid
     DevBuf mu;     // per spin moment (mu_B), path precision
-    DevBuf Nk;     // [6][Ph] real, path precision
+    std::shared_ptr<DevBuf> Nk;  // [6][Ph] real, path precision (shared by equal geometries)
     DevBuf grid;   // [3][P] real input grid
     DevBuf spec;   // [3][Ph] complex
     DevBuf hgrid;  // [3][P] real output grid
@@ -42,6 +46,20 @@ struct DipolarPlan {
         if (have_inv) hipfftDestroy(inv);
     }
 };
+
+// Process-wide cache of k-space kernels by (device, precision, grid, mesh, lattice constants).
+struct KernelKey {
+    int device;
+    bool f64;
+    int G[3];
+    uint32_t nk[3];
+    double abc[3];
+    bool operator<(const KernelKey& o) const {
+        return std::memcmp(this, &o, sizeof(KernelKey)) < 0;
+    }
+};
+static std::mutex g_nk_mu;
+static std::map<KernelKey, std::weak_ptr<DevBuf>> g_nk_cache;
 
 #define SLLG_FFT(call)                                                                       \
     do {                                                                                     \
@@ -168,8 +186,23 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
         SLLG_TRY(p->mu.alloc(n * rs));
         SLLG_HIP(hipMemcpy(p->mu.p, p->f64 ? (void*)mu64.data() : (void*)mu32.data(), n * rs, hipMemcpyHostToDevice));
     }
-    // kernel: FP64 real space -> FP64 D2Z on the device -> real part in path precision
+    // kernel: FP64 real space -> FP64 D2Z on the device -> real part in path precision.
+    // Handles with the same device, precision and geometry share one copy (replicas).
+    KernelKey key{};
+    key.device = c->device;
+    key.f64 = p->f64;
+    for (int a = 0; a < 3; ++a) {
+        key.G[a] = p->G[a];
+        key.nk[a] = d.nk[a];
+        key.abc[a] = d.abc[a];
+    }
     {
+        std::lock_guard<std::mutex> lk(g_nk_mu);
+        auto it = g_nk_cache.find(key);
+        if (it != g_nk_cache.end()) p->Nk = it->second.lock();
+    }
+    if (!p->Nk) {
+        p->Nk = std::make_shared<DevBuf>();
         std::vector<double> N;
         fill_kernel(p->G, d.dim, d.nk, d.abc, N);
         DevBuf dN, dNk;
@@ -182,14 +215,16 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
         hipfftResult r = hipfftExecD2Z(h, dN.as<double>(), dNk.as<hipfftDoubleComplex>());
         hipfftDestroy(h);
         if (r != HIPFFT_SUCCESS) SLLG_FAIL(LLG_E_DEVICE, "kernel D2Z failed: %d", (int)r);
-        SLLG_TRY(p->Nk.alloc(6 * p->Ph * rs));
+        SLLG_TRY(p->Nk->alloc(6 * p->Ph * rs));
         const size_t m = 6 * p->Ph;
         if (p->f64)
-            k_take_real<double><<<blocks_for(m, kTPB), kTPB>>>(m, dNk.as<hipfftDoubleComplex>(), p->Nk.as<double>());
+            k_take_real<double><<<blocks_for(m, kTPB), kTPB>>>(m, dNk.as<hipfftDoubleComplex>(), p->Nk->as<double>());
         else
-            k_take_real<float><<<blocks_for(m, kTPB), kTPB>>>(m, dNk.as<hipfftDoubleComplex>(), p->Nk.as<float>());
+            k_take_real<float><<<blocks_for(m, kTPB), kTPB>>>(m, dNk.as<hipfftDoubleComplex>(), p->Nk->as<float>());
         SLLG_HIP(hipGetLastError());
         SLLG_HIP(hipDeviceSynchronize());
+        std::lock_guard<std::mutex> lk(g_nk_mu);
+        g_nk_cache[key] = p->Nk;
     }
     SLLG_TRY(p->grid.alloc(3 * p->P * rs));
     SLLG_TRY(p->hgrid.alloc(3 * p->P * rs));
@@ -204,6 +239,13 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
     p->have_inv = true;
     SLLG_FFT(hipfftSetStream(p->fwd, c->stream));
     SLLG_FFT(hipfftSetStream(p->inv, c->stream));
+    return 0;
+}
+
+int dipolar_set_stream(llg_ctx* c) {
+    if (!c->dplan) return 0;
+    SLLG_FFT(hipfftSetStream(c->dplan->fwd, c->stream));
+    SLLG_FFT(hipfftSetStream(c->dplan->inv, c->stream));
     return 0;
 }
 
@@ -222,7 +264,7 @@ int dipolar_refresh(llg_ctx* c, const void* S, void* out) {
                                                        (const double*)S, p->grid.as<double>());
         SLLG_HIP(hipGetLastError());
         SLLG_FFT(hipfftExecD2Z(p->fwd, p->grid.as<double>(), p->spec.as<hipfftDoubleComplex>()));
-        k_dip_mult<double, hipfftDoubleComplex><<<gk, b, 0, c->stream>>>(p->Ph, p->Nk.as<double>(),
+        k_dip_mult<double, hipfftDoubleComplex><<<gk, b, 0, c->stream>>>(p->Ph, p->Nk->as<double>(),
                                                                          p->spec.as<hipfftDoubleComplex>());
         SLLG_HIP(hipGetLastError());
         SLLG_FFT(hipfftExecZ2D(p->inv, p->spec.as<hipfftDoubleComplex>(), p->hgrid.as<double>()));
@@ -233,7 +275,7 @@ int dipolar_refresh(llg_ctx* c, const void* S, void* out) {
                                                       (const float*)S, p->grid.as<float>());
         SLLG_HIP(hipGetLastError());
         SLLG_FFT(hipfftExecR2C(p->fwd, p->grid.as<float>(), p->spec.as<hipfftComplex>()));
-        k_dip_mult<float, hipfftComplex><<<gk, b, 0, c->stream>>>(p->Ph, p->Nk.as<float>(), p->spec.as<hipfftComplex>());
+        k_dip_mult<float, hipfftComplex><<<gk, b, 0, c->stream>>>(p->Ph, p->Nk->as<float>(), p->spec.as<hipfftComplex>());
         SLLG_HIP(hipGetLastError());
         SLLG_FFT(hipfftExecC2R(p->inv, p->spec.as<hipfftComplex>(), p->hgrid.as<float>()));
         k_dip_gather<float><<<gn, b, 0, c->stream>>>(n, p->P, 1.0f / (float)p->P, p->lin.as<uint32_t>(),

@@ -220,6 +220,55 @@ __global__ __launch_bounds__(kTPB) void k_heun_corr(FusedArgs a, const float* __
     for (int c = 0; c < 3; ++c) S[(size_t)c * n + i] = p[c] * r;
 }
 
+// Batched Heun: one launch covers the tiles of every member of a batch (independent spin
+// systems of any size, each with its own couplings, constants, temperature, field, seed
+// and step counter).  Tile -> member is a table lookup made uniform across the block, so
+// the member's arguments are read once per block with scalar loads.
+struct BatchMember {
+    FusedArgs a;
+    float* S;
+    float* Y;
+    float* A;
+    uint32_t tile0, ntiles;
+    uint64_t t0;  // step of the first step of the current launch chunk
+};
+
+template <int PHASE, int FMT, int NC>
+__global__ __launch_bounds__(kTPB) void k_heun_batch(const BatchMember* __restrict__ mem,
+                                                     const uint32_t* __restrict__ tile_member, uint32_t step_off,
+                                                     int xcd) {
+    const uint32_t tile = tile_index(xcd);
+    const uint32_t m = __builtin_amdgcn_readfirstlane(tile_member[tile]);
+    const BatchMember& bm = mem[m];
+    FusedArgs a = bm.a;
+    a.t = bm.t0 + step_off;
+    SLLG_STAGE_DICT(FMT, NC, a)
+    const uint32_t i = (tile - bm.tile0) * kTPB + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t n = a.n;
+    float s[3], f[3];
+    if constexpr (PHASE == 0) {
+        fused_rhs<FMT, NC>(a, i, bm.S, sdict, s, f);
+        float y[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
+        const float r = rsqrtf(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            bm.Y[(size_t)c * n + i] = y[c] * r;
+            bm.A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
+        }
+    } else {
+        fused_rhs<FMT, NC>(a, i, bm.Y, sdict, s, f);
+        float p[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = bm.A[(size_t)c * n + i] + a.half_rdt * f[c];
+        const float r = rsqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) bm.S[(size_t)c * n + i] = p[c] * r;
+    }
+}
+
 // RK4: stage m reads Yin (S for m = 0), k = rdt f(Yin);
 //   m0: ACC = k/6, Yout = S + k/2;  m1: ACC += k/3, Yout = S + k/2;  m2: ACC += k/3, Yout = S + k;
 //   m3: S = N(S + ACC + k/6).
@@ -478,6 +527,54 @@ int fused_detfield(llg_ctx* c, const float* Sp, double* out) {
     const dim3 g(blocks_for(c->n, kTPB)), b(kTPB);
 #define SLLG_L(F, N) k_fused_detfield<F, N><<<g, b, 0, c->stream>>>(a, Sp, out)
     SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+#undef SLLG_L
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------- batches
+
+size_t batch_member_size() { return sizeof(BatchMember); }
+
+int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<uint64_t>& t0, DevBuf& dmem,
+                      DevBuf& dmap, uint32_t& ntiles, hipStream_t s) {
+    std::vector<BatchMember> hm(members.size());
+    std::vector<uint32_t> map;
+    uint32_t tile = 0;
+    for (size_t k = 0; k < members.size(); ++k) {
+        llg_ctx* c = members[k];
+        const StepSched sch{t0[k], nullptr};
+        hm[k].a = fused_args(c, sch);
+        hm[k].S = c->S.as<float>();
+        hm[k].Y = c->Y.as<float>();
+        hm[k].A = c->A.as<float>();
+        hm[k].tile0 = tile;
+        hm[k].ntiles = blocks_for(c->n, kTPB);
+        hm[k].t0 = t0[k];
+        map.insert(map.end(), hm[k].ntiles, (uint32_t)k);
+        tile += hm[k].ntiles;
+    }
+    ntiles = tile;
+    const size_t mb = hm.size() * sizeof(BatchMember), tb = map.size() * sizeof(uint32_t);
+    if (dmem.bytes < mb) SLLG_TRY(dmem.alloc(std::max<size_t>(mb, 64 * sizeof(BatchMember))));
+    if (dmap.bytes < tb) SLLG_TRY(dmap.alloc(std::max<size_t>(tb * 2, 4096)));
+    if (mb) SLLG_HIP(hipMemcpyAsync(dmem.p, hm.data(), mb, hipMemcpyHostToDevice, s));
+    if (tb) SLLG_HIP(hipMemcpyAsync(dmap.p, map.data(), tb, hipMemcpyHostToDevice, s));
+    SLLG_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int batch_heun_launch(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,
+                      uint32_t step_off, int phase, int xcd) {
+    if (ntiles == 0) return 0;
+    const BatchMember* m = static_cast<const BatchMember*>(dmem);
+    const uint32_t* map = static_cast<const uint32_t*>(dmap);
+#define SLLG_L(F, N)                                                                          \
+    do {                                                                                      \
+        if (phase == 0) k_heun_batch<0, F, N><<<ntiles, kTPB, 0, s>>>(m, map, step_off, xcd); \
+        else k_heun_batch<1, F, N><<<ntiles, kTPB, 0, s>>>(m, map, step_off, xcd);            \
+    } while (0)
+    SLLG_DISPATCH_FMT_NC(fmt, nc, SLLG_L);
 #undef SLLG_L
     SLLG_HIP(hipGetLastError());
     return 0;

@@ -1,0 +1,129 @@
+/*
+ * llg_batch.h - many independent spin systems advanced together on one GPU, and the
+ * continuous-batching request loop built on them.
+ *
+ * No reference counterpart: the reference integrates one system per process
+ * (src/cuda.cu:278-319).  A member of a batch is an FP32 handle (llg_desc with
+ * precision LLG_FP32, scheme LLG_HEUN, counter noise); its couplings, constants,
+ * temperature, field, seed and step counter are its own.  One kernel launch per half
+ * step covers the tiles of every member, and with LLG_BATCH_GRAPH a chunk of steps is
+ * captured once into a hipGraph and replayed.  Members are admitted against an HBM
+ * budget and can join or leave between chunks; a member's trajectory does not depend on
+ * which other members share its launches (counter noise keyed by its own seed/stream,
+ * no cross-member reductions).
+ */
+#ifndef LLG_BATCH_H
+#define LLG_BATCH_H
+#include <stdint.h>
+
+#include "llg_capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LLG_BATCH_GRAPH 0x1u /* replay captured hipGraphs of whole chunks */
+
+typedef struct llg_batch llg_batch;
+
+typedef struct llg_batch_info {
+    uint32_t members;      /* live members */
+    uint32_t tiles;        /* 256-spin tiles per half-step launch */
+    uint64_t spins;        /* total spins */
+    uint64_t device_bytes; /* bytes held by live members */
+    uint64_t budget_bytes;
+    uint32_t graphs_built; /* hipGraph instantiations so far */
+    uint32_t pad_;
+} llg_batch_info;
+
+/* Upper bound of the device bytes llg_create would allocate for d. */
+uint64_t llg_estimate_bytes(const llg_desc* d);
+
+int llg_batch_create(int device, uint64_t hbm_budget_bytes, uint32_t flags, llg_batch** out);
+void llg_batch_destroy(llg_batch* b);
+
+/* Admit a system with initial spins (SoA, n doubles each) at step t0.  Returns
+ * LLG_E_NOMEM without allocating when the estimate exceeds the remaining budget. */
+int llg_batch_add(llg_batch* b, const llg_desc* d, const double* sx, const double* sy, const double* sz,
+                  double T, const double B[3], uint64_t t0, int* member);
+/* Admit a member that shares the couplings of member `proto_member` (same n; d->exch
+ * ignored); only its state counts against the budget. */
+int llg_batch_add_like(llg_batch* b, const llg_desc* d, int proto_member, const double* sx, const double* sy,
+                       const double* sz, double T, const double B[3], uint64_t t0, int* member);
+int llg_batch_remove(llg_batch* b, int member);
+int llg_batch_set_temperature(llg_batch* b, int member, double T);
+int llg_batch_set_applied(llg_batch* b, int member, const double B[3]);
+
+/* Advance every member nsteps steps (each from its own step counter). */
+int llg_batch_step(llg_batch* b, uint32_t nsteps);
+
+/* FP64 sublattice magnetisation of every member listed, m[k*nsub_max*3 ...]: each member
+ * writes n_sublattice*3 values at offset k*stride. */
+int llg_batch_reduce(llg_batch* b, const int* members, uint32_t count, uint32_t stride, double* m);
+int llg_batch_get_spins(llg_batch* b, int member, double* sx, double* sy, double* sz);
+int llg_batch_get_step(llg_batch* b, int member, uint64_t* t);
+int llg_batch_get_info(llg_batch* b, llg_batch_info* info);
+
+/* ---------------------------------------------------------------- request loop */
+
+/* Request types: the semantics of the reference drivers (src/mvt.cpp, src/ramp_field.cpp):
+ *   LLG_REQ_MT    fixed T and B: run `steps`, sample <M> every `update` steps;
+ *   LLG_REQ_RAMP  field ramp B(t) = B0 + (B1 - B0) t/steps, B updated only on sampling
+ *                 steps (src/ramp_field.cpp:209-224), T fixed. */
+enum { LLG_REQ_MT = 0, LLG_REQ_RAMP = 1 };
+
+typedef struct llg_request {
+    uint32_t type;
+    uint32_t L;        /* simple-cubic L x L x L, periodic */
+    uint32_t steps;
+    uint32_t update;   /* sampling interval */
+    double T;
+    double J;          /* nearest-neighbour exchange (J) */
+    double K;          /* uniaxial anisotropy along z (J) */
+    double mu;         /* moment (mu_B) */
+    double lambda;
+    double dt;
+    double B0[3], B1[3];
+    double arrival_s;  /* arrival time relative to the start of the run */
+    uint64_t seed;
+} llg_request;
+
+typedef struct llg_result {
+    double arrival_s, admit_s, done_s, latency_s;
+    double m_final[3];  /* <M> at the last sample */
+    double m_mean[3];   /* mean of the samples of the second half of the run */
+    uint32_t samples;
+    uint32_t pad_;
+} llg_result;
+
+typedef struct llg_serve_config {
+    int device;
+    uint32_t flags;           /* LLG_BATCH_GRAPH */
+    uint64_t hbm_budget_bytes;
+    uint32_t max_members;     /* 0: unlimited */
+    uint32_t chunk;           /* steps per batched launch chunk; 0: the smallest update interval */
+    int sequential;           /* 1: no batching - one request at a time in arrival order */
+    int precision;            /* LLG_FP32 (batched) or LLG_FP64_PLAIN (sequential only) */
+} llg_serve_config;
+
+typedef struct llg_serve_stats {
+    double wall_s;
+    double solves_per_s;
+    double p50_latency_s, p99_latency_s, mean_latency_s;
+    double spin_updates_per_s;
+    uint32_t completed, max_live;
+    uint64_t spin_updates;
+} llg_serve_stats;
+
+/* Run the request stream in real time: requests are admitted when they have arrived
+ * and memory is free, advance in shared launches, and leave when done.  results[i]
+ * belongs to reqs[i]; m_series (may be NULL) receives for request i its samples
+ * (3 doubles each) at series_offset[i] (computed by the caller as the prefix sum of
+ * ceil(steps/update)). */
+int llg_serve_run(const llg_serve_config* cfg, const llg_request* reqs, uint32_t nreq, llg_result* results,
+                  double* m_series, const uint64_t* series_offset, llg_serve_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -188,6 +188,10 @@ void llg_desc_init(llg_desc* d);
  * copied; they may be freed after llg_create returns. */
 int llg_create(const llg_desc* d, llg_ctx** out);
 void llg_destroy(llg_ctx* ctx);
+/* Create a handle that shares the device-resident couplings of `proto` (same n, precision
+ * and device; d->exch is ignored).  The couplings live until the last sharing handle is
+ * destroyed.  No reference counterpart: for replicas and serving requests of one lattice. */
+int llg_create_like(const llg_desc* d, llg_ctx* proto, llg_ctx** out);
 
 /* Spin state, SoA host arrays of n doubles.  set_spins normalises nothing: the
  * caller passes unit vectors (zero-length vectors are rejected). */
