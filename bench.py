@@ -91,6 +91,7 @@ def headline(args):
     g = sllg.Context(sysd)
     g.set_spins(s0)
     g.set_temperature(T)
+    tune = None if args.no_tune else g.autotune(20)
     g.step(0, args.warmup)
     sec, wall = time_steps(g, torch, args.warmup, args.steps)
     value = sysd.n * args.steps / sec
@@ -105,6 +106,31 @@ def headline(args):
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     g.close()
+
+    # Same system with every slice's entries stored (no sharing of repeated slices): the
+    # coupling stream then comes from HBM on every launch.
+    unshared = None
+    if not args.no_unshared:
+        sysd.flags = sllg.FLAG_NO_SLICE_SHARING
+        u = sllg.Context(sysd)
+        u.set_spins(s0)
+        u.set_temperature(T)
+        if tune is not None:
+            u.autotune(20)
+        u.step(0, args.warmup)
+        usteps = max(50, args.steps // 4)
+        usec, _ = time_steps(u, torch, args.warmup, usteps)
+        uprof = u.profile_kernels(args.warmup + usteps, 20)
+        uinfo = u.info()
+        u_ms = uprof.get("heun_corrector", max(uprof.values()))
+        u_bytes = uinfo["dominant_kernel_bytes_per_spin"] * sysd.n
+        u_ach = u_bytes / (u_ms / 1e3) / 1e9
+        unshared = {"value": sysd.n * usteps / usec, "ms_per_step": usec * 1e3 / usteps,
+                    "exch_encoding": uinfo["exch_encoding"], "kernel_ms": u_ms,
+                    "algorithmic_bytes_per_spin": uinfo["dominant_kernel_bytes_per_spin"],
+                    "achieved_GBs": u_ach, "frac": u_ach / PEAK_HBM_GBS}
+        u.close()
+        sysd.flags = 0
 
     # FP64 plain path (reference structure) on the same lattice: CSR xx, yy, zz per nnz.
     sysp, sp0 = W.fcc_3shell(L=L, precision="fp64", ncomp=3)
@@ -144,6 +170,8 @@ def headline(args):
         "fp64_plain": {"value": plain, "unit": "spin-updates/s", "ms_per_step": psec * 1e3 / psteps,
                        "fp32_over_fp64_plain": value / plain, "kernels_ms": pprof},
         "kernels_ms": prof,
+        "launch_tune": tune,
+        "unshared_encoding": unshared,
         "wall_s_timed": wall,
     }
     print(json.dumps(line), flush=True)
@@ -313,6 +341,9 @@ def main():
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--T", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="headline: skip the launch-shape autotuner")
+    ap.add_argument("--no-unshared", action="store_true",
+                    help="headline: skip the comparison run with repeated slices stored separately")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--err-steps", type=int, default=2000, help="rk4: steps for the FP32-vs-FP64 error")
     ap.add_argument("--replicas", type=int, default=64)

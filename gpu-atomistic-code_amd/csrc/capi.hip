@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -737,3 +738,119 @@ int create_with_store(const llg_desc* d, const std::shared_ptr<ExchStore>& store
     return create_impl(d, store, out);
 }
 }  // namespace sllg
+
+// ---------------------------------------------------------------- launch-shape autotuner
+
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<int, int>> g_tune_cache;
+
+std::string tune_key(const llg_ctx* c) {
+    hipDeviceProp_t p{};
+    (void)hipGetDeviceProperties(&p, c->device);
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s|%d|%d|%u|%llu|%u", p.gcnArchName, c->ex32.fmt, c->ex32.ncomp, c->n,
+                  (unsigned long long)c->ex32.nent, c->perm.na);
+    return buf;
+}
+
+void tune_file_load() {
+    const char* f = std::getenv("SLLG_TUNE_FILE");
+    if (!f) return;
+    FILE* fp = std::fopen(f, "r");
+    if (!fp) return;
+    char key[256];
+    int tpb, xcd;
+    while (std::fscanf(fp, "%255s %d %d", key, &tpb, &xcd) == 3) g_tune_cache[key] = {tpb, xcd};
+    std::fclose(fp);
+}
+
+void tune_file_append(const std::string& key, int tpb, int xcd) {
+    const char* f = std::getenv("SLLG_TUNE_FILE");
+    if (!f) return;
+    FILE* fp = std::fopen(f, "a");
+    if (!fp) return;
+    std::fprintf(fp, "%s %d %d\n", key.c_str(), tpb, xcd);
+    std::fclose(fp);
+}
+}  // namespace
+
+extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd) {
+    if (!c || (tpb != 256 && tpb != 512 && tpb != 1024)) SLLG_FAIL(LLG_E_INVALID, "block size must be 256, 512 or 1024");
+    c->tune_tpb = tpb;
+    c->tune_xcd = xcd ? 1 : 0;
+    return 0;
+}
+
+extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
+    if (!c || !out || nsteps == 0) SLLG_FAIL(LLG_E_INVALID, "bad argument");
+    std::memset(out, 0, sizeof(*out));
+    out->tpb = c->tune_tpb;
+    out->xcd = c->tune_xcd;
+    if (c->prec != LLG_FP32 || c->scheme != LLG_HEUN) return 0;
+    SLLG_HIP(hipSetDevice(c->device));
+    const std::string key = tune_key(c);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        static bool loaded = false;
+        if (!loaded) {
+            tune_file_load();
+            loaded = true;
+        }
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            c->tune_tpb = out->tpb = it->second.first;
+            c->tune_xcd = out->xcd = it->second.second;
+            out->from_cache = 1;
+            return 0;
+        }
+    }
+    // keep the state: time on a copy of the spins, restore afterwards
+    DevBuf save;
+    SLLG_TRY(save.alloc(c->S.bytes));
+    SLLG_HIP(hipMemcpyAsync(save.p, c->S.p, c->S.bytes, hipMemcpyDeviceToDevice, c->stream));
+    const int tpbs[3] = {256, 512, 1024};
+    double best = 1e300;
+    int btpb = 256, bxcd = 1;
+    hipEvent_t e0, e1;
+    SLLG_HIP(hipEventCreate(&e0));
+    SLLG_HIP(hipEventCreate(&e1));
+    const int noise = c->noise_mode;
+    c->noise_mode = LLG_NOISE_COUNTER;  // no host queue traffic while timing
+    int rc = 0;
+    for (int ti = 0; ti < 3 && !rc; ++ti)
+        for (int xcd = 1; xcd >= 0 && !rc; --xcd) {
+            c->tune_tpb = tpbs[ti];
+            c->tune_xcd = xcd;
+            rc = do_steps(c, 0, 2);
+            if (rc) break;
+            (void)hipEventRecord(e0, c->stream);
+            rc = do_steps(c, 2, nsteps);
+            (void)hipEventRecord(e1, c->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            const double per = ms / nsteps;
+            if (tpbs[ti] == 256 && xcd == 1) out->ms_per_step_default = per;
+            out->candidates++;
+            if (per < best) {
+                best = per;
+                btpb = tpbs[ti];
+                bxcd = xcd;
+            }
+        }
+    c->noise_mode = noise;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SLLG_HIP(hipMemcpyAsync(c->S.p, save.p, c->S.bytes, hipMemcpyDeviceToDevice, c->stream));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    c->det_valid = false;
+    if (rc) return rc;
+    c->tune_tpb = out->tpb = btpb;
+    c->tune_xcd = out->xcd = bxcd;
+    out->ms_per_step = best;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = {btpb, bxcd};
+    tune_file_append(key, btpb, bxcd);
+    return 0;
+}

@@ -5,9 +5,10 @@
 //                     DET, XI (noise), per-spin sigma/llgpf/lambda/k1u/k1udir, CSR with
 //                     FP64 values per nnz - the reference GPU data structures, SoA.
 //   FP32 fused path : S, Y, A (= S + rdt/2 f(S)), per-species constants, a compact
-//                     exchange encoding (sliced-ELL with packed 16-bit column delta +
-//                     16-bit coupling-dictionary index, or raw u32/FP32 when it does not
-//                     fit; periodic stencil; DIA), noise regenerated in registers.
+//                     exchange encoding (sliced ELL whose u32 entry packs the neighbour's
+//                     row offset with a coupling-dictionary index, repeated slices stored
+//                     once; raw u32/FP32 when it does not fit; periodic stencil; DIA),
+//                     noise regenerated in registers.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,7 +31,7 @@ enum ExchFmt : int {
     FMT_DIA = 1,
     FMT_CSR = 2,        // FP64 plain: CSR u32 + FP64 values
     FMT_STENCIL = 3,    // periodic stencil (shared or per-row values)
-    FMT_SELL_PACK = 4,  // FP32: sliced ELL, u32 = (int16 col delta << 16) | u16 dict index
+    FMT_SELL_PACK = 4,  // FP32: sliced ELL, u32 = (row offset << dbits) | dictionary index
     FMT_SELL_RAW = 5    // FP32: sliced ELL, u32 col + FP32 values [ncomp][entries]
 };
 
@@ -64,12 +65,15 @@ template <class V> struct Exch {
     const int4* sent = nullptr;       // [na][sk] (dx, dy, dz, t')
     const uint32_t* scount = nullptr; // [na]
     // sliced ELL
-    const uint32_t* slice_off = nullptr;  // [nslices+1] offsets in entries/64
+    // [nslices] {first entry / 64, entries per row}; slices with identical packed entries
+    // point at one stored copy
+    const uint2* slice = nullptr;
     const uint32_t* ent = nullptr;        // packed entries or columns, [..][64]
     const V* dict = nullptr;              // [ndict][ncomp] (SELL_PACK) or raw values
     uint32_t ndict = 0;
     uint64_t nent = 0;                    // total entries (SELL_RAW value stride)
     uint32_t ncell = 0;                   // rows per sublattice block of the device order
+    uint32_t dbits = 0;                   // dictionary-index bits of a packed entry
 };
 
 // Device order of the FP32 sliced-ELL path.  The reference orders spins basis-atom
@@ -83,18 +87,19 @@ struct Perm {
     __host__ __device__ uint32_t to_ref(uint32_t d) const { return (d % ncell) * na + d / ncell; }
 };
 
-// Packed sliced-ELL entry: bits 31..28 target sublattice t', 27..16 dictionary index,
-// 15..0 int16 cell delta (wrapped modulo ncell).
-__host__ __device__ __forceinline__ uint32_t pack_word(uint32_t t2, uint32_t idx, int32_t dcell) {
-    return (t2 << 28) | ((idx & 0xFFFu) << 16) | (uint32_t)(uint16_t)(int16_t)dcell;
+// Packed sliced-ELL entry: the signed device-row offset of the neighbour (j - i, exact, no
+// wrap) in the high 32 - dbits bits and the coupling-dictionary index in the low dbits bits.
+// The offset is the same for every row whose neighbourhood does not cross a periodic
+// boundary, so slices away from the boundary planes encode identically.
+__host__ __device__ __forceinline__ uint32_t pack_word(int32_t drow, uint32_t idx, uint32_t dbits) {
+    return ((uint32_t)drow << dbits) | idx;
 }
-__host__ __device__ __forceinline__ uint32_t pack_dict(uint32_t w) { return (w >> 16) & 0xFFFu; }
-// Neighbour device row of entry w for a row in cell `cell`.
-__host__ __device__ __forceinline__ uint32_t pack_nb(uint32_t w, uint32_t cell, uint32_t ncell) {
-    int32_t cj = (int32_t)cell + (int32_t)(int16_t)(w & 0xFFFFu);
-    cj += (cj < 0) ? (int32_t)ncell : 0;
-    cj -= (cj >= (int32_t)ncell) ? (int32_t)ncell : 0;
-    return (w >> 28) * ncell + (uint32_t)cj;
+__host__ __device__ __forceinline__ uint32_t pack_dict(uint32_t w, uint32_t dbits) {
+    return w & ((1u << dbits) - 1u);
+}
+// Neighbour device row of entry w in row i.
+__host__ __device__ __forceinline__ uint32_t pack_nb(uint32_t w, uint32_t i, uint32_t dbits) {
+    return i + (uint32_t)((int32_t)w >> dbits);
 }
 
 struct DipolarPlan;  // k_dipolar.hip
@@ -154,6 +159,8 @@ struct llg_ctx {
     bool multi_species = false;
     sllg::Perm perm;  // device order of the FP32 sliced-ELL path (identity elsewhere)
     bool permuted() const { return perm.na > 1; }
+    int tune_tpb = 256;  // block size of the FP32 Heun kernels (llg_autotune)
+    int tune_xcd = 1;    // XCD-aware tile remap on/off (llg_autotune)
 
     // device state
     sllg::DevBuf S, Y, A, H, FN, DET, XI, STG, STG0, DEM;  // SoA 3n (type per path)

@@ -5,13 +5,14 @@
 // DIA [ncomp][ndiag][n]; periodic stencil).
 //
 // FP32 path: CSR becomes sliced ELL with 64-row slices (one wavefront) so that every
-// load of the coupling stream is a contiguous wave access (1 KB per uint4 load).  When every column
-// lies within +-32767 of its row (modulo n, i.e. across the periodic wrap) and the
-// distinct coupling tuples fit the LDS dictionary (kDictLds floats), an entry is one u32:
-// int16 column delta << 16 | u16 index into the coupling dictionary (M_CSR = 4 B per
-// nnz; the dictionary is staged in LDS by every block).  Otherwise entries are a u32 column plus
-// ncomp FP32 values (M = 4 + 4 ncomp B per nnz).  Rows keep their stored order, so the
-// per-row summation order of the reference CSR (src/cufields.cu:192-200) is kept.
+// load of the coupling stream is a contiguous wave access (1 KB per uint4 load).  When the
+// distinct coupling tuples fit the LDS dictionary (kDictLds floats) and every neighbour's
+// row offset fits the bits the dictionary index leaves, an entry is one u32: row offset
+// << dbits | dictionary index (4 B per nnz; the dictionary is staged in LDS by every
+// block), and slices whose entries repeat an earlier slice's are stored once.  Otherwise
+// entries are a u32 column plus ncomp FP32 values (M = 4 + 4 ncomp B per nnz).  Rows keep
+// their stored order, so the per-row summation order of the reference CSR
+// (src/cufields.cu:192-200) is kept.
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -179,7 +180,7 @@ struct TupleHash {
 // CSR -> sliced ELL.  Packed layout: per slice, entries in chunks of 4 per lane,
 // [chunk][lane][4], so one uint4 load per lane reads 4 entries and a wave reads 1 KB
 // contiguous; slice lengths are rounded up to a multiple of 4 with zero-coupling entries
-// (delta 0, dictionary tuple 0.0), which add exactly 0 to the field.  Raw layout:
+// (offset 0, dictionary tuple 0.0), which add exactly 0 to the field.  Raw layout:
 // [entry][lane] u32 columns and [ncomp][entry][lane] FP32 values.
 int build_sell(llg_ctx* c) {
     ExchStore& X = *c->xs;
@@ -198,23 +199,9 @@ int build_sell(llg_ctx* c) {
             const uint32_t r = ref_of[d];
             mx[s] = std::max(mx[s], h.xadj[r + 1] - h.xadj[r]);
         }
-    // Packed form: target sublattice, cell delta within +-32767 (modulo ncell), few tuples.
-    bool pack = P.na <= 16;
-    std::vector<int16_t> delta(h.nnz);
-    std::vector<uint8_t> tgt(h.nnz);
-    const int64_t half = (int64_t)ncell / 2;
-    for (uint32_t r = 0; r < n && pack; ++r) {
-        const uint32_t cr = P.to_dev(r) % ncell;
-        for (uint32_t q = h.xadj[r]; q < h.xadj[r + 1]; ++q) {
-            const uint32_t dj = P.to_dev(h.adj[q]);
-            int64_t d = (int64_t)(dj % ncell) - (int64_t)cr;
-            if (d > half) d -= ncell;
-            if (d < -half) d += ncell;
-            if (d < -32768 || d > 32767) { pack = false; break; }
-            delta[q] = (int16_t)d;
-            tgt[q] = (uint8_t)(dj / ncell);
-        }
-    }
+    // Packed form: few distinct coupling tuples (the LDS dictionary) and row offsets that
+    // fit the bits the dictionary index leaves; 12 n bytes must fit one buffer resource.
+    bool pack = (uint64_t)n * 12u < (1ull << 31);
     std::unordered_map<TupleKey, uint32_t, TupleHash> dict;
     std::vector<TupleKey> tuples;
     std::vector<uint32_t> idx;
@@ -247,29 +234,78 @@ int build_sell(llg_ctx* c) {
             last_id = id;
         }
     }
-    std::vector<uint32_t> slice_off(nsl + 1, 0);
-    for (uint32_t s = 0; s < nsl; ++s) slice_off[s + 1] = slice_off[s] + (pack ? (mx[s] + 3u) / 4u * 4u : mx[s]);
-    const uint64_t nent = (uint64_t)slice_off[nsl] * kSlice;
+    uint32_t dbits = 1;
+    if (pack) {
+        while ((1u << dbits) < tuples.size()) ++dbits;
+        const int64_t lim = (int64_t)1 << (31 - dbits);
+        for (uint32_t r = 0; r < n && pack; ++r) {
+            const int64_t dr = P.to_dev(r);
+            for (uint32_t q = h.xadj[r]; q < h.xadj[r + 1]; ++q) {
+                const int64_t off = (int64_t)P.to_dev(h.adj[q]) - dr;
+                if (off < -lim || off >= lim) { pack = false; break; }
+            }
+        }
+    }
+    // slice table {first entry / 64, entries per row}
+    std::vector<uint2> slice(nsl);
+    uint64_t tot = 0;
+    for (uint32_t s = 0; s < nsl; ++s) {
+        slice[s].y = pack ? (mx[s] + 3u) / 4u * 4u : mx[s];
+        slice[s].x = (uint32_t)tot;
+        tot += slice[s].y;
+    }
+    if (tot > 0xFFFFFFFFull) SLLG_FAIL(LLG_E_INVALID, "exchange too large for the sliced-ELL encoding");
     Exch<float>& x = X.ex32;
     Exch<double>& x64 = X.ex64;
-    SLLG_TRY(upload(X.a, slice_off.data(), slice_off.size(), c->stream));
-    std::vector<uint32_t> ent(nent, 0);
+    std::vector<uint32_t> ent;
+    uint64_t nent = tot * kSlice;
     if (pack) {
+        // Packed entries are relative (sublattice, cell delta, coupling index), so slices of a
+        // periodic lattice away from the boundary planes come out identical: each distinct
+        // slice block is stored once and the slice table points repeats at it.
+        const bool share = !(c->flags & LLG_FLAG_NO_SLICE_SHARING);
+        std::unordered_map<uint64_t, std::vector<uint32_t>> seen;  // block hash -> stored slices
+        std::vector<uint32_t> blk;
+        ent.reserve(share ? 1u << 16 : nent);
+        uint64_t used = 0;
+        uint32_t stored = 0;
         for (uint32_t s = 0; s < nsl; ++s) {
-            const uint32_t len = slice_off[s + 1] - slice_off[s];
+            const uint32_t len = slice[s].y;
+            blk.assign((size_t)len * kSlice, 0u);
             for (uint32_t l = 0; l < kSlice; ++l) {
                 const uint32_t d = s * kSlice + l;
                 const uint32_t r = d < n ? ref_of[d] : 0;
                 for (uint32_t q = 0; q < len; ++q) {
-                    uint32_t w = pack_word(d < n ? d / ncell : 0, 0, 0);  // zero coupling on itself
+                    uint32_t w = pack_word(0, 0, dbits);  // zero coupling on itself
                     if (d < n && q < h.xadj[r + 1] - h.xadj[r]) {
                         const uint32_t qq = h.xadj[r] + q;
-                        w = pack_word(tgt[qq], idx[qq], delta[qq]);
+                        w = pack_word((int32_t)((int64_t)P.to_dev(h.adj[qq]) - (int64_t)d), idx[qq], dbits);
                     }
-                    ent[(size_t)slice_off[s] * kSlice + (size_t)(q >> 2) * (kSlice * 4) + l * 4 + (q & 3)] = w;
+                    blk[(size_t)(q >> 2) * (kSlice * 4) + l * 4 + (q & 3)] = w;
                 }
             }
+            uint64_t hsh = 1469598103934665603ull ^ len;  // FNV-1a over the block's words
+            for (uint32_t w : blk) hsh = (hsh ^ w) * 1099511628211ull;
+            bool found = false;
+            if (share) {
+                auto& cand = seen[hsh];
+                for (uint32_t o : cand)
+                    if (slice[o].y == len &&
+                        std::memcmp(ent.data() + (size_t)slice[o].x * kSlice, blk.data(), blk.size() * 4) == 0) {
+                        slice[s].x = slice[o].x;
+                        found = true;
+                        break;
+                    }
+                if (!found) cand.push_back(s);
+            }
+            if (!found) {
+                slice[s].x = (uint32_t)used;
+                ent.insert(ent.end(), blk.begin(), blk.end());
+                used += len;
+                ++stored;
+            }
         }
+        nent = used * kSlice;
         std::vector<float> d32(tuples.size() * nc);
         std::vector<double> d64(tuples.size() * nc);
         for (size_t t = 0; t < tuples.size(); ++t)
@@ -282,19 +318,20 @@ int build_sell(llg_ctx* c) {
         SLLG_TRY(upload(X.dict64, d64.data(), d64.size(), c->stream));
         x.fmt = FMT_SELL_PACK;
         x.ndict = (uint32_t)tuples.size();
-        char buf[48];
-        std::snprintf(buf, sizeof buf, "sell64x4-pack-t%u-dict%u", P.na, x.ndict);
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "sell64x4-pack-t%u-dict%u-slices%uof%u", P.na, x.ndict, stored, nsl);
         X.encoding = buf;
     } else {
+        ent.assign(nent, 0u);
         std::vector<float> v32((size_t)nc * nent, 0.0f);
         std::vector<double> v64((size_t)nc * nent, 0.0);
         for (uint32_t s = 0; s < nsl; ++s) {
-            const uint32_t len = slice_off[s + 1] - slice_off[s];
+            const uint32_t len = slice[s].y;
             for (uint32_t l = 0; l < kSlice; ++l) {
                 const uint32_t d = s * kSlice + l;
                 const uint32_t r = d < n ? ref_of[d] : 0;
                 for (uint32_t q = 0; q < len; ++q) {
-                    const size_t o = ((size_t)slice_off[s] + q) * kSlice + l;
+                    const size_t o = ((size_t)slice[s].x + q) * kSlice + l;
                     if (d < n && q < h.xadj[r + 1] - h.xadj[r]) {
                         const uint32_t qq = h.xadj[r] + q;
                         ent[o] = P.to_dev(h.adj[qq]);
@@ -315,20 +352,23 @@ int build_sell(llg_ctx* c) {
         x.ndict = 0;
         X.encoding = P.na > 1 ? "sell64-u32-f32-sublattice-major" : "sell64-u32-f32";
     }
-    x.slice_off = X.a.as<uint32_t>();
+    SLLG_TRY(upload(X.a, slice.data(), slice.size(), c->stream));
+    x.slice = X.a.as<uint2>();
     x.ent = X.b.as<uint32_t>();
     x.dict = X.c.as<float>();
     x.nent = nent;
     x.ncell = ncell;
+    x.dbits = pack ? dbits : 0;
     x64 = Exch<double>();
     fill_common(x64, h, n);
     x64.fmt = x.fmt;
-    x64.slice_off = x.slice_off;
+    x64.slice = x.slice;
     x64.ent = x.ent;
     x64.dict = X.dict64.as<double>();
     x64.ndict = x.ndict;
     x64.nent = nent;
     x64.ncell = ncell;
+    x64.dbits = x.dbits;
     return 0;
 }
 
@@ -346,7 +386,7 @@ int build_exchange_fp32(llg_ctx* c, const llg_exchange& e) {
         // Sublattice-major device order when the caller declares the basis size (spins
         // ordered cell * n_atoms + t, as the reference orders them).
         const uint32_t na = e.n_atoms;
-        if (na > 1 && na <= 16 && c->n % na == 0 && !(c->flags & 0x200u)) {
+        if (na > 1 && na <= 16 && c->n % na == 0 && !(c->flags & LLG_FLAG_NO_SUBLATTICE_ORDER)) {
             X.perm.na = na;
             X.perm.ncell = c->n / na;
         } else {

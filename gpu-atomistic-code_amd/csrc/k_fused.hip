@@ -18,25 +18,25 @@
 namespace sllg {
 
 // Sliced ELL, scalar reader (any precision; dictionary read from global memory).  Slice
-// = 64 consecutive rows (one wavefront).  Packed entry: high 16 bits = column delta
-// (j - i, wrapped mod n into int16), low 16 bits = index into the coupling dictionary
-// [ndict][NC]; entries of a slice are stored [chunk of 4][lane][4].  Raw entry: u32
-// column, values [NC][nent], both [entry][lane].
+// = 64 consecutive device rows (one wavefront), located by the slice table {first, len}.
+// Packed entry (pack_word): neighbour row offset and coupling-dictionary index
+// [ndict][NC]; a slice's entries are stored [chunk of 4][lane][4] and slices with
+// identical entries share one copy.  Raw entry: u32 column, values [NC][nent], both
+// [entry][lane].
 template <int NC, class R, class V, class SpinT>
 __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, const SpinT* __restrict__ S, R dot[3]) {
     const uint32_t n = e.n;
     dot[0] = dot[1] = dot[2] = R(0);
     const uint32_t sl = i >> 6, lane = i & 63u;
-    const uint32_t q0 = e.slice_off[sl];
-    const uint32_t len = e.slice_off[sl + 1] - q0;
+    const uint2 sd = e.slice[sl];
+    const uint32_t q0 = sd.x, len = sd.y;
     R v[NC];
     if (e.fmt == FMT_SELL_PACK) {
         const uint32_t* __restrict__ ent = e.ent + (size_t)q0 * kSlice + lane * 4;
-        const uint32_t cell = i % e.ncell;
         for (uint32_t q = 0; q < len; ++q) {
             const uint32_t w = ent[(size_t)(q >> 2) * (kSlice * 4) + (q & 3)];
-            const uint32_t j = pack_nb(w, cell, e.ncell);
-            const V* dv = e.dict + (size_t)pack_dict(w) * NC;
+            const uint32_t j = pack_nb(w, i, e.dbits);
+            const V* dv = e.dict + (size_t)pack_dict(w, e.dbits) * NC;
 #pragma unroll
             for (int k = 0; k < NC; ++k) v[k] = (R)dv[k];
             const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
@@ -56,15 +56,32 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
     }
 }
 
+// Spins as one raw buffer resource: a neighbour's three components are three buffer loads
+// sharing one 32-bit byte offset (scalar component offsets 4n, 8n), so no 64-bit address
+// arithmetic per entry; a load outside the 12n bytes returns 0 instead of faulting.
+struct SpinBuf {
+    __amdgpu_buffer_rsrc_t r;
+    uint32_t o1, o2;
+};
+__device__ __forceinline__ SpinBuf spin_buf(const float* S, uint32_t n) {
+    SpinBuf b;
+    b.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(S), 0, (int)(12u * n), 0x00020000);
+    b.o1 = 4u * n;
+    b.o2 = 8u * n;
+    return b;
+}
+
 // One packed entry: decode, gather the neighbour, read its coupling from the LDS dictionary.
 template <int NC>
-__device__ __forceinline__ void pack_entry(uint32_t w, uint32_t cell, uint32_t ncell, uint32_t n,
-                                           const float* __restrict__ S, const float* sdict, float dot[3]) {
-    const uint32_t j = pack_nb(w, cell, ncell);
-    const float s[3] = {S[j], S[n + j], S[2 * (size_t)n + j]};
+__device__ __forceinline__ void pack_entry(uint32_t w, uint32_t i, uint32_t dbits, const SpinBuf& sb,
+                                           const float* sdict, float dot[3]) {
+    const uint32_t off = pack_nb(w, i, dbits) << 2;
+    const float s[3] = {__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, off, 0, 0)),
+                        __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, off, sb.o1, 0)),
+                        __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, off, sb.o2, 0))};
     float v[NC];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) v[k] = sdict[pack_dict(w) * NC + k];
+    for (int k = 0; k < NC; ++k) v[k] = sdict[pack_dict(w, dbits) * NC + k];
     accum<NC, float>(v, s, dot);
 }
 
@@ -73,21 +90,21 @@ __device__ __forceinline__ void pack_entry(uint32_t w, uint32_t cell, uint32_t n
 template <int NC>
 __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_t i, const float* __restrict__ S,
                                                    const float* sdict, float dot[3]) {
-    const uint32_t n = e.n;
     dot[0] = dot[1] = dot[2] = 0.0f;
     const uint32_t sl = i >> 6, lane = i & 63u;
-    const uint32_t q0 = e.slice_off[sl];
-    const uint32_t nch = (e.slice_off[sl + 1] - q0) >> 2;
+    const uint2 sd = e.slice[sl];
+    const uint32_t q0 = sd.x, nch = sd.y >> 2;
     if (nch == 0) return;
     const uint4* __restrict__ p = reinterpret_cast<const uint4*>(e.ent + (size_t)q0 * kSlice) + lane;
-    const uint32_t ncell = e.ncell, cell = i % ncell;
+    const uint32_t db = e.dbits;
+    const SpinBuf sb = spin_buf(S, e.n);
 #pragma unroll 2
     for (uint32_t c = 0; c < nch; ++c) {
         const uint4 cur = p[(size_t)c * kSlice];
-        pack_entry<NC>(cur.x, cell, ncell, n, S, sdict, dot);
-        pack_entry<NC>(cur.y, cell, ncell, n, S, sdict, dot);
-        pack_entry<NC>(cur.z, cell, ncell, n, S, sdict, dot);
-        pack_entry<NC>(cur.w, cell, ncell, n, S, sdict, dot);
+        pack_entry<NC>(cur.x, i, db, sb, sdict, dot);
+        pack_entry<NC>(cur.y, i, db, sb, sdict, dot);
+        pack_entry<NC>(cur.z, i, db, sb, sdict, dot);
+        pack_entry<NC>(cur.w, i, db, sb, sdict, dot);
     }
 }
 
@@ -179,15 +196,15 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const 
     __shared__ float sdict[FMT_ == FMT_SELL_PACK ? kDictLds : 1];                 \
     if constexpr (FMT_ == FMT_SELL_PACK) {                                        \
         const uint32_t m_ = (a_).ex.ndict * NC_;                                  \
-        for (uint32_t k_ = threadIdx.x; k_ < m_; k_ += kTPB) sdict[k_] = (a_).ex.dict[k_]; \
+        for (uint32_t k_ = threadIdx.x; k_ < m_; k_ += blockDim.x) sdict[k_] = (a_).ex.dict[k_]; \
         __syncthreads();                                                          \
     }
 
-template <int FMT, int NC>
-__global__ __launch_bounds__(kTPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S, float* __restrict__ Y,
+template <int FMT, int NC, int TPB>
+__global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S, float* __restrict__ Y,
                                                     float* __restrict__ A) {
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
+    const uint32_t i = tile_index(a.xcd) * TPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
@@ -203,11 +220,11 @@ __global__ __launch_bounds__(kTPB) void k_heun_pred(FusedArgs a, const float* __
     }
 }
 
-template <int FMT, int NC>
-__global__ __launch_bounds__(kTPB) void k_heun_corr(FusedArgs a, const float* __restrict__ Y,
+template <int FMT, int NC, int TPB>
+__global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __restrict__ Y,
                                                     const float* __restrict__ A, float* __restrict__ S) {
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
+    const uint32_t i = tile_index(a.xcd) * TPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
@@ -388,7 +405,7 @@ static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
     a.perm = c->perm;
     if (c->T > 0.0) a.noise = (c->noise_mode == LLG_NOISE_INJECTED) ? 2 : 1;
     a.multi = c->multi_species ? 1 : 0;
-    a.xcd = (c->flags & 0x100u) ? 0 : 1;
+    a.xcd = ((c->flags & LLG_FLAG_NO_XCD_REMAP) || !c->tune_xcd) ? 0 : 1;
     a.rdt = (float)c->rdt;
     a.half_rdt = (float)(0.5 * c->rdt);
     a.sqrtT = (float)std::sqrt(c->T);
@@ -432,13 +449,19 @@ static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
         }                                                                           \
     } while (0)
 
+// The Heun pair runs with the autotuned block size (256, 512 or 1024 threads).
 int fused_heun_pred(llg_ctx* c, const StepSched& s) {
     FusedArgs a = fused_args(c, s);
-    const dim3 g(blocks_for(c->n, kTPB)), b(kTPB);
     float* S = c->S.as<float>();
     float* Y = c->Y.as<float>();
     float* A = c->A.as<float>();
-#define SLLG_L(F, N) k_heun_pred<F, N><<<g, b, 0, c->stream>>>(a, S, Y, A)
+    const int tpb = c->tune_tpb;
+#define SLLG_L(F, N)                                                                                   \
+    do {                                                                                               \
+        if (tpb == 1024) k_heun_pred<F, N, 1024><<<blocks_for(c->n, 1024), 1024, 0, c->stream>>>(a, S, Y, A); \
+        else if (tpb == 512) k_heun_pred<F, N, 512><<<blocks_for(c->n, 512), 512, 0, c->stream>>>(a, S, Y, A); \
+        else k_heun_pred<F, N, 256><<<blocks_for(c->n, 256), 256, 0, c->stream>>>(a, S, Y, A);        \
+    } while (0)
     SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
     SLLG_HIP(hipGetLastError());
@@ -447,11 +470,16 @@ int fused_heun_pred(llg_ctx* c, const StepSched& s) {
 
 int fused_heun_corr(llg_ctx* c, const StepSched& s) {
     FusedArgs a = fused_args(c, s);
-    const dim3 g(blocks_for(c->n, kTPB)), b(kTPB);
     float* S = c->S.as<float>();
     float* Y = c->Y.as<float>();
     float* A = c->A.as<float>();
-#define SLLG_L(F, N) k_heun_corr<F, N><<<g, b, 0, c->stream>>>(a, Y, A, S)
+    const int tpb = c->tune_tpb;
+#define SLLG_L(F, N)                                                                                   \
+    do {                                                                                               \
+        if (tpb == 1024) k_heun_corr<F, N, 1024><<<blocks_for(c->n, 1024), 1024, 0, c->stream>>>(a, Y, A, S); \
+        else if (tpb == 512) k_heun_corr<F, N, 512><<<blocks_for(c->n, 512), 512, 0, c->stream>>>(a, Y, A, S); \
+        else k_heun_corr<F, N, 256><<<blocks_for(c->n, 256), 256, 0, c->stream>>>(a, Y, A, S);        \
+    } while (0)
     SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
     SLLG_HIP(hipGetLastError());

@@ -101,8 +101,8 @@ __device__ __forceinline__ void exch_row_sell64(const Exch<double>& e, uint32_t 
     const uint32_t n = e.n;
     dot[0] = dot[1] = dot[2] = 0.0;
     const uint32_t sl = i >> 6, lane = i & 63u;
-    const uint32_t q0 = e.slice_off[sl];
-    const uint32_t len = e.slice_off[sl + 1] - q0;
+    const uint2 sd = e.slice[sl];
+    const uint32_t q0 = sd.x, len = sd.y;
     double v[NC];
     for (uint32_t q = 0; q < len; ++q) {
         uint32_t w, j;
@@ -111,8 +111,8 @@ __device__ __forceinline__ void exch_row_sell64(const Exch<double>& e, uint32_t 
         else
             w = e.ent[((size_t)q0 + q) * kSlice + lane];
         if (e.fmt == FMT_SELL_PACK) {
-            j = pack_nb(w, i % e.ncell, e.ncell);
-            const double* dv = e.dict + (size_t)pack_dict(w) * NC;
+            j = pack_nb(w, i, e.dbits);
+            const double* dv = e.dict + (size_t)pack_dict(w, e.dbits) * NC;
             for (int k = 0; k < NC; ++k) v[k] = dv[k];
         } else {
             j = w;

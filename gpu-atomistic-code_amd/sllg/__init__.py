@@ -29,6 +29,7 @@ NOISE_COUNTER, NOISE_INJECTED = 0, 1
 FLAG_CHECK_FINITE = 0x1
 FLAG_NO_XCD_REMAP = 0x100
 FLAG_NO_SUBLATTICE_ORDER = 0x200
+FLAG_NO_SLICE_SHARING = 0x400
 
 STATUS = {0: "OK", -1: "INVALID", -2: "NOMEM", -3: "DEVICE", -4: "UNSUPPORTED", -5: "NODEVICE",
           -6: "STATE", -7: "NONFINITE"}
@@ -79,6 +80,11 @@ class Info(C.Structure):
     _fields_ = [("device_bytes", C.c_uint64), ("bytes_per_spin_update", C.c_double),
                 ("dominant_kernel_bytes_per_spin", C.c_double), ("dominant_kernel", C.c_char * 48),
                 ("exch_encoding", C.c_char * 48)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("tpb", C.c_int), ("xcd", C.c_int), ("ms_per_step", C.c_double),
+                ("ms_per_step_default", C.c_double), ("candidates", C.c_int), ("from_cache", C.c_int)]
 
 
 class UnitCell(C.Structure):
@@ -132,6 +138,8 @@ def lib():
         L.llg_stream.restype = P
         L.llg_profile_kernels.argtypes = [P, C.c_uint64, C.c_uint32, C.c_int, C.c_char_p, dp, C.POINTER(C.c_int)]
         L.llg_get_info.argtypes = [P, C.POINTER(Info)]
+        L.llg_autotune.argtypes = [P, C.c_uint32, C.POINTER(TuneResult)]
+        L.llg_set_launch.argtypes = [P, C.c_int, C.c_int]
         up = C.POINTER(C.c_uint32)
         L.llg_lattice_shells.argtypes = [C.POINTER(UnitCell), C.c_uint32, dp, C.POINTER(Interaction), C.c_uint32,
                                          up, dp]
@@ -511,6 +519,17 @@ class Context:
             nm = raw[48 * k:48 * (k + 1)].split(b"\0", 1)[0].decode()
             out[nm] = float(ms[k])
         return out
+
+    def autotune(self, nsteps=20):
+        """Pick the FP32 Heun launch shape by timing the candidates on this system."""
+        r = TuneResult()
+        _check(self.L.llg_autotune(self.h, int(nsteps), C.byref(r)), self.L)
+        return {"tpb": r.tpb, "xcd": r.xcd, "ms_per_step": r.ms_per_step,
+                "ms_per_step_default": r.ms_per_step_default, "candidates": r.candidates,
+                "from_cache": bool(r.from_cache)}
+
+    def set_launch(self, tpb, xcd=1):
+        _check(self.L.llg_set_launch(self.h, int(tpb), int(xcd)), self.L)
 
     def info(self):
         i = Info()

@@ -79,6 +79,10 @@ typedef enum llg_noise {
 
 /* Flags (llg_desc.flags). */
 #define LLG_FLAG_CHECK_FINITE 0x1u /* llg_step checks spins for NaN/Inf after the run */
+/* Encoding switches of the FP32 path, for measurement; results do not depend on them. */
+#define LLG_FLAG_NO_XCD_REMAP 0x100u        /* tiles in launch order, no per-XCD row ranges */
+#define LLG_FLAG_NO_SUBLATTICE_ORDER 0x200u /* keep the reference spin order on the device */
+#define LLG_FLAG_NO_SLICE_SHARING 0x400u    /* store every 64-row slice's entries, even repeats */
 
 /* Per-species (unit-cell atom) constants; reference geom::ucm (inc/unitcell.h) and
  * the per-spin arrays filled in src/geom.cpp:239-247 and src/llg.cpp:591-592,626-631. */
@@ -247,6 +251,21 @@ void* llg_stream(llg_ctx* ctx);
  * max_kinds NUL-terminated names of up to 47 chars (48 bytes each). */
 int llg_profile_kernels(llg_ctx* ctx, uint64_t t0, uint32_t nsteps, int max_kinds,
                         char* names, double* mean_ms, int* n_kinds);
+
+/* Launch-shape autotuner (no reference counterpart): times the FP32 Heun kernel pair for
+ * every candidate block size (256, 512, 1024 threads) with and without the XCD-aware
+ * tile remap, on this handle's own couplings, for nsteps steps each; keeps the fastest.
+ * The spin state is restored afterwards.  Results are cached per (device, encoding, n,
+ * entries) in the process and, if the environment variable SLLG_TUNE_FILE names a file,
+ * across processes.  Other paths return LLG_OK with the defaults. */
+typedef struct llg_tune_result {
+    int tpb, xcd;
+    double ms_per_step;     /* chosen */
+    double ms_per_step_default; /* 256 threads, remap on */
+    int candidates, from_cache;
+} llg_tune_result;
+int llg_autotune(llg_ctx* ctx, uint32_t nsteps, llg_tune_result* out);
+int llg_set_launch(llg_ctx* ctx, int tpb, int xcd);
 
 /* Algorithmic HBM bytes per spin-update of the configured path, split into the
  * kernels of one step, and the device bytes the handle holds. */

@@ -72,15 +72,26 @@ def test_batch_graph_replay_equals_eager(gpu):
 
 
 def test_batch_budget_admission(gpu):
+    """A member is admitted when the bytes held plus its upper-bound estimate fit the budget;
+    otherwise LLG_E_NOMEM and nothing is allocated.  (The bytes held can be well below the
+    estimate: repeated slices of the encoding are stored once.)"""
     s, s0, _ = W.sc_ferro(L=16, precision="fp32")
     est = SB.estimate_bytes(s)
-    B = SB.Batch(budget_bytes=int(2.5 * est))
+    B = SB.Batch(budget_bytes=est)
     B.add(s, s0)
-    B.add(s, s0)
+    used1 = B.info()["device_bytes"]
+    assert 0 < used1 <= est
     with pytest.raises(sllg.LLGError) as e:
         B.add(s, s0)
     assert e.value.code == -2
-    assert B.info()["members"] == 2
+    assert B.info()["members"] == 1 and B.info()["device_bytes"] == used1
+    B2 = SB.Batch(budget_bytes=used1 + est)
+    B2.add(s, s0)
+    B2.add(s, s0)
+    with pytest.raises(sllg.LLGError) as e:
+        B2.add(s, s0)
+    assert e.value.code == -2
+    assert B2.info()["members"] == 2
 
 
 def test_batch_dipolar_member(gpu):

@@ -266,3 +266,67 @@ def test_sublattice_order_dipolar(gpu):
     rel = _maxdiff(hg, ho) / np.abs(ho).max()
     print("fcc dipolar field rel err", rel)
     assert rel <= 2e-5
+
+
+def test_launch_shape_does_not_change_results(gpu):
+    """Each thread of the FP32 Heun kernels owns one spin: every block size and tile order
+    gives identical bits.  The autotuner keeps the state it was given."""
+    sysd, s0 = _fcc_multi(L=4)
+    res = []
+    for tpb, xcd in ((256, 1), (512, 0), (1024, 1), (256, 0)):
+        g = sllg.Context(sysd)
+        g.set_launch(tpb, xcd)
+        g.set_spins(s0)
+        g.set_temperature(150.0)
+        g.step(0, 40)
+        res.append(g.get_spins())
+    for r in res[1:]:
+        assert np.array_equal(r, res[0])
+    g = sllg.Context(sysd)
+    g.set_spins(s0)
+    g.set_temperature(150.0)
+    g.step(0, 10)
+    before = g.get_spins()
+    t = g.autotune(5)
+    assert t["tpb"] in (256, 512, 1024) and t["xcd"] in (0, 1)
+    assert np.array_equal(g.get_spins(), before)
+    g.step(10, 30)
+    assert np.array_equal(g.get_spins(), res[0])
+    with pytest.raises(sllg.LLGError):
+        g.set_launch(384, 1)
+
+
+def _slices(enc):
+    # "...-slices<stored>of<total>"
+    tail = enc.rsplit("-slices", 1)[1]
+    a, b = tail.split("of")
+    return int(a), int(b)
+
+
+@pytest.mark.parametrize("kind", ["fcc", "sc"])
+def test_slice_sharing_is_exact(gpu, kind):
+    """Repeated 64-row slices share one stored copy of their packed entries: identical bits
+    to the encoding that stores every slice, and the oracle's result to FP32 rounding."""
+    if kind == "fcc":
+        sysd, s0 = _fcc_multi(L=8)
+    else:
+        sysd, s0, _ = W.sc_ferro(L=16, precision="fp32")
+    res, encs = [], []
+    for flags in (0, sllg.FLAG_NO_SLICE_SHARING):
+        sysd.flags = flags
+        g = sllg.Context(sysd)
+        encs.append(g.info()["exch_encoding"])
+        g.set_spins(s0)
+        g.set_temperature(100.0)
+        g.step(0, 60)
+        res.append(g.get_spins())
+        m, e = g.reduce()
+        res.append(np.array([e]))
+    sysd.flags = 0
+    st, tot = _slices(encs[0])
+    st2, tot2 = _slices(encs[1])
+    print(kind, encs)
+    assert tot == tot2 and st2 == tot and st < tot
+    assert np.array_equal(res[0], res[2]) and np.array_equal(res[1], res[3])
+    g, o = run_pair(sysd, s0, 60, T=0.0, B=(0.0, 0.3, 0.5))
+    assert _maxdiff(g.get_spins(), o.get_spins()) <= 1e-4
