@@ -200,21 +200,36 @@ def accuracy(args):
         g.close()
         return out
 
-    ref = run("fp64", s0)
-    f32 = run("fp32", s0)
-    sp = s0.copy()
-    sp[0] += 1e-15
-    sp /= np.linalg.norm(sp, axis=0)
-    lya = run("fp64", sp)
-    rows = []
-    for c in checkpoints:
-        rows.append({"step": c,
-                     "fp32_max_abs_dS": float(np.abs(f32[c][0] - ref[c][0]).max()),
-                     "fp32_abs_dM": float(np.abs(f32[c][1] - ref[c][1]).max()),
-                     "fp64_perturbed_max_abs_dS": float(np.abs(lya[c][0] - ref[c][0]).max()),
-                     "fp64_perturbed_abs_dM": float(np.abs(lya[c][1] - ref[c][1]).max()),
-                     "M_fp64": [float(x) for x in ref[c][1]]})
-    print(json.dumps({"mode": "accuracy", "n_spins": sysd.n, "T": 0.0, "rows": rows}), flush=True)
+    def compare(start, s_init):
+        ref = run("fp64", s_init)
+        f32 = run("fp32", s_init)
+        sp = s_init.copy()
+        sp[0] += 1e-15
+        sp /= np.linalg.norm(sp, axis=0)
+        lya = run("fp64", sp)
+        rows = []
+        for c in checkpoints:
+            rows.append({"step": c,
+                         "fp32_max_abs_dS": float(np.abs(f32[c][0] - ref[c][0]).max()),
+                         "fp32_abs_dM": float(np.abs(f32[c][1] - ref[c][1]).max()),
+                         "fp64_perturbed_max_abs_dS": float(np.abs(lya[c][0] - ref[c][0]).max()),
+                         "fp64_perturbed_abs_dM": float(np.abs(lya[c][1] - ref[c][1]).max()),
+                         "M_fp64": [float(x) for x in ref[c][1]]})
+        print(json.dumps({"mode": "accuracy", "start": start, "n_spins": sysd.n, "T": 0.0, "rows": rows}),
+              flush=True)
+
+    if args.start in ("random", "both"):
+        # uniform random start: a quench, chaotic (the FP64 control shows the growth)
+        compare("random", s0)
+    if args.start in ("ordered", "both"):
+        # ferromagnet along z with a Gaussian tilt of 0.05 rad per component: damped spin waves
+        rng = np.random.default_rng(args.seed)
+        so = np.zeros_like(s0)
+        so[0] = 0.05 * rng.standard_normal(sysd.n)
+        so[1] = 0.05 * rng.standard_normal(sysd.n)
+        so[2] = 1.0
+        so /= np.linalg.norm(so, axis=0)
+        compare("ordered", so)
 
 
 def rk4_dmi(args):
@@ -357,6 +372,7 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--sequential", action="store_true", help="serve: also run the sequential FP64 plain comparator")
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--start", default="both", choices=["random", "ordered", "both"], help="accuracy: initial state")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

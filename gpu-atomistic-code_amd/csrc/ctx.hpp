@@ -101,6 +101,13 @@ __host__ __device__ __forceinline__ uint32_t pack_dict(uint32_t w, uint32_t dbit
 __host__ __device__ __forceinline__ uint32_t pack_nb(uint32_t w, uint32_t i, uint32_t dbits) {
     return i + (uint32_t)((int32_t)w >> dbits);
 }
+// Position of entry q of lane `lane` within a packed slice block of len (even) entries per
+// row: full chunks of 4 stored [chunk][lane][4], then a tail of 2 stored [lane][2].
+__host__ __device__ __forceinline__ size_t pack_pos(uint32_t q, uint32_t lane, uint32_t len) {
+    const uint32_t full = len & ~3u;
+    return q < full ? (size_t)(q >> 2) * (kSlice * 4) + lane * 4 + (q & 3)
+                    : (size_t)full * kSlice + lane * 2 + (q - full);
+}
 
 struct DipolarPlan;  // k_dipolar.hip
 

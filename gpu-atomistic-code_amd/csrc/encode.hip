@@ -179,7 +179,8 @@ struct TupleHash {
 
 // CSR -> sliced ELL.  Packed layout: per slice, entries in chunks of 4 per lane,
 // [chunk][lane][4], so one uint4 load per lane reads 4 entries and a wave reads 1 KB
-// contiguous; slice lengths are rounded up to a multiple of 4 with zero-coupling entries
+// contiguous, then a tail of 2 per lane [lane][2] (one uint2 load); slice lengths are
+// rounded up to a multiple of 2 with zero-coupling entries
 // (offset 0, dictionary tuple 0.0), which add exactly 0 to the field.  Raw layout:
 // [entry][lane] u32 columns and [ncomp][entry][lane] FP32 values.
 int build_sell(llg_ctx* c) {
@@ -250,7 +251,7 @@ int build_sell(llg_ctx* c) {
     std::vector<uint2> slice(nsl);
     uint64_t tot = 0;
     for (uint32_t s = 0; s < nsl; ++s) {
-        slice[s].y = pack ? (mx[s] + 3u) / 4u * 4u : mx[s];
+        slice[s].y = pack ? (mx[s] + 1u) / 2u * 2u : mx[s];
         slice[s].x = (uint32_t)tot;
         tot += slice[s].y;
     }
@@ -281,7 +282,7 @@ int build_sell(llg_ctx* c) {
                         const uint32_t qq = h.xadj[r] + q;
                         w = pack_word((int32_t)((int64_t)P.to_dev(h.adj[qq]) - (int64_t)d), idx[qq], dbits);
                     }
-                    blk[(size_t)(q >> 2) * (kSlice * 4) + l * 4 + (q & 3)] = w;
+                    blk[pack_pos(q, l, len)] = w;
                 }
             }
             uint64_t hsh = 1469598103934665603ull ^ len;  // FNV-1a over the block's words

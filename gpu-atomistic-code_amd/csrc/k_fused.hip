@@ -20,7 +20,7 @@ namespace sllg {
 // Sliced ELL, scalar reader (any precision; dictionary read from global memory).  Slice
 // = 64 consecutive device rows (one wavefront), located by the slice table {first, len}.
 // Packed entry (pack_word): neighbour row offset and coupling-dictionary index
-// [ndict][NC]; a slice's entries are stored [chunk of 4][lane][4] and slices with
+// [ndict][NC]; a slice's entries are stored [chunk of 4][lane][4] + [lane][2] tail and slices with
 // identical entries share one copy.  Raw entry: u32 column, values [NC][nent], both
 // [entry][lane].
 template <int NC, class R, class V, class SpinT>
@@ -32,9 +32,9 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
     const uint32_t q0 = sd.x, len = sd.y;
     R v[NC];
     if (e.fmt == FMT_SELL_PACK) {
-        const uint32_t* __restrict__ ent = e.ent + (size_t)q0 * kSlice + lane * 4;
+        const uint32_t* __restrict__ ent = e.ent + (size_t)q0 * kSlice;
         for (uint32_t q = 0; q < len; ++q) {
-            const uint32_t w = ent[(size_t)(q >> 2) * (kSlice * 4) + (q & 3)];
+            const uint32_t w = ent[pack_pos(q, lane, len)];
             const uint32_t j = pack_nb(w, i, e.dbits);
             const V* dv = e.dict + (size_t)pack_dict(w, e.dbits) * NC;
 #pragma unroll
@@ -86,7 +86,8 @@ __device__ __forceinline__ void pack_entry(uint32_t w, uint32_t i, uint32_t dbit
 }
 
 // Fast packed reader of the fused kernels: one uint4 (4 entries) per lane per load
-// (1 KB per wavefront), couplings from the block's LDS copy of the dictionary.
+// (1 KB per wavefront), then at most one uint2 tail; couplings from the block's LDS copy
+// of the dictionary.
 template <int NC>
 __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_t i, const float* __restrict__ S,
                                                    const float* sdict, float dot[3]) {
@@ -94,7 +95,6 @@ __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_
     const uint32_t sl = i >> 6, lane = i & 63u;
     const uint2 sd = e.slice[sl];
     const uint32_t q0 = sd.x, nch = sd.y >> 2;
-    if (nch == 0) return;
     const uint4* __restrict__ p = reinterpret_cast<const uint4*>(e.ent + (size_t)q0 * kSlice) + lane;
     const uint32_t db = e.dbits;
     const SpinBuf sb = spin_buf(S, e.n);
@@ -105,6 +105,11 @@ __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_
         pack_entry<NC>(cur.y, i, db, sb, sdict, dot);
         pack_entry<NC>(cur.z, i, db, sb, sdict, dot);
         pack_entry<NC>(cur.w, i, db, sb, sdict, dot);
+    }
+    if (sd.y & 2u) {  // tail of 2 entries per lane, [lane][2]
+        const uint2 t = reinterpret_cast<const uint2*>(p - lane + (size_t)nch * kSlice)[lane];
+        pack_entry<NC>(t.x, i, db, sb, sdict, dot);
+        pack_entry<NC>(t.y, i, db, sb, sdict, dot);
     }
 }
 
@@ -124,6 +129,8 @@ struct FusedArgs {
     int noise;  // 0 none (T = 0), 1 counter, 2 injected
     int multi;  // per-spin species index present
     int xcd;    // XCD-aware block remap
+    uint32_t il_na;  // > 1: sublattice-interleaved tile order (il_na sublattices of il_cells rows)
+    uint32_t il_cells;
     float rdt, half_rdt, sqrtT;
     float B[3];
     uint64_t seed, stream, t;
@@ -146,6 +153,20 @@ __device__ __forceinline__ uint32_t tile_index(int xcd) {
     const uint32_t nb = gridDim.x, per = nb >> 3, rem = nb & 7u;
     const uint32_t x = b & 7u, l = b >> 3;
     return x * per + (x < rem ? x : rem) + l;
+}
+
+// Row tile of this block in the sublattice-major device order.  With the XCD remap each
+// XCD takes a contiguous range of logical tiles; logical tile L is mapped to sublattice
+// L % na, cell tile L / na, so an XCD's range covers one cell range of every sublattice and
+// the neighbours its rows gather (all sublattices, nearby cells) are fetched into its L2
+// once, not once per sublattice.  Requires il_cells % blockDim.x == 0 (checked on the host).
+__device__ __forceinline__ uint32_t row_tile(const FusedArgs& a) {
+    uint32_t L = tile_index(a.xcd);
+    if (a.il_na > 1) {
+        const uint32_t tps = a.il_cells / blockDim.x;
+        L = (L % a.il_na) * tps + L / a.il_na;
+    }
+    return L;
 }
 
 template <int FMT, int NC>
@@ -204,7 +225,7 @@ template <int FMT, int NC, int TPB>
 __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S, float* __restrict__ Y,
                                                     float* __restrict__ A) {
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = tile_index(a.xcd) * TPB + threadIdx.x;
+    const uint32_t i = row_tile(a) * TPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
@@ -224,7 +245,7 @@ template <int FMT, int NC, int TPB>
 __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __restrict__ Y,
                                                     const float* __restrict__ A, float* __restrict__ S) {
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = tile_index(a.xcd) * TPB + threadIdx.x;
+    const uint32_t i = row_tile(a) * TPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
@@ -293,7 +314,7 @@ template <int FMT, int NC, int M>
 __global__ __launch_bounds__(kTPB) void k_rk4(FusedArgs a, const float* __restrict__ Yin, float* __restrict__ Yout,
                                               float* __restrict__ ACC, float* __restrict__ S) {
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
+    const uint32_t i = row_tile(a) * kTPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
@@ -335,7 +356,7 @@ __global__ __launch_bounds__(kTPB) void k_rk5(FusedArgs a, const float* __restri
                                               float* __restrict__ S, float* __restrict__ K0, float* __restrict__ K1,
                                               float* __restrict__ K2, float* __restrict__ K3, float* __restrict__ K4) {
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = tile_index(a.xcd) * kTPB + threadIdx.x;
+    const uint32_t i = row_tile(a) * kTPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
@@ -406,6 +427,11 @@ static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
     if (c->T > 0.0) a.noise = (c->noise_mode == LLG_NOISE_INJECTED) ? 2 : 1;
     a.multi = c->multi_species ? 1 : 0;
     a.xcd = ((c->flags & LLG_FLAG_NO_XCD_REMAP) || !c->tune_xcd) ? 0 : 1;
+    // every block size used (256..1024) divides ncell when 1024 does
+    if (a.xcd && c->perm.na > 1 && c->perm.ncell % 1024u == 0) {
+        a.il_na = c->perm.na;
+        a.il_cells = c->perm.ncell;
+    }
     a.rdt = (float)c->rdt;
     a.half_rdt = (float)(0.5 * c->rdt);
     a.sqrtT = (float)std::sqrt(c->T);

@@ -107,7 +107,7 @@ __device__ __forceinline__ void exch_row_sell64(const Exch<double>& e, uint32_t 
     for (uint32_t q = 0; q < len; ++q) {
         uint32_t w, j;
         if (e.fmt == FMT_SELL_PACK)
-            w = e.ent[(size_t)q0 * kSlice + (size_t)(q >> 2) * (kSlice * 4) + lane * 4 + (q & 3)];
+            w = e.ent[(size_t)q0 * kSlice + pack_pos(q, lane, len)];
         else
             w = e.ent[((size_t)q0 + q) * kSlice + lane];
         if (e.fmt == FMT_SELL_PACK) {

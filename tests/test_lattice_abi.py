@@ -128,7 +128,7 @@ def _declared(header):
 
 def test_library_exports_every_declared_symbol():
     L = sllg.lib()
-    decl = _declared("llg_capi.h") + _declared("llg_lattice.h") + _declared("llg_batch.h")
+    decl = _declared("llg_capi.h") + _declared("llg_lattice.h") + _declared("llg_batch.h") + _declared("llg_io.h")
     assert len(decl) > 35
     missing = [s for s in decl if not hasattr(L, s)]
     assert not missing, missing
@@ -142,12 +142,15 @@ def test_struct_layout_matches_c():
 #include <stddef.h>
 #include "llg_capi.h"
 #include "llg_lattice.h"
+#include "llg_io.h"
 int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(llg_desc), sizeof(llg_species), sizeof(llg_anisotropy),
          sizeof(llg_exchange), sizeof(llg_dipolar), sizeof(llg_info), sizeof(llg_unitcell), sizeof(llg_interaction));
   printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", offsetof(llg_desc, anis), offsetof(llg_desc, exch), offsetof(llg_desc, dip),
          offsetof(llg_desc, dt), offsetof(llg_desc, seed), offsetof(llg_desc, device), offsetof(llg_exchange, stencil_vals),
          offsetof(llg_dipolar, mesh));
+  printf("%zu %zu %zu %zu\n", sizeof(llg_ucf_atom), offsetof(llg_ucf_atom, element), sizeof(llg_tune_result),
+         offsetof(llg_tune_result, candidates));
   return 0;
 }
 '''
@@ -156,7 +159,7 @@ int main(void) {
         open(c, "w").write(src)
         exe = os.path.join(d, "l")
         subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
-        a, b = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")[:2]
+        a, b, c3 = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")[:3]
     sizes = [int(x) for x in a.split()]
     offs = [int(x) for x in b.split()]
     assert sizes == [C.sizeof(t) for t in (sllg.Desc, sllg.Species, sllg.Anisotropy, sllg.Exchange, sllg.Dipolar,
@@ -164,6 +167,9 @@ int main(void) {
     assert offs == [sllg.Desc.anis.offset, sllg.Desc.exch.offset, sllg.Desc.dip.offset, sllg.Desc.dt.offset,
                     sllg.Desc.seed.offset, sllg.Desc.device.offset, sllg.Exchange.stencil_vals.offset,
                     sllg.Dipolar.mesh.offset]
+    from sllg import refinput
+    assert [int(x) for x in c3.split()] == [C.sizeof(refinput.UcfAtom), refinput.UcfAtom.element.offset,
+                                            C.sizeof(sllg.TuneResult), sllg.TuneResult.candidates.offset]
 
 
 def _create_rc(sysd):
