@@ -599,6 +599,24 @@ int llg_reduce(llg_ctx* c, double* m, double* energy) {
     return 0;
 }
 
+int llg_set_bins(llg_ctx* c, uint32_t nbins, const uint32_t* bin, const double* weight) {
+    if (!c || (!bin && nbins)) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    SLLG_HIP(hipSetDevice(c->device));
+    return set_bins(c, nbins, bin, weight);
+}
+
+int llg_reduce_bins(llg_ctx* c, double* out) {
+    if (!c || !out) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    if (c->nbins == 0) SLLG_FAIL(LLG_E_STATE, "no bins set (llg_set_bins)");
+    SLLG_HIP(hipSetDevice(c->device));
+    DevBuf d;
+    SLLG_TRY(d.alloc((size_t)c->nbins * 3 * sizeof(double)));
+    SLLG_TRY(reduce_bins(c, d.as<double>()));
+    SLLG_HIP(hipMemcpyAsync(out, d.p, (size_t)c->nbins * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int llg_run_sampled(llg_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t every, double* m_series) {
     if (!c || !m_series || every == 0) SLLG_FAIL(LLG_E_INVALID, "bad argument");
     SLLG_HIP(hipSetDevice(c->device));

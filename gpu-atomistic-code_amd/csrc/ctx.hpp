@@ -182,6 +182,9 @@ struct llg_ctx {
     sllg::Exch<float> ex32;
     // reductions
     sllg::DevBuf red_part, red_out, m_series;
+    // binned sums (llg_set_bins): device rows per chunk of one bin, weights, chunk table
+    sllg::DevBuf bin_idx, bin_w, bin_cs, bin_cb, bin_part;
+    uint32_t nbins = 0, nbin_chunks = 0;
     // dipolar
     sllg::DipolarPlan* dplan = nullptr;
     // injected noise queue (host)

@@ -194,6 +194,42 @@ __global__ __launch_bounds__(kTPB) void k_check_finite(uint32_t n3, const SpinT*
     if (!(v == v) || v > 1e300 || v < -1e300) atomicOr(flag, 1);
 }
 
+// Binned FP64 sums out[b] = sum over the spins of bin b of w_i S_i.  The spins of a bin are
+// listed in ascending device order and cut into chunks of at most kBinChunk; one block
+// reduces one chunk (fixed-order block sum), then one thread per bin adds its chunks in
+// order.  Deterministic, no atomics.
+constexpr uint32_t kBinChunk = 4096;
+
+template <class SpinT>
+__global__ __launch_bounds__(kTPB) void k_bin_partial(uint32_t n, const uint32_t* __restrict__ cs,
+                                                      const uint32_t* __restrict__ idx, const double* __restrict__ w,
+                                                      const SpinT* __restrict__ S, double* __restrict__ part) {
+    __shared__ double lds[kTPB / 64];
+    const uint32_t b = cs[blockIdx.x], e = cs[blockIdx.x + 1];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (uint32_t q = b + threadIdx.x; q < e; q += kTPB) {
+        const uint32_t i = idx[q];
+        const double wq = w ? w[q] : 1.0;
+        a[0] += wq * (double)S[i];
+        a[1] += wq * (double)S[(size_t)n + i];
+        a[2] += wq * (double)S[2 * (size_t)n + i];
+    }
+    for (int c = 0; c < 3; ++c) {
+        const double r = block_sum(a[c], lds);
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * 3 + c] = r;
+    }
+}
+
+__global__ __launch_bounds__(kTPB) void k_bin_finish(uint32_t nbins, const uint32_t* __restrict__ cb,
+                                                     const double* __restrict__ part, double* __restrict__ out) {
+    const uint32_t b = blockIdx.x * kTPB + threadIdx.x;
+    if (b >= nbins) return;
+    double a[3] = {0.0, 0.0, 0.0};
+    for (uint32_t k = cb[b]; k < cb[b + 1]; ++k)
+        for (int c = 0; c < 3; ++c) a[c] += part[(size_t)k * 3 + c];
+    for (int c = 0; c < 3; ++c) out[(size_t)b * 3 + c] = a[c];
+}
+
 // ---------------------------------------------------------------- host launchers
 
 static int ensure_red(llg_ctx* c) {
@@ -295,6 +331,66 @@ int check_finite(llg_ctx* c, int* bad) {
     SLLG_HIP(hipGetLastError());
     SLLG_HIP(hipMemcpyAsync(bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SLLG_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int set_bins(llg_ctx* c, uint32_t nbins, const uint32_t* bin, const double* weight) {
+    const uint32_t n = c->n;
+    // spins of each bin in ascending device order (bin and weight are given in reference order)
+    std::vector<uint32_t> cnt(nbins + 1, 0);
+    for (uint32_t i = 0; i < n; ++i)
+        if (bin[i] < nbins) ++cnt[bin[i] + 1];
+    for (uint32_t b = 0; b < nbins; ++b) cnt[b + 1] += cnt[b];
+    std::vector<uint32_t> idx(std::max<uint32_t>(1, cnt[nbins])), fill(cnt.begin(), cnt.end() - 1);
+    std::vector<double> w(weight ? idx.size() : 0);
+    for (uint32_t d = 0; d < n; ++d) {
+        const uint32_t i = c->perm.to_ref(d);
+        const uint32_t b = bin[i];
+        if (b >= nbins) continue;
+        if (weight) w[fill[b]] = weight[i];
+        idx[fill[b]++] = d;
+    }
+    std::vector<uint32_t> cs(1, 0), cb(nbins + 1, 0);
+    for (uint32_t b = 0; b < nbins; ++b) {
+        cb[b] = (uint32_t)cs.size() - 1;
+        for (uint32_t q = cnt[b]; q < cnt[b + 1]; q += kBinChunk) cs.push_back(std::min(cnt[b + 1], q + kBinChunk));
+    }
+    cb[nbins] = (uint32_t)cs.size() - 1;
+    const uint32_t nch = (uint32_t)cs.size() - 1;
+    auto up = [&](DevBuf& d, const void* h, size_t bytes) -> int {
+        SLLG_TRY(d.alloc(std::max<size_t>(bytes, 8)));
+        if (bytes) SLLG_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c->stream));
+        return 0;
+    };
+    SLLG_TRY(up(c->bin_idx, idx.data(), idx.size() * 4));
+    if (weight)
+        SLLG_TRY(up(c->bin_w, w.data(), w.size() * 8));
+    else
+        c->bin_w.release();
+    SLLG_TRY(up(c->bin_cs, cs.data(), cs.size() * 4));
+    SLLG_TRY(up(c->bin_cb, cb.data(), cb.size() * 4));
+    SLLG_TRY(c->bin_part.alloc(std::max<size_t>(1, nch) * 3 * sizeof(double)));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    c->nbins = nbins;
+    c->nbin_chunks = nch;
+    return 0;
+}
+
+int reduce_bins(llg_ctx* c, double* dev_out) {
+    if (c->nbins == 0) return 0;
+    const double* w = c->bin_w.p ? c->bin_w.as<double>() : nullptr;
+    if (c->nbin_chunks) {
+        if (c->prec == LLG_FP64_PLAIN)
+            k_bin_partial<double><<<c->nbin_chunks, kTPB, 0, c->stream>>>(
+                c->n, c->bin_cs.as<uint32_t>(), c->bin_idx.as<uint32_t>(), w, c->S.as<double>(), c->bin_part.as<double>());
+        else
+            k_bin_partial<float><<<c->nbin_chunks, kTPB, 0, c->stream>>>(
+                c->n, c->bin_cs.as<uint32_t>(), c->bin_idx.as<uint32_t>(), w, c->S.as<float>(), c->bin_part.as<double>());
+        SLLG_HIP(hipGetLastError());
+    }
+    k_bin_finish<<<blocks_for(c->nbins, kTPB), kTPB, 0, c->stream>>>(c->nbins, c->bin_cb.as<uint32_t>(),
+                                                                     c->bin_part.as<double>(), dev_out);
+    SLLG_HIP(hipGetLastError());
     return 0;
 }
 

@@ -40,6 +40,8 @@ int reduce_energy(llg_ctx* c, const void* dem_now /* dipolar field at the curren
                   double* dev_out /* 1 */);
 int noise_sample(llg_ctx* c, uint64_t t, double* dev_out);
 int check_finite(llg_ctx* c, int* bad);
+int set_bins(llg_ctx* c, uint32_t nbins, const uint32_t* bin, const double* weight);
+int reduce_bins(llg_ctx* c, double* dev_out /* nbins*3 */);
 
 // Dipolar convolution (k_dipolar.hip)
 int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_species);

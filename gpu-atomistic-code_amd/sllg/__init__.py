@@ -139,6 +139,8 @@ def lib():
         L.llg_profile_kernels.argtypes = [P, C.c_uint64, C.c_uint32, C.c_int, C.c_char_p, dp, C.POINTER(C.c_int)]
         L.llg_get_info.argtypes = [P, C.POINTER(Info)]
         L.llg_autotune.argtypes = [P, C.c_uint32, C.POINTER(TuneResult)]
+        L.llg_set_bins.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint32), dp]
+        L.llg_reduce_bins.argtypes = [P, dp]
         L.llg_set_launch.argtypes = [P, C.c_int, C.c_int]
         up = C.POINTER(C.c_uint32)
         L.llg_lattice_shells.argtypes = [C.POINTER(UnitCell), C.c_uint32, dp, C.POINTER(Interaction), C.c_uint32,
@@ -518,6 +520,18 @@ class Context:
         for k in range(nk.value):
             nm = raw[48 * k:48 * (k + 1)].split(b"\0", 1)[0].decode()
             out[nm] = float(ms[k])
+        return out
+
+    def set_bins(self, nbins, bin_of_spin, weight=None):
+        """Upload a bin map (reference spin order) for llg_reduce_bins."""
+        b = np.ascontiguousarray(bin_of_spin, dtype=np.uint32)
+        w = None if weight is None else np.ascontiguousarray(weight, dtype=np.float64)
+        _check(self.L.llg_set_bins(self.h, int(nbins), _p(b, C.c_uint32), _p(w, C.c_double)), self.L)
+        self._nbins = int(nbins)
+
+    def reduce_bins(self):
+        out = np.zeros((self._nbins, 3))
+        _check(self.L.llg_reduce_bins(self.h, _p(out, C.c_double)), self.L)
         return out
 
     def autotune(self, nsteps=20):

@@ -23,6 +23,8 @@
  *                          (src/fields.cpp:162-198, src/cuda_util.cu:563-588)
  *   llg_scale_staggered <- cullg::rampStag (src/cuda.cu:38-41), fields::incStagFields
  *   llg_reduce          <- util::calc_mag method 0 (src/util_mag.cpp:44-63); energy has no reference counterpart
+ *   llg_set_bins /      <- util::calc_mag methods 1-10 (src/util_mag.cpp:65-245) as device sums
+ *   llg_reduce_bins
  */
 #ifndef LLG_CAPI_H
 #define LLG_CAPI_H
@@ -233,6 +235,15 @@ int llg_get_field(llg_ctx* ctx, double* hx, double* hy, double* hz);
  * (src/util_mag.cpp:44-63); energy (J, may be NULL) = exchange/2 + anisotropy +
  * Zeeman + staggered + dipolar/2 at the current spins, thermal part excluded. */
 int llg_reduce(llg_ctx* ctx, double* m, double* energy);
+
+/* Binned FP64 sums on the device: out[b*3+c] = sum over spins i with bin[i] == b of
+ * weight[i] * S_c(i) (weight NULL = 1; bin[i] >= nbins leaves spin i out).  bin and weight
+ * are in the reference spin order and are uploaded once by llg_set_bins.  This is the
+ * device form of util::calc_mag methods 1-10 (src/util_mag.cpp:65-245): layer sums along
+ * x/y/z weighted by mu (1-3), layer x species (5-7), chequerboard (8), discretised cells
+ * (9), per unit-cell atom (10); the caller divides by counts where the reference does. */
+int llg_set_bins(llg_ctx* ctx, uint32_t nbins, const uint32_t* bin, const double* weight);
+int llg_reduce_bins(llg_ctx* ctx, double* out);
 
 /* Run nsteps steps and record the sublattice magnetisation on the device every
  * `every` steps, before the step, as the drivers sample (src/simple.cpp:87-95).
