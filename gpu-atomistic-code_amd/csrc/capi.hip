@@ -617,6 +617,34 @@ int llg_reduce_bins(llg_ctx* c, double* out) {
     return 0;
 }
 
+int llg_set_sq(llg_ctx* c, const uint32_t grid[3], const uint32_t* mesh, uint32_t nk, const uint32_t* kpts, int axis,
+               const double* ua, const double* ub, uint32_t ngroups, const uint32_t* group) {
+    if (!c || !grid || !mesh || (nk && !kpts) || ngroups == 0) SLLG_FAIL(LLG_E_INVALID, "bad argument");
+    if (axis < 0 || axis > 2) SLLG_FAIL(LLG_E_INVALID, "quantisation axis must be 0, 1 or 2");
+    for (int d = 0; d < 3; ++d)
+        if (grid[d] == 0) SLLG_FAIL(LLG_E_INVALID, "empty grid");
+    for (uint64_t q = 0; q < (uint64_t)c->n * 3; ++q)
+        if (mesh[q] >= grid[q % 3]) SLLG_FAIL(LLG_E_INVALID, "mesh coordinate of spin %llu outside the grid",
+                                              (unsigned long long)(q / 3));
+    for (uint64_t q = 0; q < (uint64_t)nk * 3; ++q)
+        if (kpts[q] >= grid[q % 3]) SLLG_FAIL(LLG_E_INVALID, "k-point %llu outside the grid", (unsigned long long)(q / 3));
+    SLLG_HIP(hipSetDevice(c->device));
+    return set_sq(c, grid, mesh, nk, kpts, axis, ua, ub, ngroups, group);
+}
+
+int llg_sample_sq(llg_ctx* c, double* out) {
+    if (!c || !out) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    if (c->sq_nk == 0) SLLG_FAIL(LLG_E_STATE, "no k-points set (llg_set_sq)");
+    SLLG_HIP(hipSetDevice(c->device));
+    const size_t cnt = (size_t)c->sq_ngroups * c->sq_nk * 2;
+    DevBuf d;
+    SLLG_TRY(d.alloc(cnt * sizeof(double)));
+    SLLG_TRY(sample_sq(c, d.as<double>()));
+    SLLG_HIP(hipMemcpyAsync(out, d.p, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 int llg_run_sampled(llg_ctx* c, uint64_t t0, uint32_t nsteps, uint32_t every, double* m_series) {
     if (!c || !m_series || every == 0) SLLG_FAIL(LLG_E_INVALID, "bad argument");
     SLLG_HIP(hipSetDevice(c->device));

@@ -185,6 +185,11 @@ struct llg_ctx {
     // binned sums (llg_set_bins): device rows per chunk of one bin, weights, chunk table
     sllg::DevBuf bin_idx, bin_w, bin_cs, bin_cb, bin_part;
     uint32_t nbins = 0, nbin_chunks = 0;
+    // transverse structure factor (llg_set_sq): per listed spin {device row, mesh x, y, z},
+    // weights (ua, ub), chunk table, wave-vectors
+    sllg::DevBuf sq_spin, sq_w, sq_cs, sq_cg, sq_k, sq_part;
+    uint32_t sq_nk = 0, sq_ngroups = 0, sq_nch = 0, sq_grid[3] = {0, 0, 0};
+    int sq_axis = 2;
     // dipolar
     sllg::DipolarPlan* dplan = nullptr;
     // injected noise queue (host)

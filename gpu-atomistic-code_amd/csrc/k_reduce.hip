@@ -230,6 +230,70 @@ __global__ __launch_bounds__(kTPB) void k_bin_finish(uint32_t nbins, const uint3
     for (int c = 0; c < 3; ++c) out[(size_t)b * 3 + c] = a[c];
 }
 
+// Transverse structure factor at a list of mesh wave-vectors k:
+//   out[g][k] = sum_{i in g} (S_a(i) ua_i + i S_b(i) ub_i) exp(-2 pi i sum_d k_d x_d(i) / N_d).
+// The spins of each group are cut into chunks of kSqChunk; a block stages one chunk's
+// (c_i, mesh position) in LDS and each thread accumulates one k over it in FP64; a second
+// pass adds the chunks of each group in order.  Phase: frac(sum_d k_d x_d / N_d) in FP64,
+// then sincospi(2 frac).
+constexpr uint32_t kSqChunk = 512;
+
+struct SqArgs {
+    uint32_t n, nk, axis;
+    double inv[3];             // 1 / N_d
+    const uint4* spin;         // {device row, x, y, z} per listed spin
+    const double2* w;          // (ua, ub)
+    const uint32_t* cs;        // chunk starts [nch + 1]
+    const uint32_t* kp;        // [nk][3]
+};
+
+template <class SpinT>
+__global__ __launch_bounds__(kTPB) void k_sq_partial(SqArgs a, const SpinT* __restrict__ S, double* __restrict__ part) {
+    __shared__ double sre[kSqChunk], sim[kSqChunk], px[kSqChunk], py[kSqChunk], pz[kSqChunk];
+    const uint32_t ch = blockIdx.x, b = a.cs[ch], e = a.cs[ch + 1];
+    const uint32_t ca = a.axis == 0 ? 1u : 0u, cb = a.axis == 2 ? 1u : 2u;
+    for (uint32_t q = b + threadIdx.x; q < e; q += kTPB) {
+        const uint4 sp = a.spin[q];
+        const double2 w = a.w[q];
+        const uint32_t l = q - b;
+        sre[l] = (double)S[(size_t)ca * a.n + sp.x] * w.x;
+        sim[l] = (double)S[(size_t)cb * a.n + sp.x] * w.y;
+        px[l] = (double)sp.y * a.inv[0];
+        py[l] = (double)sp.z * a.inv[1];
+        pz[l] = (double)sp.w * a.inv[2];
+    }
+    __syncthreads();
+    const uint32_t k = blockIdx.y * kTPB + threadIdx.x;
+    if (k >= a.nk) return;
+    const double kx = a.kp[3 * k], ky = a.kp[3 * k + 1], kz = a.kp[3 * k + 2];
+    double re = 0.0, im = 0.0;
+    for (uint32_t l = 0; l < e - b; ++l) {
+        double t = kx * px[l] + ky * py[l] + kz * pz[l];
+        t -= floor(t);
+        double sn, cn;
+        sincospi(2.0 * t, &sn, &cn);
+        // (c_re + i c_im)(cos - i sin)
+        re += sre[l] * cn + sim[l] * sn;
+        im += sim[l] * cn - sre[l] * sn;
+    }
+    part[((size_t)ch * a.nk + k) * 2] = re;
+    part[((size_t)ch * a.nk + k) * 2 + 1] = im;
+}
+
+__global__ __launch_bounds__(kTPB) void k_sq_finish(uint32_t ngroups, uint32_t nk, const uint32_t* __restrict__ cg,
+                                                    const double* __restrict__ part, double* __restrict__ out) {
+    const uint32_t v = blockIdx.x * kTPB + threadIdx.x;
+    if (v >= ngroups * nk) return;
+    const uint32_t g = v / nk, k = v % nk;
+    double re = 0.0, im = 0.0;
+    for (uint32_t ch = cg[g]; ch < cg[g + 1]; ++ch) {
+        re += part[((size_t)ch * nk + k) * 2];
+        im += part[((size_t)ch * nk + k) * 2 + 1];
+    }
+    out[(size_t)v * 2] = re;
+    out[(size_t)v * 2 + 1] = im;
+}
+
 // ---------------------------------------------------------------- host launchers
 
 static int ensure_red(llg_ctx* c) {
@@ -390,6 +454,78 @@ int reduce_bins(llg_ctx* c, double* dev_out) {
     }
     k_bin_finish<<<blocks_for(c->nbins, kTPB), kTPB, 0, c->stream>>>(c->nbins, c->bin_cb.as<uint32_t>(),
                                                                      c->bin_part.as<double>(), dev_out);
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+int set_sq(llg_ctx* c, const uint32_t grid[3], const uint32_t* mesh, uint32_t nk, const uint32_t* kpts, int axis,
+           const double* ua, const double* ub, uint32_t ngroups, const uint32_t* group) {
+    const uint32_t n = c->n;
+    std::vector<uint32_t> cnt(ngroups + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t g = group ? group[i] : 0u;
+        if (g < ngroups) ++cnt[g + 1];
+    }
+    for (uint32_t g = 0; g < ngroups; ++g) cnt[g + 1] += cnt[g];
+    std::vector<uint4> sp(std::max<uint32_t>(1, cnt[ngroups]));
+    std::vector<double2> w(sp.size());
+    std::vector<uint32_t> fill(cnt.begin(), cnt.end() - 1);
+    for (uint32_t d = 0; d < n; ++d) {
+        const uint32_t i = c->perm.to_ref(d);
+        const uint32_t g = group ? group[i] : 0u;
+        if (g >= ngroups) continue;
+        const uint32_t q = fill[g]++;
+        sp[q] = make_uint4(d, mesh[3 * (size_t)i], mesh[3 * (size_t)i + 1], mesh[3 * (size_t)i + 2]);
+        w[q] = make_double2(ua ? ua[i] : 1.0, ub ? ub[i] : 1.0);
+    }
+    std::vector<uint32_t> cs(1, 0), cg(ngroups + 1, 0);
+    for (uint32_t g = 0; g < ngroups; ++g) {
+        cg[g] = (uint32_t)cs.size() - 1;
+        for (uint32_t q = cnt[g]; q < cnt[g + 1]; q += kSqChunk) cs.push_back(std::min(cnt[g + 1], q + kSqChunk));
+    }
+    cg[ngroups] = (uint32_t)cs.size() - 1;
+    const uint32_t nch = (uint32_t)cs.size() - 1;
+    auto up = [&](DevBuf& d, const void* h, size_t bytes) -> int {
+        SLLG_TRY(d.alloc(std::max<size_t>(bytes, 8)));
+        if (bytes) SLLG_HIP(hipMemcpyAsync(d.p, h, bytes, hipMemcpyHostToDevice, c->stream));
+        return 0;
+    };
+    SLLG_TRY(up(c->sq_spin, sp.data(), sp.size() * sizeof(uint4)));
+    SLLG_TRY(up(c->sq_w, w.data(), w.size() * sizeof(double2)));
+    SLLG_TRY(up(c->sq_cs, cs.data(), cs.size() * 4));
+    SLLG_TRY(up(c->sq_cg, cg.data(), cg.size() * 4));
+    SLLG_TRY(up(c->sq_k, kpts, (size_t)nk * 3 * 4));
+    SLLG_TRY(c->sq_part.alloc(std::max<size_t>(1, (size_t)nch * nk) * 2 * sizeof(double)));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
+    c->sq_nk = nk;
+    c->sq_ngroups = ngroups;
+    c->sq_nch = nch;
+    c->sq_axis = axis;
+    for (int d = 0; d < 3; ++d) c->sq_grid[d] = grid[d];
+    return 0;
+}
+
+int sample_sq(llg_ctx* c, double* dev_out) {
+    if (c->sq_nk == 0 || c->sq_ngroups == 0) return 0;
+    if (c->sq_nch) {
+        SqArgs a{};
+        a.n = c->n;
+        a.nk = c->sq_nk;
+        a.axis = (uint32_t)c->sq_axis;
+        for (int d = 0; d < 3; ++d) a.inv[d] = 1.0 / (double)c->sq_grid[d];
+        a.spin = c->sq_spin.as<uint4>();
+        a.w = c->sq_w.as<double2>();
+        a.cs = c->sq_cs.as<uint32_t>();
+        a.kp = c->sq_k.as<uint32_t>();
+        const dim3 grid(c->sq_nch, blocks_for(c->sq_nk, kTPB));
+        if (c->prec == LLG_FP64_PLAIN)
+            k_sq_partial<double><<<grid, kTPB, 0, c->stream>>>(a, c->S.as<double>(), c->sq_part.as<double>());
+        else
+            k_sq_partial<float><<<grid, kTPB, 0, c->stream>>>(a, c->S.as<float>(), c->sq_part.as<double>());
+        SLLG_HIP(hipGetLastError());
+    }
+    k_sq_finish<<<blocks_for(c->sq_ngroups * c->sq_nk, kTPB), kTPB, 0, c->stream>>>(
+        c->sq_ngroups, c->sq_nk, c->sq_cg.as<uint32_t>(), c->sq_part.as<double>(), dev_out);
     SLLG_HIP(hipGetLastError());
     return 0;
 }

@@ -42,6 +42,9 @@ int noise_sample(llg_ctx* c, uint64_t t, double* dev_out);
 int check_finite(llg_ctx* c, int* bad);
 int set_bins(llg_ctx* c, uint32_t nbins, const uint32_t* bin, const double* weight);
 int reduce_bins(llg_ctx* c, double* dev_out /* nbins*3 */);
+int set_sq(llg_ctx* c, const uint32_t grid[3], const uint32_t* mesh, uint32_t nk, const uint32_t* kpts, int axis,
+           const double* ua, const double* ub, uint32_t ngroups, const uint32_t* group);
+int sample_sq(llg_ctx* c, double* dev_out /* ngroups*nk*2 */);
 
 // Dipolar convolution (k_dipolar.hip)
 int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_species);

@@ -143,6 +143,8 @@ def lib():
         L.llg_reduce_bins.argtypes = [P, dp]
         L.llg_set_launch.argtypes = [P, C.c_int, C.c_int]
         up = C.POINTER(C.c_uint32)
+        L.llg_set_sq.argtypes = [P, up, up, C.c_uint32, up, C.c_int, dp, dp, C.c_uint32, up]
+        L.llg_sample_sq.argtypes = [P, dp]
         L.llg_lattice_shells.argtypes = [C.POINTER(UnitCell), C.c_uint32, dp, C.POINTER(Interaction), C.c_uint32,
                                          up, dp]
         L.llg_lattice_build_csr.argtypes = [C.POINTER(UnitCell), up, up, C.c_uint32, C.POINTER(Interaction), dp,
@@ -533,6 +535,24 @@ class Context:
         out = np.zeros((self._nbins, 3))
         _check(self.L.llg_reduce_bins(self.h, _p(out, C.c_double)), self.L)
         return out
+
+    def set_sq(self, grid, mesh, kpts, axis=2, ua=None, ub=None, ngroups=1, group=None):
+        """Wave-vectors (mesh indices) of the transverse structure factor sampled by sample_sq."""
+        g = np.ascontiguousarray(grid, dtype=np.uint32)
+        m = np.ascontiguousarray(mesh, dtype=np.uint32).reshape(-1, 3)
+        k = np.ascontiguousarray(kpts, dtype=np.uint32).reshape(-1, 3)
+        a = None if ua is None else np.ascontiguousarray(ua, dtype=np.float64)
+        b = None if ub is None else np.ascontiguousarray(ub, dtype=np.float64)
+        gr = None if group is None else np.ascontiguousarray(group, dtype=np.uint32)
+        _check(self.L.llg_set_sq(self.h, _p(g, C.c_uint32), _p(m, C.c_uint32), len(k), _p(k, C.c_uint32), int(axis),
+                                 _p(a, C.c_double), _p(b, C.c_double), int(ngroups), _p(gr, C.c_uint32)), self.L)
+        self._sq = (int(ngroups), len(k))
+
+    def sample_sq(self):
+        """Complex array [ngroups, nk]."""
+        out = np.zeros(self._sq + (2,))
+        _check(self.L.llg_sample_sq(self.h, _p(out, C.c_double)), self.L)
+        return out[..., 0] + 1j * out[..., 1]
 
     def autotune(self, nsteps=20):
         """Pick the FP32 Heun launch shape by timing the candidates on this system."""

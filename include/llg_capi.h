@@ -25,6 +25,8 @@
  *   llg_reduce          <- util::calc_mag method 0 (src/util_mag.cpp:44-63); energy has no reference counterpart
  *   llg_set_bins /      <- util::calc_mag methods 1-10 (src/util_mag.cpp:65-245) as device sums
  *   llg_reduce_bins
+ *   llg_set_sq /        <- the time-series S(k, t) sampler (src/timeseries.cpp:262-380): FFTW of the
+ *   llg_sample_sq          transverse spin grid read at chosen k, here a direct sum at those k
  */
 #ifndef LLG_CAPI_H
 #define LLG_CAPI_H
@@ -244,6 +246,19 @@ int llg_reduce(llg_ctx* ctx, double* m, double* energy);
  * (9), per unit-cell atom (10); the caller divides by counts where the reference does. */
 int llg_set_bins(llg_ctx* ctx, uint32_t nbins, const uint32_t* bin, const double* weight);
 int llg_reduce_bins(llg_ctx* ctx, double* out);
+
+/* Transverse spin structure factor at mesh wave-vectors: the device form of the spin-wave
+ * sampler of the time-series driver (src/timeseries.cpp:262-380, settings src/sf_glob.cpp).
+ * Per spin c_i = S_a(i) ua[i] + i S_b(i) ub[i], (a, b) the components transverse to the
+ * quantisation axis (axis 0: y, z; 1: x, z; 2: x, y); ua/ub carry the reference's
+ * per-sublattice sign transform uo (NULL = 1).  For k-point k (mesh indices kpts[k][3]):
+ *   out[(g*nk + k)*2 + {0,1}] = sum_{i: group[i] == g} c_i exp(-2 pi i sum_d kpts[k][d] mesh[i][d] / grid[d]),
+ * the FFTW_FORWARD convention of the reference's plan, unnormalised (group NULL: one group of
+ * every spin; group[i] >= ngroups leaves spin i out).  mesh (n x 3), group, ua, ub are in the
+ * reference spin order; mesh[i][d] < grid[d] and kpts[k][d] < grid[d]. */
+int llg_set_sq(llg_ctx* ctx, const uint32_t grid[3], const uint32_t* mesh, uint32_t nk, const uint32_t* kpts,
+               int axis, const double* ua, const double* ub, uint32_t ngroups, const uint32_t* group);
+int llg_sample_sq(llg_ctx* ctx, double* out);
 
 /* Run nsteps steps and record the sublattice magnetisation on the device every
  * `every` steps, before the step, as the drivers sample (src/simple.cpp:87-95).

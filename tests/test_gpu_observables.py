@@ -69,3 +69,39 @@ def test_bins_skip_and_state(gpu):
     g.set_bins(1, b)
     r = g.reduce_bins()
     assert np.allclose(r[0], s0[:, ::3].sum(axis=1), atol=1e-12)
+
+
+@pytest.mark.parametrize("prec,axis", [("fp64", 2), ("fp32", 0)])
+def test_structure_factor_matches_fft(gpu, prec, axis):
+    """Device S(k) at chosen k equals numpy's FFT of the transverse spin grid read there
+    (the reference's FFTW_FORWARD plan over the dim x Nk mesh)."""
+    sysd, s0, mesh, sub, atom, mu = _system(prec)
+    grid = tuple(int(x) for x in mesh.max(axis=0) + 1)
+    g = sllg.Context(sysd)
+    g.set_spins(s0)
+    g.set_temperature(100.0)
+    g.step(0, 20)
+    S = g.get_spins()
+    rng = np.random.default_rng(5)
+    kpts = np.stack([rng.integers(0, grid[d], 40) for d in range(3)], axis=1)
+    kpts[0] = 0
+    ua = np.where(sub == 0, 1.0, -1.0)
+    ub = np.where(atom == 3, -1.0, 1.0)
+    a, b = {0: (1, 2), 1: (0, 2), 2: (0, 1)}[axis]
+    # total and per sublattice
+    g.set_sq(grid, mesh, kpts, axis=axis, ua=ua, ub=ub)
+    tot = g.sample_sq()[0]
+    g.set_sq(grid, mesh, kpts, axis=axis, ua=ua, ub=ub, ngroups=2, group=sub)
+    per = g.sample_sq()
+    for grp, dev in ((None, tot), (0, per[0]), (1, per[1])):
+        sel = np.ones(sysd.n, bool) if grp is None else (sub == grp)
+        s3d = np.zeros(grid, complex)
+        m = mesh[sel]
+        s3d[m[:, 0], m[:, 1], m[:, 2]] = S[a][sel] * ua[sel] + 1j * S[b][sel] * ub[sel]
+        ref = np.fft.fftn(s3d)[kpts[:, 0], kpts[:, 1], kpts[:, 2]]
+        err = np.abs(dev - ref).max() / max(1.0, np.abs(ref).max())
+        print(prec, axis, grp, err)
+        assert err <= 1e-11
+    assert np.allclose(per.sum(axis=0), tot, rtol=0, atol=1e-9)
+    with pytest.raises(sllg.LLGError):
+        g.set_sq(grid, mesh, np.array([[grid[0], 0, 0]]), axis=axis)
