@@ -402,6 +402,7 @@ static int create_impl(const llg_desc* d, const std::shared_ptr<ExchStore>& shar
         }
         if (!proto) SLLG_TRY(build_exchange_fp32(c, d->exch));
     }
+    if (!proto && d->four.nq) SLLG_TRY(build_fourspin(c, d->four));
     c->ex32 = c->xs->ex32;
     c->ex64 = c->xs->ex64;
     c->perm = c->xs->perm;

@@ -74,6 +74,11 @@ template <class V> struct Exch {
     uint64_t nent = 0;                    // total entries (SELL_RAW value stride)
     uint32_t ncell = 0;                   // rows per sublattice block of the device order
     uint32_t dbits = 0;                   // dictionary-index bits of a packed entry
+    // four-spin quartets of each row (same row order as the spins of this path)
+    const uint32_t* fs_xadj = nullptr;    // [n+1]; null = term off
+    const uint4* fs_jkl = nullptr;        // (j, k, l, 0)
+    V fs_jq = V(0);
+    int fs_div3 = 1;
 };
 
 // Device order of the FP32 sliced-ELL path.  The reference orders spins basis-atom
@@ -132,13 +137,15 @@ struct KernelTimer;  // capi.hip
 struct ExchStore {
     int device = 0, prec = 0;
     uint32_t n = 0;
-    DevBuf a, b, c, d, e, dict64;
+    DevBuf a, b, c, d, e, dict64, fs_x, fs_q;
     Exch<float> ex32;
     Exch<double> ex64;
     Perm perm;
     std::string encoding;
     HostExch h;  // scalars kept; the large host vectors are released after upload
-    uint64_t bytes() const { return a.bytes + b.bytes + c.bytes + d.bytes + e.bytes + dict64.bytes; }
+    uint64_t bytes() const {
+        return a.bytes + b.bytes + c.bytes + d.bytes + e.bytes + dict64.bytes + fs_x.bytes + fs_q.bytes;
+    }
 };
 
 }  // namespace sllg

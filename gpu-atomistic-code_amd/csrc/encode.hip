@@ -417,4 +417,40 @@ int build_exchange_fp32(llg_ctx* c, const llg_exchange& e) {
     return 0;
 }
 
+// Four-spin quartets: rows and (j, k, l) in the device order of this path; both exchange
+// views (the kernels' precision and the FP64 one of the reductions) point at them.
+int build_fourspin(llg_ctx* c, const llg_fourspin& f) {
+    ExchStore& X = *c->xs;
+    const uint32_t n = c->n;
+    if (!f.xadj || !f.jkl) SLLG_FAIL(LLG_E_INVALID, "four-spin term without quartets");
+    if (f.xadj[0] != 0 || f.xadj[n] != f.nq) SLLG_FAIL(LLG_E_INVALID, "four-spin row pointers do not span the quartets");
+    for (uint32_t i = 0; i < n; ++i)
+        if (f.xadj[i + 1] < f.xadj[i]) SLLG_FAIL(LLG_E_INVALID, "four-spin row pointers decrease at row %u", i);
+    for (uint64_t q = 0; q < 3 * f.nq; ++q)
+        if (f.jkl[q] >= n) SLLG_FAIL(LLG_E_INVALID, "four-spin quartet %llu names spin %u", (unsigned long long)(q / 3),
+                                    f.jkl[q]);
+    const Perm P = X.perm;
+    std::vector<uint32_t> xadj(n + 1, 0);
+    std::vector<uint4> q4(std::max<uint64_t>(1, f.nq));
+    for (uint32_t d = 0; d < n; ++d) {
+        const uint32_t i = P.to_ref(d);
+        xadj[d + 1] = xadj[d] + (f.xadj[i + 1] - f.xadj[i]);
+    }
+    for (uint32_t d = 0; d < n; ++d) {
+        const uint32_t i = P.to_ref(d);
+        uint32_t o = xadj[d];
+        for (uint32_t q = f.xadj[i]; q < f.xadj[i + 1]; ++q, ++o)
+            q4[o] = make_uint4(P.to_dev(f.jkl[3 * (size_t)q]), P.to_dev(f.jkl[3 * (size_t)q + 1]),
+                               P.to_dev(f.jkl[3 * (size_t)q + 2]), 0u);
+    }
+    SLLG_TRY(upload(X.fs_x, xadj.data(), xadj.size(), c->stream));
+    SLLG_TRY(upload(X.fs_q, q4.data(), q4.size(), c->stream));
+    X.ex32.fs_xadj = X.ex64.fs_xadj = X.fs_x.as<uint32_t>();
+    X.ex32.fs_jkl = X.ex64.fs_jkl = X.fs_q.as<uint4>();
+    X.ex32.fs_jq = (float)f.jq;
+    X.ex64.fs_jq = f.jq;
+    X.ex32.fs_div3 = X.ex64.fs_div3 = f.cpu_form ? 0 : 1;
+    return 0;
+}
+
 }  // namespace sllg

@@ -60,6 +60,31 @@ __device__ __forceinline__ bool stencil_neighbour(const Exch<V>& e, uint32_t i, 
 // Exchange sum of row i in the reference formats (CSR, DIA) and the periodic stencil,
 // reading spins from S (SoA, precision SpinT) and couplings of precision V,
 // accumulating in R.
+// Four-spin field of row i (cufields::CSpMV_CSR_FourSpin, src/cufields.cu:59-90), from zero:
+// each quartet subtracts (jq (S_j (S_k.S_l) + S_k (S_j.S_l) + S_l (S_k.S_j))) / 3 (no division
+// in the reference CPU form).  Same operation order as the oracle.
+template <class R, class V, class SpinT>
+__device__ __forceinline__ void fourspin_row(const Exch<V>& e, uint32_t i, const SpinT* __restrict__ S, R h[3]) {
+    const uint32_t n = e.n;
+    h[0] = h[1] = h[2] = R(0);
+    const R jq = (R)e.fs_jq;
+    for (uint32_t q = e.fs_xadj[i]; q < e.fs_xadj[i + 1]; ++q) {
+        const uint4 t = e.fs_jkl[q];
+        const R sj[3] = {(R)S[t.x], (R)S[n + t.x], (R)S[2 * (size_t)n + t.x]};
+        const R sk[3] = {(R)S[t.y], (R)S[n + t.y], (R)S[2 * (size_t)n + t.y]};
+        const R sl[3] = {(R)S[t.z], (R)S[n + t.z], (R)S[2 * (size_t)n + t.z]};
+        const R skdotsl = sk[0] * sl[0] + sk[1] * sl[1] + sk[2] * sl[2];
+        const R sjdotsl = sj[0] * sl[0] + sj[1] * sl[1] + sj[2] * sl[2];
+        const R skdotsj = sk[0] * sj[0] + sk[1] * sj[1] + sk[2] * sj[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            R v = jq * (sj[a] * skdotsl + sk[a] * sjdotsl + sl[a] * skdotsj);
+            if (e.fs_div3) v = v / R(3);
+            h[a] -= v;
+        }
+    }
+}
+
 template <int NC, class R, class V, class SpinT>
 __device__ __forceinline__ void exch_row_ref(const Exch<V>& e, uint32_t i, const SpinT* __restrict__ S,
                                              R dot[3]) {

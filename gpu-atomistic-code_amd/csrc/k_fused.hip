@@ -181,6 +181,13 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const 
         exch_row_pack_fast<NC>(a.ex, i, src, sdict, h);
     else
         exch_any<FMT, NC, float, float, float>(a.ex, i, src, h);
+    if (a.ex.fs_xadj) {
+        float h4[3];
+        fourspin_row<float, float, float>(a.ex, i, src, h4);
+        h[0] += h4[0];
+        h[1] += h4[1];
+        h[2] += h4[2];
+    }
     if (a.dem) {
         h[0] += a.dem[i];
         h[1] += a.dem[n + i];
@@ -407,6 +414,11 @@ __global__ __launch_bounds__(kTPB) void k_fused_detfield(DetArgs a, const float*
     const uint32_t n = a.n;
     double h[3];
     exch_any<FMT, NC, double, double, float>(a.ex, i, Sp, h);
+    if (a.ex.fs_xadj) {
+        double h4[3];
+        fourspin_row<double, double, float>(a.ex, i, Sp, h4);
+        for (int c = 0; c < 3; ++c) h[c] += h4[c];
+    }
     const double s[3] = {(double)Sp[i], (double)Sp[n + i], (double)Sp[2 * (size_t)n + i]};
     for (int c = 0; c < 3; ++c) {
         if (a.dem) h[c] += (double)a.dem[(size_t)c * n + i];

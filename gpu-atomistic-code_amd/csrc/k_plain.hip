@@ -39,7 +39,14 @@ __global__ __launch_bounds__(kTPB) void k_plain_spmv(Exch<double> e, const doubl
     if (i >= n) return;
     double dot[3];
     exch_row_ref<NC, double, double, double>(e, i, S, dot);
-    for (int a = 0; a < 3; ++a) H[(size_t)a * n + i] = (dem ? dem[(size_t)a * n + i] : 0.0) + dot[a];
+    double hx[3];
+    for (int a = 0; a < 3; ++a) hx[a] = (dem ? dem[(size_t)a * n + i] : 0.0) + dot[a];
+    if (e.fs_xadj) {  // CH += four-spin field (src/cuda.cu:143-146)
+        double h4[3];
+        fourspin_row<double, double, double>(e, i, S, h4);
+        for (int a = 0; a < 3; ++a) hx[a] = hx[a] + h4[a];
+    }
+    for (int a = 0; a < 3; ++a) H[(size_t)a * n + i] = hx[a];
 }
 
 __global__ __launch_bounds__(kTPB) void k_plain_noise(uint32_t n, uint64_t seed, uint64_t stream, uint64_t t,

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(kTPB) void k_sum_partials(uint32_t nblocks, uint32_
 }
 
 // Per-spin energy (J) in FP64:
-//   mu mu_B [ -1/2 S.(H_ex + H_dip + H_quad) - 1/4 S.H_cubic - S.(B + H_stag) ]
+//   mu mu_B [ -1/2 S.(H_ex + H_dip + H_quad) - 1/4 S.(H_cubic + H_4spin) - S.(B + H_stag) ]
 // where H_quad collects the terms linear in S (uniaxial, K2, K4).
 struct EnergyArgs {
     uint32_t n;
@@ -163,7 +163,13 @@ __global__ __launch_bounds__(kTPB) void k_energy_partial(EnergyArgs a, const Spi
         const double eq = s[0] * hq[0] + s[1] * hq[1] + s[2] * hq[2];
         const double ec = sp.kc * (s[0] * s[0] * s[0] * s[0] + s[1] * s[1] * s[1] * s[1] + s[2] * s[2] * s[2] * s[2]);
         const double ez = s[0] * hz[0] + s[1] * hz[1] + s[2] * hz[2];
-        acc += a.mu[spx] * LLG_MU_B * (-0.5 * ep - 0.5 * eq - 0.25 * ec - ez);
+        double e4 = 0.0;  // four-spin: quartic in the spins, -1/4 S.H4
+        if (a.ex.fs_xadj) {
+            double h4[3];
+            fourspin_row<double, double, SpinT>(a.ex, i, S, h4);
+            e4 = s[0] * h4[0] + s[1] * h4[1] + s[2] * h4[2];
+        }
+        acc += a.mu[spx] * LLG_MU_B * (-0.5 * ep - 0.5 * eq - 0.25 * ec - 0.25 * e4 - ez);
     }
     const double r = block_sum(acc, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = r;

@@ -54,5 +54,6 @@ int dipolar_refresh(llg_ctx* c, const void* S, void* out);  // spins of the path
 // Encodings (encode.cpp)
 int build_exchange_fp64(llg_ctx* c, const llg_exchange& e);
 int build_exchange_fp32(llg_ctx* c, const llg_exchange& e);
+int build_fourspin(llg_ctx* c, const llg_fourspin& f);
 
 }  // namespace sllg

@@ -67,13 +67,18 @@ class Dipolar(C.Structure):
                 ("mesh", C.POINTER(C.c_uint32))]
 
 
+class FourSpin(C.Structure):
+    _fields_ = [("nq", C.c_uint64), ("xadj", C.POINTER(C.c_uint32)), ("jkl", C.POINTER(C.c_uint32)),
+                ("jq", C.c_double), ("cpu_form", C.c_int32), ("pad_", C.c_uint32)]
+
+
 class Desc(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("n", C.c_uint32), ("n_species", C.c_uint32),
                 ("n_sublattice", C.c_uint32), ("species", C.POINTER(Species)),
                 ("spin_species", C.POINTER(C.c_uint32)), ("anis", Anisotropy), ("exch", Exchange),
                 ("dip", Dipolar), ("dt", C.c_double), ("scheme", C.c_int32), ("precision", C.c_int32),
                 ("noise", C.c_int32), ("update", C.c_uint32), ("seed", C.c_uint64), ("stream", C.c_uint64),
-                ("flags", C.c_uint32), ("device", C.c_int32)]
+                ("flags", C.c_uint32), ("device", C.c_int32), ("four", FourSpin)]
 
 
 class Info(C.Structure):
@@ -334,6 +339,7 @@ class System:
     stream: int = 0
     flags: int = 0
     device: int = 0
+    four: dict = field(default_factory=dict)  # xadj [n+1], jkl [nq,3], jq (T), cpu_form
 
     def desc(self) -> Desc:
         d = Desc()
@@ -411,6 +417,16 @@ class System:
         d.stream = self.stream
         d.flags = self.flags
         d.device = self.device
+        f = self.four
+        if f and len(f.get("jkl", [])):
+            fx = np.ascontiguousarray(f["xadj"], dtype=np.uint32)
+            fj = np.ascontiguousarray(f["jkl"], dtype=np.uint32).reshape(-1, 3)
+            self._keep += [fx, fj]
+            d.four.nq = fj.shape[0]
+            d.four.xadj = _p(fx, C.c_uint32)
+            d.four.jkl = _p(fj, C.c_uint32)
+            d.four.jq = float(f["jq"])
+            d.four.cpu_form = int(f.get("cpu_form", 0))
         return d
 
 

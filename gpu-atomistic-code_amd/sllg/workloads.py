@@ -137,3 +137,55 @@ def sc_film_dip(Lx=128, Ly=128, Lz=16, precision="fp32", J=3e-21, a=3e-10, lam=0
                   dt=dt, scheme=scheme, precision=precision, noise="counter", update=update, seed=seed,
                   stream=stream)
     return sysd, random_spins(n, seed, seed + 100)
+
+
+def plaquette_quartets(dims, pbc=(1, 1, 1)):
+    """Four-spin quartets of a simple-cubic lattice (spin index (x*d1 + y)*d2 + z): every
+    elementary square plaquette of the xy, yz and zx planes, listed once from the point of
+    view of each of its four spins (j, k, l = the other three in cyclic order).  Returns
+    (xadj [n+1], jkl [nq, 3]) for System.four."""
+    d0, d1, d2 = dims
+    n = d0 * d1 * d2
+
+    def idx(x, y, z):
+        return (x * d1 + y) * d2 + z
+
+    rows = [[] for _ in range(n)]
+    planes = ((0, 1), (1, 2), (2, 0))
+    for x in range(d0):
+        for y in range(d1):
+            for z in range(d2):
+                for (a, b) in planes:
+                    corners = []
+                    ok = True
+                    for da, db in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        c = [x, y, z]
+                        c[a] += da
+                        c[b] += db
+                        for ax in range(3):
+                            if c[ax] >= dims[ax]:
+                                if not pbc[ax]:
+                                    ok = False
+                                c[ax] %= dims[ax]
+                        corners.append(idx(*c))
+                    if not ok or len(set(corners)) < 4:
+                        continue
+                    for m in range(4):
+                        rows[corners[m]].append([corners[(m + 1) % 4], corners[(m + 2) % 4], corners[(m + 3) % 4]])
+    xadj = np.zeros(n + 1, np.uint32)
+    xadj[1:] = np.cumsum([len(r) for r in rows])
+    jkl = np.array([q for r in rows for q in r], np.uint32).reshape(-1, 3)
+    return xadj, jkl
+
+
+def quartets_from_sets(n, sets):
+    """xadj/jkl of the four-spin term for arbitrary 4-spin sets, each listed from every member."""
+    rows = [[] for _ in range(n)]
+    for q in sets:
+        q = [int(v) for v in q]
+        for m in range(4):
+            rows[q[m]].append([q[(m + 1) % 4], q[(m + 2) % 4], q[(m + 3) % 4]])
+    xadj = np.zeros(n + 1, np.uint32)
+    xadj[1:] = np.cumsum([len(r) for r in rows])
+    jkl = np.array([v for r in rows for v in r], np.uint32).reshape(-1, 3)
+    return xadj, jkl

@@ -161,6 +161,22 @@ typedef struct llg_dipolar {
     const uint32_t* mesh; /* [n][3] mesh coordinates lu(i,0..2) */
 } llg_dipolar;
 
+/* Four-spin exchange, reference cufields::CSpMV_CSR_FourSpin (src/cufields.cu:59-90; host form
+ * src/llgCPU.cpp:142-173).  Spin i has quartets (j, k, l) at xadj[i]..xadj[i+1]-1 and, after the
+ * pair exchange, receives
+ *   H_i -= (jq (S_j (S_k.S_l) + S_k (S_j.S_l) + S_l (S_k.S_j))) / 3
+ * with jq = JQ/(mu_0 mu_B) in tesla, one constant as in the GPU kernel (species 0's value,
+ * src/cufields.cu:38-40).  cpu_form = 1 drops the /3 as the reference CPU path does
+ * (src/llgCPU.cpp:163-165).  nq = 0 turns the term off.  Energy: -1/4 mu mu_B S.H4 per spin. */
+typedef struct llg_fourspin {
+    uint64_t nq;
+    const uint32_t* xadj; /* [n+1] */
+    const uint32_t* jkl;  /* [nq][3] */
+    double jq;
+    int32_t cpu_form;
+    uint32_t pad_;
+} llg_fourspin;
+
 typedef struct llg_desc {
     uint32_t abi_version; /* LLG_ABI_VERSION */
     uint32_t n;           /* number of spins */
@@ -180,6 +196,7 @@ typedef struct llg_desc {
     uint64_t stream;      /* counter-noise stream id (replica / request) */
     uint32_t flags;
     int32_t device;       /* HIP device ordinal */
+    llg_fourspin four;    /* optional four-spin term (zeroed by llg_desc_init) */
 } llg_desc;
 
 typedef struct llg_ctx llg_ctx;

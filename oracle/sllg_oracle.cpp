@@ -51,6 +51,10 @@ struct Ctx {
     std::vector<uint32_t> scount;
     std::vector<int32_t> sent;
     std::vector<double> svals;
+    // four-spin quartets (j, k, l) per row, jq in tesla
+    std::vector<uint32_t> fs_xadj, fs_jkl;
+    double fs_jq = 0.0;
+    int fs_cpu = 0;
     // dipolar
     bool dip = false;
     int every_stage = 0;
@@ -263,10 +267,37 @@ inline void thermal(const Ctx& c, uint32_t i, double th[3]) {
     th[2] = ((c.opts & OR_COMPAT_LLGCPU) ? c.XI[0][i] : c.XI[2][i]) * TP;
 }
 
+// Four-spin field of row i, cufields::CSpMV_CSR_FourSpin (src/cufields.cu:59-90): from zero,
+// each quartet subtracts (JQ (S_j (S_k.S_l) + S_k (S_j.S_l) + S_l (S_k.S_j))) / 3; the
+// reference CPU form (src/llgCPU.cpp:163-165) has no division.
+void fourspin_row(const Ctx& c, uint32_t i, const std::vector<double>* sp, double h[3]) {
+    h[0] = h[1] = h[2] = 0.0;
+    for (uint32_t q = c.fs_xadj[i]; q < c.fs_xadj[i + 1]; ++q) {
+        const uint32_t j = c.fs_jkl[3 * (size_t)q], k = c.fs_jkl[3 * (size_t)q + 1], l = c.fs_jkl[3 * (size_t)q + 2];
+        const double sj[3] = {sp[0][j], sp[1][j], sp[2][j]};
+        const double sk[3] = {sp[0][k], sp[1][k], sp[2][k]};
+        const double sl[3] = {sp[0][l], sp[1][l], sp[2][l]};
+        const double skdotsl = sk[0] * sl[0] + sk[1] * sl[1] + sk[2] * sl[2];
+        const double sjdotsl = sj[0] * sl[0] + sj[1] * sl[1] + sj[2] * sl[2];
+        const double skdotsj = sk[0] * sj[0] + sk[1] * sj[1] + sk[2] * sj[2];
+        for (int a = 0; a < 3; ++a) {
+            double v = c.fs_jq * (sj[a] * skdotsl + sk[a] * sjdotsl + sl[a] * skdotsj);
+            if (!c.fs_cpu) v = v / 3.0;
+            h[a] -= v;
+        }
+    }
+}
+
+// CH = HDemag + exchange (src/cufields.cu:201-203), then CH += four-spin (src/cuda.cu:143-146).
 inline void exch_plus_demag(const Ctx& c, uint32_t i, const std::vector<double>* sp, double hex[3]) {
     double dot[3];
     exch_row(c, i, sp, dot);
     for (int a = 0; a < 3; ++a) hex[a] = (c.dip ? c.DEM[a][i] : 0.0) + dot[a];
+    if (!c.fs_xadj.empty()) {
+        double h4[3];
+        fourspin_row(c, i, sp, h4);
+        for (int a = 0; a < 3; ++a) hex[a] = hex[a] + h4[a];
+    }
 }
 
 // One Heun step at the GPU semantics: llgGPU (src/cuda.cu:43-258), CHeun1/CHeun2.
@@ -630,6 +661,19 @@ int or_create(const llg_desc* d, uint32_t options, or_ctx** out) {
         delete c;
         return LLG_E_INVALID;
     }
+    if (d->four.nq) {
+        const llg_fourspin& f = d->four;
+        if (!f.xadj || !f.jkl) { delete c; return LLG_E_INVALID; }
+        c->fs_xadj.assign(f.xadj, f.xadj + n + 1);
+        c->fs_jkl.assign(f.jkl, f.jkl + 3 * (size_t)f.nq);
+        c->fs_jq = f.jq;
+        c->fs_cpu = f.cpu_form;
+        if (c->fs_xadj[0] != 0 || c->fs_xadj[n] != f.nq) { delete c; return LLG_E_INVALID; }
+        for (uint32_t i = 0; i < n; ++i)
+            if (c->fs_xadj[i + 1] < c->fs_xadj[i]) { delete c; return LLG_E_INVALID; }
+        for (uint32_t v : c->fs_jkl)
+            if (v >= n) { delete c; return LLG_E_INVALID; }
+    }
     if (d->dip.enabled) {
         if (!d->dip.mesh) { delete c; return LLG_E_INVALID; }
         c->dip = true;
@@ -737,7 +781,8 @@ int or_get_dipolar(or_ctx* c, double* hx, double* hy, double* hz) {
     return 0;
 }
 // util::calc_mag method 0 (src/util_mag.cpp:44-63) and the energy
-// E = sum_i mu_i mu_B [ -1/2 S.(H_ex + H_dip + H_quadratic-anisotropy) - 1/4 S.H_cubic - S.(B + H_stag) ].
+// E = sum_i mu_i mu_B [ -1/2 S.(H_ex + H_dip + H_quadratic-anisotropy) - 1/4 S.(H_cubic + H_4spin)
+//     - S.(B + H_stag) ].
 int or_reduce(or_ctx* c, double* m, double* energy) {
     if (!c || !m) return LLG_E_INVALID;
     std::vector<double> acc((size_t)c->nsub * 3, 0.0);
@@ -787,11 +832,17 @@ int or_reduce(or_ctx* c, double* m, double* energy) {
                 for (int a = 0; a < 3; ++a) hq[a] += 2.0 * an.k4par / c->basemm * sda * da[a];
             }
             const double e_q = s[0] * hq[0] + s[1] * hq[1] + s[2] * hq[2];
+            double e_4 = 0.0;  // quartic in the spins: -1/4 S.H4
+            if (!c->fs_xadj.empty()) {
+                double h4[3];
+                fourspin_row(*c, i, c->S, h4);
+                e_4 = s[0] * h4[0] + s[1] * h4[1] + s[2] * h4[2];
+            }
             const double kc = c->kc_sp[sp];
             const double e_c = kc * (s[0] * s[0] * s[0] * s[0] + s[1] * s[1] * s[1] * s[1] + s[2] * s[2] * s[2] * s[2]);
             double e_z = 0.0;
             for (int a = 0; a < 3; ++a) e_z += s[a] * (c->B[a] + c->STG[a][i]);
-            E += mm * (-0.5 * e_pair - 0.5 * e_q - 0.25 * e_c - e_z);
+            E += mm * (-0.5 * e_pair - 0.5 * e_q - 0.25 * e_c - 0.25 * e_4 - e_z);
         }
         *energy = E;
     }

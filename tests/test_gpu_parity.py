@@ -330,3 +330,41 @@ def test_slice_sharing_is_exact(gpu, kind):
     assert np.array_equal(res[0], res[2]) and np.array_equal(res[1], res[3])
     g, o = run_pair(sysd, s0, 60, T=0.0, B=(0.0, 0.3, 0.5))
     assert _maxdiff(g.get_spins(), o.get_spins()) <= 1e-4
+
+
+@pytest.mark.parametrize("cpu_form", [0, 1])
+def test_fourspin_plain_bit_parity_and_fp32(gpu, cpu_form):
+    """Four-spin term (plaquette quartets): FP64 plain path bit-identical to the oracle at
+    T = 0 and with injected noise; FP32 path within float rounding; energies agree."""
+    L = 6
+    xadj, jkl = W.plaquette_quartets((L, L, L))
+    for prec, noise, tol in (("fp64", "counter", TOL64), ("fp64", "injected", TOL64), ("fp32", "counter", 1e-4)):
+        sysd, s0, _ = W.sc_ferro(L=L, precision=prec, noise=noise)
+        sysd.four = dict(xadj=xadj, jkl=jkl, jq=25.0, cpu_form=cpu_form)
+        xi = None
+        T = 0.0
+        if noise == "injected":
+            xi = np.random.default_rng(3).standard_normal((60, 3, sysd.n))
+            T = 300.0
+        g, o = run_pair(sysd, s0, 60, T=T, B=(0.0, 0.2, 0.4), xi=xi)
+        d = _maxdiff(g.get_spins(), o.get_spins())
+        print("fourspin", cpu_form, prec, noise, d)
+        assert d <= tol
+        _, eg = g.reduce()
+        _, eo = o.reduce()
+        assert abs(eg - eo) <= (1e-12 if prec == "fp64" else 1e-5) * abs(eo)
+
+
+def test_fourspin_sublattice_order(gpu):
+    """Four-spin quartets on the sublattice-major FP32 path (indices permuted at the boundary)."""
+    sysd, s0 = _fcc_multi(L=4)
+    rng = np.random.default_rng(12)
+    sets = [rng.choice(sysd.n, 4, replace=False) for _ in range(300)]
+    xadj, jkl = W.quartets_from_sets(sysd.n, sets)
+    sysd.four = dict(xadj=xadj, jkl=jkl, jq=30.0)
+    g, o = run_pair(sysd, s0, 50, T=0.0, B=(0.1, 0.0, 0.3))
+    assert "-t4-" in g.info()["exch_encoding"]
+    d = _maxdiff(g.get_spins(), o.get_spins())
+    print("fourspin fcc sublattice-major", d)
+    assert d <= 1e-4
+    assert _maxdiff(g.field(), o.field()) <= 1e-4 * np.abs(o.field()).max()

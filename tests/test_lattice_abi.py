@@ -149,8 +149,8 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", offsetof(llg_desc, anis), offsetof(llg_desc, exch), offsetof(llg_desc, dip),
          offsetof(llg_desc, dt), offsetof(llg_desc, seed), offsetof(llg_desc, device), offsetof(llg_exchange, stencil_vals),
          offsetof(llg_dipolar, mesh));
-  printf("%zu %zu %zu %zu\n", sizeof(llg_ucf_atom), offsetof(llg_ucf_atom, element), sizeof(llg_tune_result),
-         offsetof(llg_tune_result, candidates));
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(llg_ucf_atom), offsetof(llg_ucf_atom, element), sizeof(llg_tune_result),
+         offsetof(llg_tune_result, candidates), sizeof(llg_fourspin), offsetof(llg_desc, four));
   return 0;
 }
 '''
@@ -169,7 +169,8 @@ int main(void) {
                     sllg.Dipolar.mesh.offset]
     from sllg import refinput
     assert [int(x) for x in c3.split()] == [C.sizeof(refinput.UcfAtom), refinput.UcfAtom.element.offset,
-                                            C.sizeof(sllg.TuneResult), sllg.TuneResult.candidates.offset]
+                                            C.sizeof(sllg.TuneResult), sllg.TuneResult.candidates.offset,
+                                            C.sizeof(sllg.FourSpin), sllg.Desc.four.offset]
 
 
 def _create_rc(sysd):

@@ -339,3 +339,60 @@ def test_spin_wave_frequency_matches_lswt():
     dph = (ph1 - ph0) % (2 * math.pi)
     expect = (omega * tp) % (2 * math.pi)
     assert abs(dph - expect) < 1e-3 * omega * tp
+
+
+# ----------------------------------------------------------------------------- four-spin term
+
+def _fourspin_system(L=3, cpu_form=0, jq=40.0):
+    sysd, s0, _ = W.sc_ferro(L=L, precision="fp64")
+    xadj, jkl = W.plaquette_quartets((L, L, L))
+    sysd.four = dict(xadj=xadj, jkl=jkl, jq=jq, cpu_form=cpu_form)
+    return sysd, s0
+
+
+def test_fourspin_field_is_minus_energy_gradient():
+    """Plaquette quartets listed from every member: the four-spin field equals
+    -dE/dS / (mu mu_B) of the oracle energy (a wrong neighbour index, a dropped pairing or
+    a sign slip breaks the identity)."""
+    sysd, s0 = _fourspin_system(L=3)
+    base, _ = _fourspin_system(L=3)
+    base.four = {}
+    o = pyoracle.OracleContext(sysd)
+    ob = pyoracle.OracleContext(base)
+    o.set_spins(s0)
+    ob.set_spins(s0)
+    h4 = o.field() - ob.field()
+    assert np.abs(h4).max() > 1.0
+    mu = 2.22 * sllg.MU_B
+    eps = 1e-6
+    for i in (0, 5, 13, 26):
+        for a in range(3):
+            s = s0.copy()
+            s[a, i] += eps
+            o.set_spins(s)
+            ob.set_spins(s)
+            Ep = o.reduce()[1] - ob.reduce()[1]
+            s[a, i] -= 2 * eps
+            o.set_spins(s)
+            ob.set_spins(s)
+            Em = o.reduce()[1] - ob.reduce()[1]
+            g = -(Ep - Em) / (2 * eps) / mu
+            assert abs(g - h4[a, i]) < 1e-6 * max(1.0, abs(h4[a, i])), (i, a, g, h4[a, i])
+
+
+def test_fourspin_uniform_state_closed_form():
+    """Parallel unit spins: each quartet contributes 3 S, so H4 = -jq nq_i S with the GPU
+    kernel's /3 and -3 jq nq_i S in the CPU form (src/cufields.cu:76-78 vs src/llgCPU.cpp:163-165)."""
+    for cpu_form, f in ((0, 1.0), (1, 3.0)):
+        sysd, s0 = _fourspin_system(L=4, cpu_form=cpu_form, jq=7.0)
+        base, _ = _fourspin_system(L=4)
+        base.four = {}
+        u = np.array([0.36, 0.48, 0.8])
+        s = np.tile(u[:, None], (1, sysd.n))
+        o, ob = pyoracle.OracleContext(sysd), pyoracle.OracleContext(base)
+        o.set_spins(s)
+        ob.set_spins(s)
+        h4 = o.field() - ob.field()
+        nq = np.diff(sysd.four["xadj"]).astype(float)
+        assert np.all(nq == 12)  # 12 plaquettes touch every site of a periodic sc lattice
+        assert np.allclose(h4, -f * 7.0 * nq[None, :] * u[:, None], rtol=0, atol=1e-12)
