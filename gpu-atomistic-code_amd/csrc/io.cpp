@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <unordered_map>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -539,6 +540,118 @@ int llg_csr_write(const char* path, uint32_t n, const uint32_t* xadj, const uint
     for (uint64_t q = 0; q < nnz; ++q)
         std::fprintf(o, "%u\t%.17g\t%.17g\t%.17g\t", adjncy[q], vals3[q], vals3[nnz + q], vals3[2 * nnz + q]);
     std::fprintf(o, "\n");
+    if (std::ferror(o)) IO_FAIL(LLG_E_INVALID, "write error on %s", path);
+    return 0;
+}
+
+int llg_read_spins_text(const char* path, uint32_t n, double* sx, double* sy, double* sz) {
+    if (!path || !sx || !sy || !sz) IO_FAIL(LLG_E_INVALID, "null argument");
+    std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "rb"));
+    if (!f) IO_FAIL(LLG_E_INVALID, "could not open spin file %s", path);
+    Tokens tk(f.get());
+    std::string t;
+    uint64_t cnt = 0;
+    if (!tk.next(t) || !parse_u64(t, &cnt)) IO_FAIL(LLG_E_INVALID, "%s: could not read the number of spins", path);
+    if (cnt != n) IO_FAIL(LLG_E_INVALID, "%s: %llu spins in the file, %u in the system", path, (unsigned long long)cnt, n);
+    for (uint32_t i = 0; i < n; ++i) {
+        double v[3];
+        for (int c = 0; c < 3; ++c)
+            if (!tk.next(t) || !parse_f64(t, &v[c])) IO_FAIL(LLG_E_INVALID, "%s: bad spin %u", path, i);
+        const double m = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (m < 1e-12) IO_FAIL(LLG_E_INVALID, "%s: spin %u has zero length", path, i);
+        sx[i] = v[0] / m;
+        sy[i] = v[1] / m;
+        sz[i] = v[2] / m;
+    }
+    return 0;
+}
+
+int llg_write_spins_text(const char* path, uint32_t n, const double* sx, const double* sy, const double* sz) {
+    if (!path || !sx || !sy || !sz) IO_FAIL(LLG_E_INVALID, "null argument");
+    std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "w"));
+    if (!f) IO_FAIL(LLG_E_INVALID, "could not open %s for writing", path);
+    std::fprintf(f.get(), "%u\n", n);
+    for (uint32_t i = 0; i < n; ++i) std::fprintf(f.get(), "%.17g\t%.17g\t%.17g\n", sx[i], sy[i], sz[i]);
+    if (std::ferror(f.get())) IO_FAIL(LLG_E_INVALID, "write error on %s", path);
+    return 0;
+}
+
+int llg_read_grid(const char* path, uint32_t n, const uint32_t* mesh, double* sx, double* sy, double* sz) {
+    if (!path || !mesh || !sx || !sy || !sz) IO_FAIL(LLG_E_INVALID, "null argument");
+    std::unordered_map<uint64_t, uint32_t> site;
+    site.reserve(n * 2);
+    auto key = [](uint64_t x, uint64_t y, uint64_t z) { return (x << 42) | (y << 21) | z; };
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t* m = mesh + 3 * (size_t)i;
+        if (m[0] >= (1u << 21) || m[1] >= (1u << 21) || m[2] >= (1u << 21))
+            IO_FAIL(LLG_E_INVALID, "mesh coordinate of spin %u too large", i);
+        site[key(m[0], m[1], m[2])] = i;
+    }
+    std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "rb"));
+    if (!f) IO_FAIL(LLG_E_INVALID, "could not open grid file %s", path);
+    Tokens tk(f.get());
+    std::string t;
+    uint64_t cnt = 0;
+    if (!tk.next(t) || !parse_u64(t, &cnt)) IO_FAIL(LLG_E_INVALID, "%s: could not read the number of spins", path);
+    if (cnt != n) IO_FAIL(LLG_E_INVALID, "%s: %llu spins in the file, %u in the system", path, (unsigned long long)cnt, n);
+    std::vector<uint8_t> seen(n, 0);
+    for (uint32_t line = 0; line < n; ++line) {
+        uint64_t c[3];
+        double v[3];
+        for (int a = 0; a < 3; ++a)
+            if (!tk.next(t) || !parse_u64(t, &c[a])) IO_FAIL(LLG_E_INVALID, "%s: bad grid point on record %u", path, line);
+        for (int a = 0; a < 3; ++a)
+            if (!tk.next(t) || !parse_f64(t, &v[a])) IO_FAIL(LLG_E_INVALID, "%s: bad spin on record %u", path, line);
+        auto it = (c[0] < (1u << 21) && c[1] < (1u << 21) && c[2] < (1u << 21)) ? site.find(key(c[0], c[1], c[2]))
+                                                                                : site.end();
+        if (it == site.end())
+            IO_FAIL(LLG_E_INVALID, "%s: no spin at site (%llu, %llu, %llu)", path, (unsigned long long)c[0],
+                    (unsigned long long)c[1], (unsigned long long)c[2]);
+        const uint32_t i = it->second;
+        const double m = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (m < 1e-12) IO_FAIL(LLG_E_INVALID, "%s: spin of record %u has zero length", path, line);
+        sx[i] = v[0] / m;
+        sy[i] = v[1] / m;
+        sz[i] = v[2] / m;
+        seen[i] = 1;
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (!seen[i]) IO_FAIL(LLG_E_INVALID, "%s: spin %u is not in the file", path, i);
+    return 0;
+}
+
+int llg_write_grid(const char* path, uint32_t n, const uint32_t* mesh, const double* sx, const double* sy,
+                   const double* sz) {
+    if (!path || !mesh || !sx || !sy || !sz) IO_FAIL(LLG_E_INVALID, "null argument");
+    std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "w"));
+    if (!f) IO_FAIL(LLG_E_INVALID, "could not open %s for writing", path);
+    std::fprintf(f.get(), "%u\n", n);
+    for (uint32_t i = 0; i < n; ++i)
+        std::fprintf(f.get(), "%u\t%u\t%u\t%.17g\t%.17g\t%.17g\n", mesh[3 * (size_t)i], mesh[3 * (size_t)i + 1],
+                     mesh[3 * (size_t)i + 2], sx[i], sy[i], sz[i]);
+    if (std::ferror(f.get())) IO_FAIL(LLG_E_INVALID, "write error on %s", path);
+    return 0;
+}
+
+int llg_write_vtu(const char* path, uint32_t n, const double* pos, const double* sx, const double* sy,
+                  const double* sz) {
+    if (!path || !pos || !sx || !sy || !sz) IO_FAIL(LLG_E_INVALID, "null argument");
+    std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "w"));
+    if (!f) IO_FAIL(LLG_E_INVALID, "could not open %s for writing", path);
+    FILE* o = f.get();
+    std::fprintf(o, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\">\n<UnstructuredGrid>\n");
+    std::fprintf(o, "<Piece NumberOfPoints=\"%u\"  NumberOfCells=\"1\">\n<PointData Scalar=\"Spin\">\n", n);
+    std::fprintf(o, "<DataArray type=\"Float32\" Name=\"Spin\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+    for (uint32_t i = 0; i < n; ++i) std::fprintf(o, "%.9g\t%.9g\t%.9g\n", sx[i], sy[i], sz[i]);
+    std::fprintf(o, "</DataArray>\n</PointData>\n<CellData>\n</CellData>\n<Points>\n");
+    std::fprintf(o, "<DataArray type=\"Float32\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+    for (uint32_t i = 0; i < n; ++i)
+        std::fprintf(o, "%.9g\t%.9g\t%.9g\n", pos[3 * (size_t)i], pos[3 * (size_t)i + 1], pos[3 * (size_t)i + 2]);
+    std::fprintf(o, "</DataArray>\n</Points>\n<Cells>\n");
+    std::fprintf(o, "<DataArray type=\"Int32\" Name=\"connectivity\" format=\"ascii\">\n1\n</DataArray>\n");
+    std::fprintf(o, "<DataArray type=\"Int32\" Name=\"offsets\" format=\"ascii\">\n1\n</DataArray>\n");
+    std::fprintf(o, "<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n1\n</DataArray>\n");
+    std::fprintf(o, "</Cells>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
     if (std::ferror(o)) IO_FAIL(LLG_E_INVALID, "write error on %s", path);
     return 0;
 }

@@ -9,7 +9,8 @@ configuration file onto the descriptor:
   exchange.Read_exch_method = "unitcell", exchange.Exchfile, or exchange.ReadExchangeMatrix
   with exchange.ExchangeMatrixFilename                 (src/exch_routines.cpp:38-114)
   llg.dt, llg.intscheme, llg.applied, llg.update, llg.staggered_field.sapplied<k>,
-  llg.SpinConfigMethod ("random" or the unit cell's initial spins)   (src/llg.cpp:237-500)
+  llg.SpinConfigMethod ("random", "restart_text", "grid", or the unit cell's initial
+  spins)                                               (src/llg.cpp:237-500)
   anis.k2perp/k2par/k4perp/k4par and their directions (src/anis.cpp)
   Seed                                                  (src/config.cpp:64)
 
@@ -58,6 +59,11 @@ def _io():
         L.llg_read_exch_unitcell.argtypes = [P, C.c_uint32, C.POINTER(Interaction), C.c_uint32, up]
         L.llg_csr_read.argtypes = [C.c_char_p, C.c_uint32, up, up, dp, C.POINTER(C.c_uint64)]
         L.llg_csr_write.argtypes = [C.c_char_p, C.c_uint32, up, up, dp, C.c_uint64]
+        for nm in ("llg_read_spins_text", "llg_write_spins_text"):
+            getattr(L, nm).argtypes = [C.c_char_p, C.c_uint32, dp, dp, dp]
+        for nm in ("llg_read_grid", "llg_write_grid"):
+            getattr(L, nm).argtypes = [C.c_char_p, C.c_uint32, up, dp, dp, dp]
+        L.llg_write_vtu.argtypes = [C.c_char_p, C.c_uint32, dp, dp, dp, dp]
         _bound = True
     return L
 
@@ -171,6 +177,48 @@ def csr_write(path: str, xadj, adjncy, vals3):
                            _p(v, C.c_double), len(adj)), L)
 
 
+def read_spins_text(path: str, n: int):
+    """restart_text spin file -> (3, n), normalised."""
+    L = _io()
+    s = np.zeros((3, n))
+    _check(L.llg_read_spins_text(path.encode(), n, _p(s[0], C.c_double), _p(s[1], C.c_double), _p(s[2], C.c_double)), L)
+    return s
+
+
+def write_spins_text(path: str, s):
+    L = _io()
+    s = np.ascontiguousarray(s, np.float64)
+    _check(L.llg_write_spins_text(path.encode(), s.shape[1], _p(s[0], C.c_double), _p(s[1], C.c_double),
+                                  _p(s[2], C.c_double)), L)
+
+
+def read_grid(path: str, mesh):
+    """grid spin file (x y z sx sy sz per line, any order) -> (3, n) in spin order."""
+    L = _io()
+    m = np.ascontiguousarray(mesh, np.uint32).reshape(-1, 3)
+    n = m.shape[0]
+    s = np.zeros((3, n))
+    _check(L.llg_read_grid(path.encode(), n, _p(m, C.c_uint32), _p(s[0], C.c_double), _p(s[1], C.c_double),
+                           _p(s[2], C.c_double)), L)
+    return s
+
+
+def write_grid(path: str, mesh, s):
+    L = _io()
+    m = np.ascontiguousarray(mesh, np.uint32).reshape(-1, 3)
+    s = np.ascontiguousarray(s, np.float64)
+    _check(L.llg_write_grid(path.encode(), m.shape[0], _p(m, C.c_uint32), _p(s[0], C.c_double), _p(s[1], C.c_double),
+                            _p(s[2], C.c_double)), L)
+
+
+def write_vtu(path: str, pos, s):
+    L = _io()
+    p = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    s = np.ascontiguousarray(s, np.float64)
+    _check(L.llg_write_vtu(path.encode(), p.shape[0], _p(p, C.c_double), _p(s[0], C.c_double), _p(s[1], C.c_double),
+                           _p(s[2], C.c_double)), L)
+
+
 @dataclass
 class ReferenceInput:
     system: System
@@ -265,6 +313,11 @@ def load_reference_input(cff_path: str, precision: str = "fp32", scheme: Optiona
     scm = cfg.get("llg.SpinConfigMethod")
     if scm == "random":
         s0 = random_spins(n, seed, seed + 100)
+    elif scm == "restart_text":
+        s0 = read_spins_text(rel(cfg.get("llg.SpinFile")), n)
+    elif scm == "grid":
+        from . import mesh as _mesh
+        s0 = read_grid(rel(cfg.get("llg.SpinFile")), _mesh(uc, dims))
     elif scm is None or scm == "align":
         s0 = np.array([list(atoms[i % na].spin0) for i in range(n)], float).T.copy()
         s0 /= np.linalg.norm(s0, axis=0)

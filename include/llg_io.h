@@ -11,6 +11,9 @@
  *                            energy-unit conversion (src/exch_determine_exchange_matrix.cpp:27-144)
  *   llg_csr_read / _write <- csr_exch_mat.dat reading (src/exch.cpp:63-132) and writing
  *                            (src/exch_unitcell.cpp:35-70)
+ *   llg_*_spins_text,     <- restart_text / grid spin files and the ParaView spin map
+ *   llg_*_grid,              (src/spins.cpp:342-468, src/simple.cpp:114-141, src/util.cpp:265-370)
+ *   llg_write_vtu
  * Errors: negative llg_status, message in llg_last_error() (with the file's line number for
  * syntax errors); nothing exits.
  */
@@ -95,6 +98,28 @@ int llg_read_exch_unitcell(const llg_cfg* exch, uint32_t n_atoms, llg_interactio
 int llg_csr_read(const char* path, uint32_t n, uint32_t* xadj, uint32_t* adjncy, double* vals3, uint64_t* nnz);
 int llg_csr_write(const char* path, uint32_t n, const uint32_t* xadj, const uint32_t* adjncy, const double* vals3,
                   uint64_t nnz);
+
+/* ---------------------------------------------------------------- spin configuration files */
+
+/* restart_text: "n" then n lines "sx sy sz"; spins are normalised on reading and a spin
+ * shorter than 1e-12 is an error (spins::setSpinsResText, src/spins.cpp:342-395). */
+int llg_read_spins_text(const char* path, uint32_t n, double* sx, double* sy, double* sz);
+int llg_write_spins_text(const char* path, uint32_t n, const double* sx, const double* sy, const double* sz);
+
+/* grid (dw_grid.out / SpinConfigMethod "grid"): "n" then n lines "x y z sx sy sz" with the
+ * mesh coordinates lu(i,0..2) of the spin (src/simple.cpp:114-141 writes it;
+ * spins::setSpinsGrid, src/spins.cpp:397-468, reads it in any line order).  mesh is the
+ * [n][3] coordinate table of the system; a line naming no spin is an error.  The reading
+ * normalises the spin it read (the reference normalises spin i of the line counter
+ * instead, src/spins.cpp:451-453). */
+int llg_read_grid(const char* path, uint32_t n, const uint32_t* mesh, double* sx, double* sy, double* sz);
+int llg_write_grid(const char* path, uint32_t n, const uint32_t* mesh, const double* sx, const double* sy,
+                   const double* sz);
+
+/* ParaView spin map (util::outputSpinsVTU, src/util.cpp:265-370): VTK UnstructuredGrid,
+ * point data "Spin" (3 components) at the spin positions pos[n][3], one placeholder cell. */
+int llg_write_vtu(const char* path, uint32_t n, const double* pos, const double* sx, const double* sy,
+                  const double* sz);
 
 #ifdef __cplusplus
 }

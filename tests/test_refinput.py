@@ -246,3 +246,49 @@ def test_reference_example_inputs_parse():
     ints = R.read_exch_unitcell(R.Config(os.path.join(REF, "Mn2Au_ua.exch")), 4)
     assert len(ints) > 0 and all(i.atom_i < 4 and i.atom_j < 4 for i in ints)
     assert ints[0].J[0] == 0.6484 * 2.179872172e-18 * 1.0e-3
+
+
+def test_spin_files_round_trip_and_reference_layout(tmp_path):
+    n = 8 * 4
+    mesh = sllg.mesh(sllg.unit_cell("bcc"), (2, 2, 4))
+    s = np.random.default_rng(2).normal(size=(3, n))
+    s /= np.linalg.norm(s, axis=0)
+    p = tmp_path / "restart.txt"
+    R.write_spins_text(str(p), s)
+    assert np.array_equal(R.read_spins_text(str(p), n), s / np.linalg.norm(s, axis=0))
+    # the reference writes 6 significant digits and normalises on reading
+    q = tmp_path / "r6.txt"
+    q.write_text("2\n0 0 2\n3\t4\t0\n")
+    r = R.read_spins_text(str(q), 2)
+    assert np.allclose(r[:, 0], [0, 0, 1]) and np.allclose(r[:, 1], [0.6, 0.8, 0])
+    q.write_text("2\n0 0 0\n1 0 0\n")
+    with pytest.raises(sllg.LLGError):
+        R.read_spins_text(str(q), 2)
+    with pytest.raises(sllg.LLGError):
+        R.read_spins_text(str(p), n + 1)
+    # grid: written in spin order, read back in any line order
+    g = tmp_path / "dw_grid.out"
+    R.write_grid(str(g), mesh, s)
+    lines = g.read_text().splitlines()
+    body = lines[1:]
+    np.random.default_rng(4).shuffle(body)
+    g.write_text("\n".join([lines[0]] + body) + "\n")
+    assert np.allclose(R.read_grid(str(g), mesh), s, rtol=0, atol=1e-15)
+    g.write_text(f"{n}\n" + "\n".join(body[:-1] + ["99 99 99 1 0 0"]) + "\n")
+    with pytest.raises(sllg.LLGError):
+        R.read_grid(str(g), mesh)
+
+
+def test_vtu_is_well_formed(tmp_path):
+    import xml.etree.ElementTree as ET
+    n = 10
+    pos = np.arange(3 * n, dtype=float).reshape(n, 3)
+    s = np.tile(np.array([[0.0], [0.6], [0.8]]), (1, n))
+    p = tmp_path / "spinmap_0.vtu"
+    R.write_vtu(str(p), pos, s)
+    root = ET.parse(str(p)).getroot()
+    piece = root.find("UnstructuredGrid/Piece")
+    assert piece.get("NumberOfPoints") == str(n)
+    spin = np.array(piece.find("PointData/DataArray").text.split(), float).reshape(n, 3)
+    pts = np.array(piece.find("Points/DataArray").text.split(), float).reshape(n, 3)
+    assert np.allclose(spin, s.T) and np.array_equal(pts, pos)
