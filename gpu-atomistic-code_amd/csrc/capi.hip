@@ -790,7 +790,10 @@ int create_with_store(const llg_desc* d, const std::shared_ptr<ExchStore>& store
 
 namespace {
 std::mutex g_tune_mu;
-std::map<std::string, std::pair<int, int>> g_tune_cache;
+struct TuneChoice {
+    int tpb, xcd, unr;
+};
+std::map<std::string, TuneChoice> g_tune_cache;
 
 std::string tune_key(const llg_ctx* c) {
     hipDeviceProp_t p{};
@@ -807,25 +810,27 @@ void tune_file_load() {
     FILE* fp = std::fopen(f, "r");
     if (!fp) return;
     char key[256];
-    int tpb, xcd;
-    while (std::fscanf(fp, "%255s %d %d", key, &tpb, &xcd) == 3) g_tune_cache[key] = {tpb, xcd};
+    TuneChoice t{};
+    while (std::fscanf(fp, "%255s %d %d %d", key, &t.tpb, &t.xcd, &t.unr) == 4) g_tune_cache[key] = t;
     std::fclose(fp);
 }
 
-void tune_file_append(const std::string& key, int tpb, int xcd) {
+void tune_file_append(const std::string& key, const TuneChoice& t) {
     const char* f = std::getenv("SLLG_TUNE_FILE");
     if (!f) return;
     FILE* fp = std::fopen(f, "a");
     if (!fp) return;
-    std::fprintf(fp, "%s %d %d\n", key.c_str(), tpb, xcd);
+    std::fprintf(fp, "%s %d %d %d\n", key.c_str(), t.tpb, t.xcd, t.unr);
     std::fclose(fp);
 }
 }  // namespace
 
-extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd) {
+extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd, int unroll) {
     if (!c || (tpb != 256 && tpb != 512 && tpb != 1024)) SLLG_FAIL(LLG_E_INVALID, "block size must be 256, 512 or 1024");
+    if (unroll != 2 && unroll != 4) SLLG_FAIL(LLG_E_INVALID, "unroll must be 2 or 4");
     c->tune_tpb = tpb;
     c->tune_xcd = xcd ? 1 : 0;
+    c->tune_unr = unroll;
     return 0;
 }
 
@@ -834,6 +839,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     std::memset(out, 0, sizeof(*out));
     out->tpb = c->tune_tpb;
     out->xcd = c->tune_xcd;
+    out->unroll = c->tune_unr;
     if (c->prec != LLG_FP32 || c->scheme != LLG_HEUN) return 0;
     SLLG_HIP(hipSetDevice(c->device));
     const std::string key = tune_key(c);
@@ -846,8 +852,9 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
         }
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
-            c->tune_tpb = out->tpb = it->second.first;
-            c->tune_xcd = out->xcd = it->second.second;
+            c->tune_tpb = out->tpb = it->second.tpb;
+            c->tune_xcd = out->xcd = it->second.xcd;
+            c->tune_unr = out->unroll = it->second.unr;
             out->from_cache = 1;
             return 0;
         }
@@ -857,35 +864,38 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     SLLG_TRY(save.alloc(c->S.bytes));
     SLLG_HIP(hipMemcpyAsync(save.p, c->S.p, c->S.bytes, hipMemcpyDeviceToDevice, c->stream));
     const int tpbs[3] = {256, 512, 1024};
+    const int nunr = c->ex32.fmt == FMT_SELL_PACK ? 2 : 1;
+    const int unrs[2] = {2, 4};
     double best = 1e300;
-    int btpb = 256, bxcd = 1;
+    TuneChoice b{256, 1, 2};
     hipEvent_t e0, e1;
     SLLG_HIP(hipEventCreate(&e0));
     SLLG_HIP(hipEventCreate(&e1));
     const int noise = c->noise_mode;
     c->noise_mode = LLG_NOISE_COUNTER;  // no host queue traffic while timing
     int rc = 0;
-    for (int ti = 0; ti < 3 && !rc; ++ti)
-        for (int xcd = 1; xcd >= 0 && !rc; --xcd) {
-            c->tune_tpb = tpbs[ti];
-            c->tune_xcd = xcd;
-            rc = do_steps(c, 0, 2);
-            if (rc) break;
-            (void)hipEventRecord(e0, c->stream);
-            rc = do_steps(c, 2, nsteps);
-            (void)hipEventRecord(e1, c->stream);
-            (void)hipEventSynchronize(e1);
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            const double per = ms / nsteps;
-            if (tpbs[ti] == 256 && xcd == 1) out->ms_per_step_default = per;
-            out->candidates++;
-            if (per < best) {
-                best = per;
-                btpb = tpbs[ti];
-                bxcd = xcd;
+    for (int ui = 0; ui < nunr && !rc; ++ui)
+        for (int ti = 0; ti < 3 && !rc; ++ti)
+            for (int xcd = 1; xcd >= 0 && !rc; --xcd) {
+                c->tune_tpb = tpbs[ti];
+                c->tune_xcd = xcd;
+                c->tune_unr = unrs[ui];
+                rc = do_steps(c, 0, 2);
+                if (rc) break;
+                (void)hipEventRecord(e0, c->stream);
+                rc = do_steps(c, 2, nsteps);
+                (void)hipEventRecord(e1, c->stream);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                const double per = ms / nsteps;
+                if (tpbs[ti] == 256 && xcd == 1 && unrs[ui] == 2) out->ms_per_step_default = per;
+                out->candidates++;
+                if (per < best) {
+                    best = per;
+                    b = TuneChoice{tpbs[ti], xcd, unrs[ui]};
+                }
             }
-        }
     c->noise_mode = noise;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -893,11 +903,12 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     SLLG_HIP(hipStreamSynchronize(c->stream));
     c->det_valid = false;
     if (rc) return rc;
-    c->tune_tpb = out->tpb = btpb;
-    c->tune_xcd = out->xcd = bxcd;
+    c->tune_tpb = out->tpb = b.tpb;
+    c->tune_xcd = out->xcd = b.xcd;
+    c->tune_unr = out->unroll = b.unr;
     out->ms_per_step = best;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[key] = {btpb, bxcd};
-    tune_file_append(key, btpb, bxcd);
+    g_tune_cache[key] = b;
+    tune_file_append(key, b);
     return 0;
 }

@@ -175,6 +175,7 @@ struct llg_ctx {
     bool permuted() const { return perm.na > 1; }
     int tune_tpb = 256;  // block size of the FP32 Heun kernels (llg_autotune)
     int tune_xcd = 1;    // XCD-aware tile remap on/off (llg_autotune)
+    int tune_unr = 2;    // packed-exchange chunks loaded ahead per wave (llg_autotune)
 
     // device state
     sllg::DevBuf S, Y, A, H, FN, DET, XI, STG, STG0, DEM;  // SoA 3n (type per path)

@@ -88,7 +88,9 @@ __device__ __forceinline__ void pack_entry(uint32_t w, uint32_t i, uint32_t dbit
 // Fast packed reader of the fused kernels: one uint4 (4 entries) per lane per load
 // (1 KB per wavefront), then at most one uint2 tail; couplings from the block's LDS copy
 // of the dictionary.
-template <int NC>
+// UNR chunks (4 UNR entries, 12 UNR gathers) are loaded before any is used, so that many
+// neighbour loads are in flight per wave; the launch-shape tuner picks UNR.
+template <int NC, int UNR>
 __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_t i, const float* __restrict__ S,
                                                    const float* sdict, float dot[3]) {
     dot[0] = dot[1] = dot[2] = 0.0f;
@@ -98,8 +100,20 @@ __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_
     const uint4* __restrict__ p = reinterpret_cast<const uint4*>(e.ent + (size_t)q0 * kSlice) + lane;
     const uint32_t db = e.dbits;
     const SpinBuf sb = spin_buf(S, e.n);
-#pragma unroll 2
-    for (uint32_t c = 0; c < nch; ++c) {
+    uint32_t c = 0;
+    for (; c + UNR <= nch; c += UNR) {
+        uint4 cur[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) cur[u] = p[(size_t)(c + u) * kSlice];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            pack_entry<NC>(cur[u].x, i, db, sb, sdict, dot);
+            pack_entry<NC>(cur[u].y, i, db, sb, sdict, dot);
+            pack_entry<NC>(cur[u].z, i, db, sb, sdict, dot);
+            pack_entry<NC>(cur[u].w, i, db, sb, sdict, dot);
+        }
+    }
+    for (; c < nch; ++c) {
         const uint4 cur = p[(size_t)c * kSlice];
         pack_entry<NC>(cur.x, i, db, sb, sdict, dot);
         pack_entry<NC>(cur.y, i, db, sb, sdict, dot);
@@ -169,7 +183,7 @@ __device__ __forceinline__ uint32_t row_tile(const FusedArgs& a) {
     return L;
 }
 
-template <int FMT, int NC>
+template <int FMT, int NC, int UNR = 2>
 __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const float* __restrict__ src,
                                           const float* sdict, float s[3], float f[3]) {
     const uint32_t n = a.n;
@@ -178,7 +192,7 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const 
     s[2] = src[2 * (size_t)n + i];
     float h[3];
     if constexpr (FMT == FMT_SELL_PACK)
-        exch_row_pack_fast<NC>(a.ex, i, src, sdict, h);
+        exch_row_pack_fast<NC, UNR>(a.ex, i, src, sdict, h);
     else
         exch_any<FMT, NC, float, float, float>(a.ex, i, src, h);
     if (a.ex.fs_xadj) {
@@ -228,7 +242,7 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const 
         __syncthreads();                                                          \
     }
 
-template <int FMT, int NC, int TPB>
+template <int FMT, int NC, int TPB, int UNR>
 __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S, float* __restrict__ Y,
                                                     float* __restrict__ A) {
     SLLG_STAGE_DICT(FMT, NC, a)
@@ -236,7 +250,7 @@ __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __r
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, S, sdict, s, f);
+    fused_rhs<FMT, NC, UNR>(a, i, S, sdict, s, f);
     float y[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -248,7 +262,7 @@ __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __r
     }
 }
 
-template <int FMT, int NC, int TPB>
+template <int FMT, int NC, int TPB, int UNR>
 __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __restrict__ Y,
                                                     const float* __restrict__ A, float* __restrict__ S) {
     SLLG_STAGE_DICT(FMT, NC, a)
@@ -256,7 +270,7 @@ __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __r
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, Y, sdict, s, f);
+    fused_rhs<FMT, NC, UNR>(a, i, Y, sdict, s, f);
     float p[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = A[(size_t)c * n + i] + a.half_rdt * f[c];
@@ -488,17 +502,41 @@ static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
     } while (0)
 
 // The Heun pair runs with the autotuned block size (256, 512 or 1024 threads).
+template <int F, int N, int TPB>
+static void launch_heun_pred(const FusedArgs& a, const float* S, float* Y, float* A, uint32_t n, int unr,
+                             hipStream_t st) {
+    if constexpr (F == FMT_SELL_PACK) {
+        if (unr == 4) {
+            k_heun_pred<F, N, TPB, 4><<<blocks_for(n, TPB), TPB, 0, st>>>(a, S, Y, A);
+            return;
+        }
+    }
+    k_heun_pred<F, N, TPB, 2><<<blocks_for(n, TPB), TPB, 0, st>>>(a, S, Y, A);
+}
+
+template <int F, int N, int TPB>
+static void launch_heun_corr(const FusedArgs& a, const float* Y, const float* A, float* S, uint32_t n, int unr,
+                             hipStream_t st) {
+    if constexpr (F == FMT_SELL_PACK) {
+        if (unr == 4) {
+            k_heun_corr<F, N, TPB, 4><<<blocks_for(n, TPB), TPB, 0, st>>>(a, Y, A, S);
+            return;
+        }
+    }
+    k_heun_corr<F, N, TPB, 2><<<blocks_for(n, TPB), TPB, 0, st>>>(a, Y, A, S);
+}
+
 int fused_heun_pred(llg_ctx* c, const StepSched& s) {
     FusedArgs a = fused_args(c, s);
     float* S = c->S.as<float>();
     float* Y = c->Y.as<float>();
     float* A = c->A.as<float>();
-    const int tpb = c->tune_tpb;
-#define SLLG_L(F, N)                                                                                   \
-    do {                                                                                               \
-        if (tpb == 1024) k_heun_pred<F, N, 1024><<<blocks_for(c->n, 1024), 1024, 0, c->stream>>>(a, S, Y, A); \
-        else if (tpb == 512) k_heun_pred<F, N, 512><<<blocks_for(c->n, 512), 512, 0, c->stream>>>(a, S, Y, A); \
-        else k_heun_pred<F, N, 256><<<blocks_for(c->n, 256), 256, 0, c->stream>>>(a, S, Y, A);        \
+    const int tpb = c->tune_tpb, unr = c->tune_unr;
+#define SLLG_L(F, N)                                                                  \
+    do {                                                                              \
+        if (tpb == 1024) launch_heun_pred<F, N, 1024>(a, S, Y, A, c->n, unr, c->stream); \
+        else if (tpb == 512) launch_heun_pred<F, N, 512>(a, S, Y, A, c->n, unr, c->stream); \
+        else launch_heun_pred<F, N, 256>(a, S, Y, A, c->n, unr, c->stream);           \
     } while (0)
     SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
@@ -511,12 +549,12 @@ int fused_heun_corr(llg_ctx* c, const StepSched& s) {
     float* S = c->S.as<float>();
     float* Y = c->Y.as<float>();
     float* A = c->A.as<float>();
-    const int tpb = c->tune_tpb;
-#define SLLG_L(F, N)                                                                                   \
-    do {                                                                                               \
-        if (tpb == 1024) k_heun_corr<F, N, 1024><<<blocks_for(c->n, 1024), 1024, 0, c->stream>>>(a, Y, A, S); \
-        else if (tpb == 512) k_heun_corr<F, N, 512><<<blocks_for(c->n, 512), 512, 0, c->stream>>>(a, Y, A, S); \
-        else k_heun_corr<F, N, 256><<<blocks_for(c->n, 256), 256, 0, c->stream>>>(a, Y, A, S);        \
+    const int tpb = c->tune_tpb, unr = c->tune_unr;
+#define SLLG_L(F, N)                                                                  \
+    do {                                                                              \
+        if (tpb == 1024) launch_heun_corr<F, N, 1024>(a, Y, A, S, c->n, unr, c->stream); \
+        else if (tpb == 512) launch_heun_corr<F, N, 512>(a, Y, A, S, c->n, unr, c->stream); \
+        else launch_heun_corr<F, N, 256>(a, Y, A, S, c->n, unr, c->stream);           \
     } while (0)
     SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
 #undef SLLG_L

@@ -273,9 +273,9 @@ def test_launch_shape_does_not_change_results(gpu):
     gives identical bits.  The autotuner keeps the state it was given."""
     sysd, s0 = _fcc_multi(L=4)
     res = []
-    for tpb, xcd in ((256, 1), (512, 0), (1024, 1), (256, 0)):
+    for tpb, xcd, unr in ((256, 1, 2), (512, 0, 4), (1024, 1, 4), (256, 0, 2), (256, 1, 4)):
         g = sllg.Context(sysd)
-        g.set_launch(tpb, xcd)
+        g.set_launch(tpb, xcd, unr)
         g.set_spins(s0)
         g.set_temperature(150.0)
         g.step(0, 40)
@@ -288,12 +288,15 @@ def test_launch_shape_does_not_change_results(gpu):
     g.step(0, 10)
     before = g.get_spins()
     t = g.autotune(5)
-    assert t["tpb"] in (256, 512, 1024) and t["xcd"] in (0, 1)
+    assert t["tpb"] in (256, 512, 1024) and t["xcd"] in (0, 1) and t["unroll"] in (2, 4)
+    assert t["candidates"] == 12
     assert np.array_equal(g.get_spins(), before)
     g.step(10, 30)
     assert np.array_equal(g.get_spins(), res[0])
     with pytest.raises(sllg.LLGError):
         g.set_launch(384, 1)
+    with pytest.raises(sllg.LLGError):
+        g.set_launch(256, 1, 3)
 
 
 def _slices(enc):
