@@ -791,7 +791,7 @@ int create_with_store(const llg_desc* d, const std::shared_ptr<ExchStore>& store
 namespace {
 std::mutex g_tune_mu;
 struct TuneChoice {
-    int tpb, xcd, unr;
+    int tpb, xcd, unr, thc;
 };
 std::map<std::string, TuneChoice> g_tune_cache;
 
@@ -811,7 +811,7 @@ void tune_file_load() {
     if (!fp) return;
     char key[256];
     TuneChoice t{};
-    while (std::fscanf(fp, "%255s %d %d %d", key, &t.tpb, &t.xcd, &t.unr) == 4) g_tune_cache[key] = t;
+    while (std::fscanf(fp, "%255s %d %d %d %d", key, &t.tpb, &t.xcd, &t.unr, &t.thc) == 5) g_tune_cache[key] = t;
     std::fclose(fp);
 }
 
@@ -820,17 +820,18 @@ void tune_file_append(const std::string& key, const TuneChoice& t) {
     if (!f) return;
     FILE* fp = std::fopen(f, "a");
     if (!fp) return;
-    std::fprintf(fp, "%s %d %d %d\n", key.c_str(), t.tpb, t.xcd, t.unr);
+    std::fprintf(fp, "%s %d %d %d %d\n", key.c_str(), t.tpb, t.xcd, t.unr, t.thc);
     std::fclose(fp);
 }
 }  // namespace
 
-extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd, int unroll) {
+extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd, int unroll, int thermal_cache) {
     if (!c || (tpb != 256 && tpb != 512 && tpb != 1024)) SLLG_FAIL(LLG_E_INVALID, "block size must be 256, 512 or 1024");
     if (unroll != 2 && unroll != 4) SLLG_FAIL(LLG_E_INVALID, "unroll must be 2 or 4");
     c->tune_tpb = tpb;
     c->tune_xcd = xcd ? 1 : 0;
     c->tune_unr = unroll;
+    c->tune_thc = thermal_cache ? 1 : 0;
     return 0;
 }
 
@@ -840,6 +841,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     out->tpb = c->tune_tpb;
     out->xcd = c->tune_xcd;
     out->unroll = c->tune_unr;
+    out->thermal_cache = c->tune_thc;
     if (c->prec != LLG_FP32 || c->scheme != LLG_HEUN) return 0;
     SLLG_HIP(hipSetDevice(c->device));
     const std::string key = tune_key(c);
@@ -855,6 +857,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
             c->tune_tpb = out->tpb = it->second.tpb;
             c->tune_xcd = out->xcd = it->second.xcd;
             c->tune_unr = out->unroll = it->second.unr;
+            c->tune_thc = out->thermal_cache = it->second.thc;
             out->from_cache = 1;
             return 0;
         }
@@ -866,20 +869,24 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     const int tpbs[3] = {256, 512, 1024};
     const int nunr = c->ex32.fmt == FMT_SELL_PACK ? 2 : 1;
     const int unrs[2] = {2, 4};
+    // handing the thermal field over only matters at T > 0 with counter noise
+    const int nthc = (c->T > 0.0 && c->noise_mode == LLG_NOISE_COUNTER) ? 2 : 1;
     double best = 1e300;
-    TuneChoice b{256, 1, 2};
+    TuneChoice b{256, 1, 2, c->tune_thc};
     hipEvent_t e0, e1;
     SLLG_HIP(hipEventCreate(&e0));
     SLLG_HIP(hipEventCreate(&e1));
     const int noise = c->noise_mode;
     c->noise_mode = LLG_NOISE_COUNTER;  // no host queue traffic while timing
     int rc = 0;
+    for (int hi = 0; hi < nthc && !rc; ++hi)
     for (int ui = 0; ui < nunr && !rc; ++ui)
         for (int ti = 0; ti < 3 && !rc; ++ti)
             for (int xcd = 1; xcd >= 0 && !rc; --xcd) {
                 c->tune_tpb = tpbs[ti];
                 c->tune_xcd = xcd;
                 c->tune_unr = unrs[ui];
+                if (nthc == 2) c->tune_thc = 1 - hi;
                 rc = do_steps(c, 0, 2);
                 if (rc) break;
                 (void)hipEventRecord(e0, c->stream);
@@ -889,11 +896,11 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 const double per = ms / nsteps;
-                if (tpbs[ti] == 256 && xcd == 1 && unrs[ui] == 2) out->ms_per_step_default = per;
+                if (tpbs[ti] == 256 && xcd == 1 && unrs[ui] == 2 && hi == 0) out->ms_per_step_default = per;
                 out->candidates++;
                 if (per < best) {
                     best = per;
-                    b = TuneChoice{tpbs[ti], xcd, unrs[ui]};
+                    b = TuneChoice{tpbs[ti], xcd, unrs[ui], c->tune_thc};
                 }
             }
     c->noise_mode = noise;
@@ -906,6 +913,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     c->tune_tpb = out->tpb = b.tpb;
     c->tune_xcd = out->xcd = b.xcd;
     c->tune_unr = out->unroll = b.unr;
+    c->tune_thc = out->thermal_cache = b.thc;
     out->ms_per_step = best;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = b;

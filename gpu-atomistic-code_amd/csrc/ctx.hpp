@@ -176,9 +176,11 @@ struct llg_ctx {
     int tune_tpb = 256;  // block size of the FP32 Heun kernels (llg_autotune)
     int tune_xcd = 1;    // XCD-aware tile remap on/off (llg_autotune)
     int tune_unr = 2;    // packed-exchange chunks loaded ahead per wave (llg_autotune)
+    int tune_thc = 1;    // predictor hands its thermal field to the corrector (llg_autotune)
 
     // device state
     sllg::DevBuf S, Y, A, H, FN, DET, XI, STG, STG0, DEM;  // SoA 3n (type per path)
+    sllg::DevBuf HT;  // FP32 Heun: thermal field of the current step, predictor -> corrector
     sllg::DevBuf K[6];                                     // RK stage increments
     sllg::DevBuf spec;      // u8 species per spin (FP32 path, multi-species)
     sllg::DevBuf sub;       // u8 sublattice per spin

@@ -145,6 +145,9 @@ struct FusedArgs {
     int xcd;    // XCD-aware block remap
     uint32_t il_na;  // > 1: sublattice-interleaved tile order (il_na sublattices of il_cells rows)
     uint32_t il_cells;
+    int tile_sub;    // every tile lies in one sublattice block (perm.ncell % blockDim.x == 0)
+    float* th_out;   // predictor: store the thermal field it drew (or null)
+    const float* th_in;  // corrector: read it instead of drawing again (or null)
     float rdt, half_rdt, sqrtT;
     float B[3];
     uint64_t seed, stream, t;
@@ -183,8 +186,25 @@ __device__ __forceinline__ uint32_t row_tile(const FusedArgs& a) {
     return L;
 }
 
+// Device row of this thread and the reference index that keys its noise.  When a tile
+// lies inside one sublattice block, the block's sublattice and first cell are uniform
+// (scalar arithmetic once per block) and the lane adds its offset; otherwise the lane
+// inverts the permutation itself.
+__device__ __forceinline__ void row_of_thread(const FusedArgs& a, uint32_t tpb, uint32_t& i, uint32_t& iref) {
+    const uint32_t d0 = row_tile(a) * tpb;
+    i = d0 + threadIdx.x;
+    if (a.perm.na <= 1) {
+        iref = i;
+    } else if (a.tile_sub) {
+        const uint32_t t = d0 / a.perm.ncell;
+        iref = (d0 - t * a.perm.ncell + threadIdx.x) * a.perm.na + t;
+    } else {
+        iref = a.perm.to_ref(i);
+    }
+}
+
 template <int FMT, int NC, int UNR = 2>
-__device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const float* __restrict__ src,
+__device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32_t iref, const float* __restrict__ src,
                                           const float* sdict, float s[3], float f[3]) {
     const uint32_t n = a.n;
     s[0] = src[i];
@@ -214,12 +234,27 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, const 
     }
     const Species<float> sp = a.multi ? a.sp[a.spec[i]] : a.sp0;
     if (a.noise == 1) {
-        float z[3];
-        normals3_f32(a.seed, a.stream, a.perm.to_ref(i), a.t, z);
-        const float TP = a.sqrtT * sp.sigma;
-        h[0] += z[0] * TP;
-        h[1] += z[1] * TP;
-        h[2] += z[2] * TP;
+        float th[3];
+        if (a.th_in) {
+            th[0] = a.th_in[i];
+            th[1] = a.th_in[n + i];
+            th[2] = a.th_in[2 * (size_t)n + i];
+        } else {
+            float z[3];
+            normals3_f32(a.seed, a.stream, iref, a.t, z);
+            const float TP = a.sqrtT * sp.sigma;
+            th[0] = z[0] * TP;
+            th[1] = z[1] * TP;
+            th[2] = z[2] * TP;
+            if (a.th_out) {
+                a.th_out[i] = th[0];
+                a.th_out[n + i] = th[1];
+                a.th_out[2 * (size_t)n + i] = th[2];
+            }
+        }
+        h[0] += th[0];
+        h[1] += th[1];
+        h[2] += th[2];
     } else if (a.noise == 2) {
         const float TP = a.sqrtT * sp.sigma;
         h[0] += (float)a.xi[i] * TP;
@@ -246,11 +281,12 @@ template <int FMT, int NC, int TPB, int UNR>
 __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S, float* __restrict__ Y,
                                                     float* __restrict__ A) {
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = row_tile(a) * TPB + threadIdx.x;
+    uint32_t i, iref;
+    row_of_thread(a, TPB, i, iref);
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC, UNR>(a, i, S, sdict, s, f);
+    fused_rhs<FMT, NC, UNR>(a, i, iref, S, sdict, s, f);
     float y[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -266,11 +302,12 @@ template <int FMT, int NC, int TPB, int UNR>
 __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __restrict__ Y,
                                                     const float* __restrict__ A, float* __restrict__ S) {
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = row_tile(a) * TPB + threadIdx.x;
+    uint32_t i, iref;
+    row_of_thread(a, TPB, i, iref);
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC, UNR>(a, i, Y, sdict, s, f);
+    fused_rhs<FMT, NC, UNR>(a, i, iref, Y, sdict, s, f);
     float p[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = A[(size_t)c * n + i] + a.half_rdt * f[c];
@@ -307,7 +344,7 @@ __global__ __launch_bounds__(kTPB) void k_heun_batch(const BatchMember* __restri
     const uint32_t n = a.n;
     float s[3], f[3];
     if constexpr (PHASE == 0) {
-        fused_rhs<FMT, NC>(a, i, bm.S, sdict, s, f);
+        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.S, sdict, s, f);
         float y[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -318,7 +355,7 @@ __global__ __launch_bounds__(kTPB) void k_heun_batch(const BatchMember* __restri
             bm.A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
         }
     } else {
-        fused_rhs<FMT, NC>(a, i, bm.Y, sdict, s, f);
+        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.Y, sdict, s, f);
         float p[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[c] = bm.A[(size_t)c * n + i] + a.half_rdt * f[c];
@@ -339,7 +376,7 @@ __global__ __launch_bounds__(kTPB) void k_rk4(FusedArgs a, const float* __restri
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, M == 0 ? S : Yin, sdict, s, f);
+    fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), M == 0 ? S : Yin, sdict, s, f);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const size_t o = (size_t)c * n + i;
@@ -381,7 +418,7 @@ __global__ __launch_bounds__(kTPB) void k_rk5(FusedArgs a, const float* __restri
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, M == 0 ? S : Yin, sdict, s, f);
+    fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), M == 0 ? S : Yin, sdict, s, f);
     float* Ks[5] = {K0, K1, K2, K3, K4};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -458,6 +495,7 @@ static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
         a.il_na = c->perm.na;
         a.il_cells = c->perm.ncell;
     }
+    a.tile_sub = (c->perm.na > 1 && c->perm.ncell % 1024u == 0) ? 1 : 0;
     a.rdt = (float)c->rdt;
     a.half_rdt = (float)(0.5 * c->rdt);
     a.sqrtT = (float)std::sqrt(c->T);
@@ -526,11 +564,19 @@ static void launch_heun_corr(const FusedArgs& a, const float* Y, const float* A,
     k_heun_corr<F, N, TPB, 2><<<blocks_for(n, TPB), TPB, 0, st>>>(a, Y, A, S);
 }
 
+// The predictor stores the thermal field it draws for the corrector of the same step
+// (12 B written + 12 B read per spin instead of a second Philox + Box-Muller draw).
+static bool cache_thermal(const llg_ctx* c) { return c->tune_thc && c->T > 0.0 && c->noise_mode == LLG_NOISE_COUNTER; }
+
 int fused_heun_pred(llg_ctx* c, const StepSched& s) {
     FusedArgs a = fused_args(c, s);
     float* S = c->S.as<float>();
     float* Y = c->Y.as<float>();
     float* A = c->A.as<float>();
+    if (cache_thermal(c)) {
+        if (c->HT.bytes < 3 * (size_t)c->n * sizeof(float)) SLLG_TRY(c->HT.alloc(3 * (size_t)c->n * sizeof(float)));
+        a.th_out = c->HT.as<float>();
+    }
     const int tpb = c->tune_tpb, unr = c->tune_unr;
 #define SLLG_L(F, N)                                                                  \
     do {                                                                              \
@@ -549,6 +595,7 @@ int fused_heun_corr(llg_ctx* c, const StepSched& s) {
     float* S = c->S.as<float>();
     float* Y = c->Y.as<float>();
     float* A = c->A.as<float>();
+    if (cache_thermal(c) && c->HT.p) a.th_in = c->HT.as<float>();
     const int tpb = c->tune_tpb, unr = c->tune_unr;
 #define SLLG_L(F, N)                                                                  \
     do {                                                                              \

@@ -297,8 +297,10 @@ int llg_profile_kernels(llg_ctx* ctx, uint64_t t0, uint32_t nsteps, int max_kind
 
 /* Launch-shape autotuner (no reference counterpart): times the FP32 Heun kernel pair for
  * every candidate block size (256, 512, 1024 threads) with and without the XCD-aware
- * tile remap, and for the packed encoding 2 or 4 coupling chunks loaded ahead per wave,
- * on this handle's own couplings, for nsteps steps each; keeps the fastest.
+ * tile remap, for the packed encoding 2 or 4 coupling chunks loaded ahead per wave, and
+ * with the thermal field drawn in both passes or handed from predictor to corrector
+ * (bit-identical either way), on this handle's own couplings, for nsteps steps each;
+ * keeps the fastest.
  * The spin state is restored afterwards.  Results are cached per (device, encoding, n,
  * entries) in the process and, if the environment variable SLLG_TUNE_FILE names a file,
  * across processes.  Other paths return LLG_OK with the defaults. */
@@ -307,10 +309,11 @@ typedef struct llg_tune_result {
     double ms_per_step;     /* chosen */
     double ms_per_step_default; /* 256 threads, remap on */
     int candidates, from_cache;
-    int unroll, pad_;
+    int unroll;
+    int thermal_cache;  /* 1: the predictor hands its thermal field to the corrector */
 } llg_tune_result;
 int llg_autotune(llg_ctx* ctx, uint32_t nsteps, llg_tune_result* out);
-int llg_set_launch(llg_ctx* ctx, int tpb, int xcd, int unroll);
+int llg_set_launch(llg_ctx* ctx, int tpb, int xcd, int unroll, int thermal_cache);
 
 /* Algorithmic HBM bytes per spin-update of the configured path, split into the
  * kernels of one step, and the device bytes the handle holds. */

@@ -273,9 +273,9 @@ def test_launch_shape_does_not_change_results(gpu):
     gives identical bits.  The autotuner keeps the state it was given."""
     sysd, s0 = _fcc_multi(L=4)
     res = []
-    for tpb, xcd, unr in ((256, 1, 2), (512, 0, 4), (1024, 1, 4), (256, 0, 2), (256, 1, 4)):
+    for tpb, xcd, unr, thc in ((256, 1, 2, 1), (512, 0, 4, 0), (1024, 1, 4, 1), (256, 0, 2, 0), (256, 1, 4, 0)):
         g = sllg.Context(sysd)
-        g.set_launch(tpb, xcd, unr)
+        g.set_launch(tpb, xcd, unr, thc)
         g.set_spins(s0)
         g.set_temperature(150.0)
         g.step(0, 40)
@@ -289,7 +289,7 @@ def test_launch_shape_does_not_change_results(gpu):
     before = g.get_spins()
     t = g.autotune(5)
     assert t["tpb"] in (256, 512, 1024) and t["xcd"] in (0, 1) and t["unroll"] in (2, 4)
-    assert t["candidates"] == 12
+    assert t["candidates"] == 24 and t["thermal_cache"] in (0, 1)
     assert np.array_equal(g.get_spins(), before)
     g.step(10, 30)
     assert np.array_equal(g.get_spins(), res[0])
@@ -297,6 +297,20 @@ def test_launch_shape_does_not_change_results(gpu):
         g.set_launch(384, 1)
     with pytest.raises(sllg.LLGError):
         g.set_launch(256, 1, 3)
+
+
+def test_fp32_counter_noise_statistics(gpu):
+    """The FP32 path's Gaussian draws (hardware log/sqrt/sin/cos in Box-Muller) over 786k
+    samples: mean, variance, kurtosis within sampling error and a Kolmogorov-Smirnov test."""
+    from scipy import stats
+    sysd, _, _ = W.sc_ferro(L=64, precision="fp32", seed=99)
+    g = sllg.Context(sysd)
+    z = g.noise_sample(5).reshape(-1)
+    N = z.size
+    assert abs(z.mean()) < 5 / np.sqrt(N)
+    assert abs(z.var() - 1.0) < 5 * np.sqrt(2.0 / N)
+    assert abs(stats.kurtosis(z, fisher=False) - 3.0) < 5 * np.sqrt(24.0 / N)
+    assert stats.kstest(z, "norm").pvalue > 1e-3
 
 
 def _slices(enc):
