@@ -111,7 +111,7 @@ def headline(args):
     # coupling stream then comes from HBM on every launch.
     unshared = None
     if not args.no_unshared:
-        sysd.flags = sllg.FLAG_NO_SLICE_SHARING
+        sysd.flags = sllg.FLAG_NO_SLICE_SHARING | sllg.FLAG_NO_LINE_MODE
         u = sllg.Context(sysd)
         u.set_spins(s0)
         u.set_temperature(T)

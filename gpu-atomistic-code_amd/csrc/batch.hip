@@ -114,6 +114,9 @@ int upload_tables(llg_batch* b) {
 }
 
 int launch_chunk(llg_batch* b, uint32_t nsteps, bool dipolar) {
+    // the batch kernels write S without the members' AoS3 gather copies
+    for (auto& m : b->m)
+        if (m.c) m.c->g3_valid = false;
     for (uint32_t k = 0; k < nsteps; ++k) {
         if (dipolar)
             for (auto& mm : b->m)

@@ -80,6 +80,7 @@ size_t prec_bytes(const llg_ctx* c) { return c->prec == LLG_FP64_PLAIN ? sizeof(
 int upload_state(llg_ctx* c, DevBuf& dst, const double* x, const double* y, const double* z) {
     const uint32_t n = c->n;
     const double* src[3] = {x, y, z};
+    if (&dst == &c->S) c->g3_valid = false;  // the AoS3 gather copy no longer matches
     if (c->prec == LLG_FP64_PLAIN) {
         for (int a = 0; a < 3; ++a)
             SLLG_HIP(hipMemcpyAsync(dst.as<double>() + (size_t)a * n, src[a], n * 8, hipMemcpyHostToDevice, c->stream));
@@ -213,6 +214,7 @@ int step_fused(llg_ctx* c, uint64_t t) {
 }
 
 int do_steps(llg_ctx* c, uint64_t t0, uint32_t nsteps) {
+    if (c->scheme != LLG_HEUN) c->g3_valid = false;  // RK stages write S without its gather copy
     for (uint32_t k = 0; k < nsteps; ++k) {
         if (c->prec == LLG_FP64_PLAIN)
             SLLG_TRY(step_plain(c, t0 + k));
@@ -731,7 +733,7 @@ int llg_get_info(llg_ctx* c, llg_info* info) {
         else if (h.fmt == LLG_EXCH_STENCIL && h.per_row) M = (double)h.sk * 8.0 * nc;
     } else {
         const Exch<float>& x = c->ex32;
-        if (x.fmt == FMT_SELL_PACK) M = (double)x.nent / n * 4.0;
+        if (x.fmt == FMT_SELL_PACK || x.fmt == FMT_SELL_LINE) M = (double)x.nent / n * 4.0;
         else if (x.fmt == FMT_SELL_RAW) M = (double)x.nent / n * (4.0 + 4.0 * nc);
         else if (x.fmt == FMT_DIA) M = (double)h.ndiag * 4.0 * nc;
         else if (x.fmt == FMT_STENCIL && h.per_row) M = (double)h.sk * 4.0 * nc;
@@ -791,7 +793,7 @@ int create_with_store(const llg_desc* d, const std::shared_ptr<ExchStore>& store
 namespace {
 std::mutex g_tune_mu;
 struct TuneChoice {
-    int tpb, xcd, unr, thc;
+    int tpb, xcd, unr, thc, g3;
 };
 std::map<std::string, TuneChoice> g_tune_cache;
 
@@ -811,7 +813,7 @@ void tune_file_load() {
     if (!fp) return;
     char key[256];
     TuneChoice t{};
-    while (std::fscanf(fp, "%255s %d %d %d %d", key, &t.tpb, &t.xcd, &t.unr, &t.thc) == 5) g_tune_cache[key] = t;
+    while (std::fscanf(fp, "%255s %d %d %d %d %d", key, &t.tpb, &t.xcd, &t.unr, &t.thc, &t.g3) == 6) g_tune_cache[key] = t;
     std::fclose(fp);
 }
 
@@ -820,18 +822,19 @@ void tune_file_append(const std::string& key, const TuneChoice& t) {
     if (!f) return;
     FILE* fp = std::fopen(f, "a");
     if (!fp) return;
-    std::fprintf(fp, "%s %d %d %d %d\n", key.c_str(), t.tpb, t.xcd, t.unr, t.thc);
+    std::fprintf(fp, "%s %d %d %d %d %d\n", key.c_str(), t.tpb, t.xcd, t.unr, t.thc, t.g3);
     std::fclose(fp);
 }
 }  // namespace
 
-extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd, int unroll, int thermal_cache) {
+extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd, int unroll, int thermal_cache, int gather_copy) {
     if (!c || (tpb != 256 && tpb != 512 && tpb != 1024)) SLLG_FAIL(LLG_E_INVALID, "block size must be 256, 512 or 1024");
     if (unroll != 2 && unroll != 4) SLLG_FAIL(LLG_E_INVALID, "unroll must be 2 or 4");
     c->tune_tpb = tpb;
     c->tune_xcd = xcd ? 1 : 0;
     c->tune_unr = unroll;
     c->tune_thc = thermal_cache ? 1 : 0;
+    c->tune_g3 = gather_copy ? 1 : 0;
     return 0;
 }
 
@@ -842,6 +845,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     out->xcd = c->tune_xcd;
     out->unroll = c->tune_unr;
     out->thermal_cache = c->tune_thc;
+    out->gather_copy = c->tune_g3;
     if (c->prec != LLG_FP32 || c->scheme != LLG_HEUN) return 0;
     SLLG_HIP(hipSetDevice(c->device));
     const std::string key = tune_key(c);
@@ -858,6 +862,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
             c->tune_xcd = out->xcd = it->second.xcd;
             c->tune_unr = out->unroll = it->second.unr;
             c->tune_thc = out->thermal_cache = it->second.thc;
+            c->tune_g3 = out->gather_copy = it->second.g3;
             out->from_cache = 1;
             return 0;
         }
@@ -871,14 +876,17 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     const int unrs[2] = {2, 4};
     // handing the thermal field over only matters at T > 0 with counter noise
     const int nthc = (c->T > 0.0 && c->noise_mode == LLG_NOISE_COUNTER) ? 2 : 1;
+    // the AoS3 gather copy applies to the packed and line encodings
+    const int ng3 = (c->ex32.fmt == FMT_SELL_PACK || c->ex32.fmt == FMT_SELL_LINE) ? 2 : 1;
     double best = 1e300;
-    TuneChoice b{256, 1, 2, c->tune_thc};
+    TuneChoice b{256, 1, 2, c->tune_thc, c->tune_g3};
     hipEvent_t e0, e1;
     SLLG_HIP(hipEventCreate(&e0));
     SLLG_HIP(hipEventCreate(&e1));
     const int noise = c->noise_mode;
     c->noise_mode = LLG_NOISE_COUNTER;  // no host queue traffic while timing
     int rc = 0;
+    for (int gi = 0; gi < ng3 && !rc; ++gi)
     for (int hi = 0; hi < nthc && !rc; ++hi)
     for (int ui = 0; ui < nunr && !rc; ++ui)
         for (int ti = 0; ti < 3 && !rc; ++ti)
@@ -887,6 +895,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
                 c->tune_xcd = xcd;
                 c->tune_unr = unrs[ui];
                 if (nthc == 2) c->tune_thc = 1 - hi;
+                if (ng3 == 2) c->tune_g3 = 1 - gi;
                 rc = do_steps(c, 0, 2);
                 if (rc) break;
                 (void)hipEventRecord(e0, c->stream);
@@ -896,11 +905,11 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 const double per = ms / nsteps;
-                if (tpbs[ti] == 256 && xcd == 1 && unrs[ui] == 2 && hi == 0) out->ms_per_step_default = per;
+                if (tpbs[ti] == 256 && xcd == 1 && unrs[ui] == 2 && hi == 0 && gi == 0) out->ms_per_step_default = per;
                 out->candidates++;
                 if (per < best) {
                     best = per;
-                    b = TuneChoice{tpbs[ti], xcd, unrs[ui], c->tune_thc};
+                    b = TuneChoice{tpbs[ti], xcd, unrs[ui], c->tune_thc, c->tune_g3};
                 }
             }
     c->noise_mode = noise;
@@ -909,11 +918,13 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     SLLG_HIP(hipMemcpyAsync(c->S.p, save.p, c->S.bytes, hipMemcpyDeviceToDevice, c->stream));
     SLLG_HIP(hipStreamSynchronize(c->stream));
     c->det_valid = false;
+    c->g3_valid = false;
     if (rc) return rc;
     c->tune_tpb = out->tpb = b.tpb;
     c->tune_xcd = out->xcd = b.xcd;
     c->tune_unr = out->unroll = b.unr;
     c->tune_thc = out->thermal_cache = b.thc;
+    c->tune_g3 = out->gather_copy = b.g3;
     out->ms_per_step = best;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = b;

@@ -26,13 +26,20 @@ constexpr int kSlice = 64;      // sliced-ELL slice height = one wavefront
 constexpr int kMaxSub = 16;     // sublattices reduced on the device
 constexpr int kDictLds = 2048;  // floats of coupling dictionary staged in LDS (8 KB)
 
+// Line-mode group records: line_group(nc) neighbour lines per record, line_words(nc) words
+// (this group's line words, 3 shifts x nc couplings per line, the next group's line words
+// last; padded to whole 16-byte units).
+constexpr int line_group(int nc) { return nc == 1 ? 4 : (nc == 3 ? 2 : 1); }
+constexpr int line_words(int nc) { return (2 * line_group(nc) + 3 * line_group(nc) * nc + 3) / 4 * 4; }
+
 enum ExchFmt : int {
     FMT_NONE = 0,
     FMT_DIA = 1,
     FMT_CSR = 2,        // FP64 plain: CSR u32 + FP64 values
     FMT_STENCIL = 3,    // periodic stencil (shared or per-row values)
     FMT_SELL_PACK = 4,  // FP32: sliced ELL, u32 = (row offset << dbits) | dictionary index
-    FMT_SELL_RAW = 5    // FP32: sliced ELL, u32 col + FP32 values [ncomp][entries]
+    FMT_SELL_RAW = 5,   // FP32: sliced ELL, u32 col + FP32 values [ncomp][entries]
+    FMT_SELL_LINE = 6   // FP32: slices that are periodic lines, lane-uniform scalar coupling stream
 };
 
 // Per-species constants, precision R.
@@ -177,10 +184,13 @@ struct llg_ctx {
     int tune_xcd = 1;    // XCD-aware tile remap on/off (llg_autotune)
     int tune_unr = 2;    // packed-exchange chunks loaded ahead per wave (llg_autotune)
     int tune_thc = 1;    // predictor hands its thermal field to the corrector (llg_autotune)
+    int tune_g3 = 1;     // gather neighbours from the AoS3 copy of the spins (llg_autotune)
+    bool g3_valid = false;  // S3 holds the current S
 
     // device state
     sllg::DevBuf S, Y, A, H, FN, DET, XI, STG, STG0, DEM;  // SoA 3n (type per path)
     sllg::DevBuf HT;  // FP32 Heun: thermal field of the current step, predictor -> corrector
+    sllg::DevBuf S3, Y3;  // FP32 Heun: AoS3 gather copies [3 i + c] of S and Y
     sllg::DevBuf K[6];                                     // RK stage increments
     sllg::DevBuf spec;      // u8 species per spin (FP32 path, multi-species)
     sllg::DevBuf sub;       // u8 sublattice per spin

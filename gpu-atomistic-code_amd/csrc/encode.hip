@@ -177,6 +177,139 @@ struct TupleHash {
     }
 };
 
+// Line mode of the FP32 path.  When every 64-row slice is one periodic line of cells along
+// the fastest cell index (so neighbour j of lane l is lane (l + dz) mod 64 of an aligned
+// line starting at device row L, dz in {0, +1, -1}), the couplings of a slice are
+// lane-uniform: per line, one aligned load of the spins, and the shifted neighbours come
+// from a wave rotation.  A slice becomes a list of group records read with scalar loads,
+// G = line_group(ncomp) lines each, line_words(ncomp) words:
+//   [0, G)           line words: (L - slice base) | mask, L a multiple of 64, mask bit 0/1/2
+//                    = couplings present for dz = 0 / +1 / -1 (0 for an unused slot)
+//   [G, G + 3 G nc)  couplings [line][dz 0, +1, -1][component], FP32, zero where absent
+//   [W - G, W)       the next group's line words (zero after the last group); W is
+//                    rounded up to whole 16-byte units, the words between are zero
+// so the reader issues the next group's line loads before it uses the current group.
+// Entries of one row that name the same (line, dz) are summed (in FP64, rounded once).
+// Identical record lists are stored once; the first record is a zero record for empty
+// slices.  Entries are summed line by line, not in stored CSR order.
+bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vector<uint32_t>& idx,
+                 const std::vector<TupleKey>& tuples, uint32_t& stored, uint32_t& lines_row) {
+    ExchStore& X = *c->xs;
+    const HostExch& h = X.h;
+    const uint32_t n = c->n;
+    const int nc = h.ncomp;
+    const Perm P = X.perm;
+    if (n % kSlice != 0) return false;
+    const uint32_t nsl = n / kSlice;
+    const int G = line_group(nc), W = line_words(nc);
+    std::vector<uint32_t> words((size_t)W, 0u);  // the zero record
+    std::vector<uint2> table(nsl);
+    std::unordered_map<uint64_t, std::vector<uint32_t>> seen;  // record-list hash -> stored slices
+    std::vector<uint32_t> st;
+    stored = 0;
+    uint64_t lines_total = 0;
+    struct Line {
+        int32_t lrel;
+        uint32_t mask;
+        double v[3][9];
+    };
+    for (uint32_t s = 0; s < nsl; ++s) {
+        const uint32_t d0 = s * kSlice;
+        const uint32_t r0 = ref_of[d0];
+        const uint32_t cnt = h.xadj[r0 + 1] - h.xadj[r0];
+        std::vector<Line> lines;
+        for (uint32_t q = 0; q < cnt; ++q) {
+            int dzq = 2;  // none found
+            int64_t Lq = 0;
+            uint32_t tq = 0;
+            for (int dz : {0, 1, -1}) {
+                bool ok = true;
+                int64_t L0 = -1;
+                for (uint32_t l = 0; l < kSlice && ok; ++l) {
+                    const uint32_t r = ref_of[d0 + l];
+                    if (h.xadj[r + 1] - h.xadj[r] != cnt) return false;
+                    const uint32_t qq = h.xadj[r] + q;
+                    const int64_t j = P.to_dev(h.adj[qq]);
+                    const int64_t L = j - (int64_t)((l + (uint32_t)(dz + 64)) & 63u);
+                    if (l == 0) {
+                        L0 = L;
+                        tq = idx[qq];
+                    }
+                    ok = (L == L0) && (L % kSlice == 0) && (idx[qq] == tq);
+                }
+                if (ok) {
+                    dzq = dz;
+                    Lq = L0;
+                    break;
+                }
+            }
+            if (dzq == 2) return false;
+            const int32_t lrel = (int32_t)(Lq - (int64_t)d0);
+            Line* g = nullptr;
+            for (Line& x : lines)
+                if (x.lrel == lrel) g = &x;
+            if (!g) {
+                lines.push_back(Line{lrel, 0u, {}});
+                g = &lines.back();
+            }
+            const int k = dzq == 0 ? 0 : (dzq == 1 ? 1 : 2);
+            g->mask |= 1u << k;
+            for (int a = 0; a < nc; ++a) g->v[k][a] += tuples[tq].v[a];
+        }
+        st.clear();
+        const size_t ng = (lines.size() + (size_t)G - 1) / (size_t)G;
+        auto line_word = [&](size_t l) -> uint32_t {
+            return l < lines.size() ? ((uint32_t)lines[l].lrel | lines[l].mask) : 0u;
+        };
+        for (size_t gi = 0; gi < ng; ++gi) {
+            std::vector<uint32_t> rec((size_t)W, 0u);
+            for (int u = 0; u < G; ++u) {
+                const size_t l = gi * (size_t)G + (size_t)u;
+                rec[(size_t)u] = line_word(l);
+                if (gi + 1 < ng) rec[(size_t)(W - G + u)] = line_word(l + (size_t)G);
+                if (l >= lines.size()) continue;
+                for (int k = 0; k < 3; ++k)
+                    for (int a = 0; a < nc; ++a) {
+                        const float v = (float)lines[l].v[k][a];
+                        std::memcpy(&rec[(size_t)(G + (u * 3 + k) * nc + a)], &v, 4);
+                    }
+            }
+            st.insert(st.end(), rec.begin(), rec.end());
+        }
+        lines_total += lines.size();
+        const uint32_t nrec = (uint32_t)ng;
+        if (nrec == 0) {
+            table[s] = make_uint2(0u, 0u);
+            continue;
+        }
+        uint64_t hsh = 1469598103934665603ull;
+        for (uint32_t w : st) hsh = (hsh ^ w) * 1099511628211ull;
+        bool found = false;
+        auto& cand = seen[hsh];
+        for (uint32_t o : cand)
+            if (table[o].y == nrec && std::memcmp(words.data() + table[o].x, st.data(), st.size() * 4) == 0) {
+                table[s] = table[o];
+                found = true;
+                break;
+            }
+        if (!found) {
+            cand.push_back(s);
+            table[s] = make_uint2((uint32_t)words.size(), nrec);
+            words.insert(words.end(), st.begin(), st.end());
+            ++stored;
+        }
+    }
+    if (upload(X.d, words.data(), words.size(), c->stream)) return false;
+    if (upload(X.e, table.data(), table.size(), c->stream)) return false;
+    Exch<float>& x = X.ex32;
+    x.fmt = FMT_SELL_LINE;
+    x.slice = X.e.as<uint2>();
+    x.ent = X.d.as<uint32_t>();
+    x.nent = words.size();
+    lines_row = (uint32_t)(lines_total / nsl);
+    return true;
+}
+
 // CSR -> sliced ELL.  Packed layout: per slice, entries in chunks of 4 per lane,
 // [chunk][lane][4], so one uint4 load per lane reads 4 entries and a wave reads 1 KB
 // contiguous, then a tail of 2 per lane [lane][2] (one uint2 load); slice lengths are
@@ -319,7 +452,7 @@ int build_sell(llg_ctx* c) {
         SLLG_TRY(upload(X.dict64, d64.data(), d64.size(), c->stream));
         x.fmt = FMT_SELL_PACK;
         x.ndict = (uint32_t)tuples.size();
-        char buf[64];
+        char buf[96];
         std::snprintf(buf, sizeof buf, "sell64x4-pack-t%u-dict%u-slices%uof%u", P.na, x.ndict, stored, nsl);
         X.encoding = buf;
     } else {
@@ -370,6 +503,16 @@ int build_sell(llg_ctx* c) {
     x64.nent = nent;
     x64.ncell = ncell;
     x64.dbits = x.dbits;
+    // Line mode replaces only the FP32 kernels' view; the FP64 view (energy, deterministic
+    // field) keeps the packed encoding set up above.
+    if (pack && !(c->flags & (LLG_FLAG_NO_LINE_MODE | LLG_FLAG_NO_SLICE_SHARING))) {
+        uint32_t lstored = 0, lines_row = 0;
+        if (build_lines(c, ref_of, idx, tuples, lstored, lines_row)) {
+            char buf[96];
+            std::snprintf(buf, sizeof buf, "sell64-line-t%u-slices%uof%u-lines%u", P.na, lstored, nsl, lines_row);
+            X.encoding = buf;
+        }
+    }
     return 0;
 }
 

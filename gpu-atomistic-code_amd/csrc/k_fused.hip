@@ -71,14 +71,44 @@ __device__ __forceinline__ SpinBuf spin_buf(const float* S, uint32_t n) {
     return b;
 }
 
+// Neighbour gathers from either copy of the spins: the SoA arrays (three dword loads) or
+// the AoS3 gather copy [3 j + c] the Heun kernels keep beside them (one dwordx3 load).
+template <bool G3>
+__device__ __forceinline__ void gather3(const SpinBuf& sb, uint32_t j, float s[3]) {
+    if constexpr (G3) {
+        const auto q = __builtin_amdgcn_raw_buffer_load_b96(sb.r, j * 12u, 0, 0);
+        s[0] = __uint_as_float(q[0]);
+        s[1] = __uint_as_float(q[1]);
+        s[2] = __uint_as_float(q[2]);
+    } else {
+        const uint32_t off = j << 2;
+        s[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, off, 0, 0));
+        s[1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, off, sb.o1, 0));
+        s[2] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, off, sb.o2, 0));
+    }
+}
+// Lane l's spin of the 64-row line starting at device row line0 (line0 wave-uniform).
+template <bool G3>
+__device__ __forceinline__ void line3(const SpinBuf& sb, uint32_t lane, uint32_t line0, float s[3]) {
+    if constexpr (G3) {
+        const auto q = __builtin_amdgcn_raw_buffer_load_b96(sb.r, lane * 12u, line0 * 12u, 0);
+        s[0] = __uint_as_float(q[0]);
+        s[1] = __uint_as_float(q[1]);
+        s[2] = __uint_as_float(q[2]);
+    } else {
+        const uint32_t vo = lane << 2, so = line0 << 2;
+        s[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, vo, so, 0));
+        s[1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, vo, so + sb.o1, 0));
+        s[2] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, vo, so + sb.o2, 0));
+    }
+}
+
 // One packed entry: decode, gather the neighbour, read its coupling from the LDS dictionary.
-template <int NC>
+template <int NC, bool G3>
 __device__ __forceinline__ void pack_entry(uint32_t w, uint32_t i, uint32_t dbits, const SpinBuf& sb,
                                            const float* sdict, float dot[3]) {
-    const uint32_t off = pack_nb(w, i, dbits) << 2;
-    const float s[3] = {__uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, off, 0, 0)),
-                        __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, off, sb.o1, 0)),
-                        __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(sb.r, off, sb.o2, 0))};
+    float s[3];
+    gather3<G3>(sb, pack_nb(w, i, dbits), s);
     float v[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) v[k] = sdict[pack_dict(w, dbits) * NC + k];
@@ -90,7 +120,7 @@ __device__ __forceinline__ void pack_entry(uint32_t w, uint32_t i, uint32_t dbit
 // of the dictionary.
 // UNR chunks (4 UNR entries, 12 UNR gathers) are loaded before any is used, so that many
 // neighbour loads are in flight per wave; the launch-shape tuner picks UNR.
-template <int NC, int UNR>
+template <int NC, int UNR, bool G3>
 __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_t i, const float* __restrict__ S,
                                                    const float* sdict, float dot[3]) {
     dot[0] = dot[1] = dot[2] = 0.0f;
@@ -107,23 +137,154 @@ __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_
         for (int u = 0; u < UNR; ++u) cur[u] = p[(size_t)(c + u) * kSlice];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) {
-            pack_entry<NC>(cur[u].x, i, db, sb, sdict, dot);
-            pack_entry<NC>(cur[u].y, i, db, sb, sdict, dot);
-            pack_entry<NC>(cur[u].z, i, db, sb, sdict, dot);
-            pack_entry<NC>(cur[u].w, i, db, sb, sdict, dot);
+            pack_entry<NC, G3>(cur[u].x, i, db, sb, sdict, dot);
+            pack_entry<NC, G3>(cur[u].y, i, db, sb, sdict, dot);
+            pack_entry<NC, G3>(cur[u].z, i, db, sb, sdict, dot);
+            pack_entry<NC, G3>(cur[u].w, i, db, sb, sdict, dot);
         }
     }
     for (; c < nch; ++c) {
         const uint4 cur = p[(size_t)c * kSlice];
-        pack_entry<NC>(cur.x, i, db, sb, sdict, dot);
-        pack_entry<NC>(cur.y, i, db, sb, sdict, dot);
-        pack_entry<NC>(cur.z, i, db, sb, sdict, dot);
-        pack_entry<NC>(cur.w, i, db, sb, sdict, dot);
+        pack_entry<NC, G3>(cur.x, i, db, sb, sdict, dot);
+        pack_entry<NC, G3>(cur.y, i, db, sb, sdict, dot);
+        pack_entry<NC, G3>(cur.z, i, db, sb, sdict, dot);
+        pack_entry<NC, G3>(cur.w, i, db, sb, sdict, dot);
     }
     if (sd.y & 2u) {  // tail of 2 entries per lane, [lane][2]
         const uint2 t = reinterpret_cast<const uint2*>(p - lane + (size_t)nch * kSlice)[lane];
-        pack_entry<NC>(t.x, i, db, sb, sdict, dot);
-        pack_entry<NC>(t.y, i, db, sb, sdict, dot);
+        pack_entry<NC, G3>(t.x, i, db, sb, sdict, dot);
+        pack_entry<NC, G3>(t.y, i, db, sb, sdict, dot);
+    }
+}
+
+// Wave rotations by one lane (DPP): rot_next gives lane l the value of lane l+1, rot_prev
+// the value of lane l-1 (both wrap around the 64 lanes).
+__device__ __forceinline__ float rot_next(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float rot_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C, 0xF, 0xF, false));
+}
+
+// Line-mode group record, read with scalar loads by the whole wave (constant indices only,
+// so it stays in SGPRs):
+//   w[0, G)            this group's line words: start row relative to the slice's first row
+//                      (int32, multiple of 64) | mask (bit 0: unshifted, 1: +1 lane, 2: -1 lane)
+//   w[G, G + 3 G NC)   couplings [line][shift][component], FP32 (then zero padding)
+//   w[W - G, W)        the next group's line words (zero after the last group)
+template <int NC>
+struct LineRecW {
+    static constexpr int G = line_group(NC), W = line_words(NC), NX = W - G;
+    uint32_t w[W];
+};
+// Whole 16-byte units, so the record arrives in few wide scalar loads.
+template <int NC>
+__device__ __forceinline__ LineRecW<NC> load_line_rec(const uint32_t* __restrict__ p) {
+    const uint4* __restrict__ q = reinterpret_cast<const uint4*>(p);
+    LineRecW<NC> o;
+#pragma unroll
+    for (int k = 0; k < LineRecW<NC>::W / 4; ++k) {
+        const uint4 v = q[k];
+        o.w[4 * k] = v.x;
+        o.w[4 * k + 1] = v.y;
+        o.w[4 * k + 2] = v.z;
+        o.w[4 * k + 3] = v.w;
+    }
+    return o;
+}
+
+// Couplings of line u, shift k of the record against the spin vector s.
+template <int NC>
+__device__ __forceinline__ void line_entry(const LineRecW<NC>& r, int u, int k, const float s[3], float dot[3]) {
+    float v[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) v[a] = __uint_as_float(r.w[LineRecW<NC>::G + (u * 3 + k) * NC + a]);
+    accum<NC, float>(v, s, dot);
+}
+
+// A loaded line stays one 96-bit value across the pipelined loop (an aligned VGPR tuple,
+// so the loads write their final registers and need no copies).
+typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+
+template <bool G3>
+__device__ __forceinline__ u32x3 line_v(const SpinBuf& sb, uint32_t lane, uint32_t line0) {
+    if constexpr (G3) {
+        return __builtin_amdgcn_raw_buffer_load_b96(sb.r, lane * 12u, line0 * 12u, 0);
+    } else {
+        const uint32_t vo = lane << 2, so = line0 << 2;
+        u32x3 v;
+        v.x = __builtin_amdgcn_raw_buffer_load_b32(sb.r, vo, so, 0);
+        v.y = __builtin_amdgcn_raw_buffer_load_b32(sb.r, vo, so + sb.o1, 0);
+        v.z = __builtin_amdgcn_raw_buffer_load_b32(sb.r, vo, so + sb.o2, 0);
+        return v;
+    }
+}
+
+// The G lines named by line words lw, loaded into x.
+template <int NC, bool G3>
+__device__ __forceinline__ void line_loads(const SpinBuf& sb, uint32_t lane, int32_t base, const uint32_t* lw,
+                                           u32x3* x) {
+#pragma unroll
+    for (int u = 0; u < line_group(NC); ++u)
+        x[u] = line_v<G3>(sb, lane, __builtin_amdgcn_readfirstlane((uint32_t)(base + (int32_t)(lw[u] & ~63u))));
+}
+
+// One group's arithmetic: per line, the unshifted and the two rotated neighbours.
+template <int NC>
+__device__ __forceinline__ void line_group_sum(const LineRecW<NC>& r, const u32x3* x, float dot[3]) {
+#pragma unroll
+    for (int u = 0; u < LineRecW<NC>::G; ++u) {
+        const uint32_t m = r.w[u];
+        const float s0[3] = {__uint_as_float(x[u].x), __uint_as_float(x[u].y), __uint_as_float(x[u].z)};
+        if (m & 1u) line_entry<NC>(r, u, 0, s0, dot);
+        if (m & 2u) {
+            const float xs[3] = {rot_next(s0[0]), rot_next(s0[1]), rot_next(s0[2])};
+            line_entry<NC>(r, u, 1, xs, dot);
+        }
+        if (m & 4u) {
+            const float xs[3] = {rot_prev(s0[0]), rot_prev(s0[1]), rot_prev(s0[2])};
+            line_entry<NC>(r, u, 2, xs, dot);
+        }
+    }
+}
+
+// Line-mode reader (FMT_SELL_LINE): per group of G neighbour lines, one aligned line load
+// each, the +-1 shifted neighbours as wave rotations of the loaded line.  Software
+// pipelined: a group's record names the next group's lines, whose loads are issued before
+// the current group's arithmetic.
+template <int NC, bool G3>
+__device__ __forceinline__ void exch_row_line(const Exch<float>& e, uint32_t i, const float* __restrict__ S,
+                                              float dot[3]) {
+    using RW = LineRecW<NC>;
+    constexpr int G = RW::G, W = RW::W, NX = RW::NX;
+    dot[0] = dot[1] = dot[2] = 0.0f;
+    const uint32_t sl = __builtin_amdgcn_readfirstlane(i >> 6);
+    const uint2 sd = e.slice[sl];
+    const uint32_t* __restrict__ rp = e.ent + sd.x;
+    const uint32_t ngrp = sd.y, lane = i & 63u;
+    const int32_t base = (int32_t)(sl << 6);
+    if (ngrp == 0) return;
+    const SpinBuf sb = spin_buf(S, e.n);
+    // Two line buffers used in turn (no register copies, so no early waits on the loads in
+    // flight); past the last group the record's next-line words name the slice's own line.
+    u32x3 xa[G], xb[G];
+    {
+        const RW r0 = load_line_rec<NC>(rp);
+        line_loads<NC, G3>(sb, lane, base, r0.w, xa);
+    }
+    for (uint32_t r = 0;; r += 2) {
+        {
+            const RW cur = load_line_rec<NC>(rp + (size_t)r * W);
+            line_loads<NC, G3>(sb, lane, base, cur.w + NX, xb);
+            line_group_sum<NC>(cur, xa, dot);
+        }
+        if (r + 1 >= ngrp) break;
+        {
+            const RW cur = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
+            line_loads<NC, G3>(sb, lane, base, cur.w + NX, xa);
+            line_group_sum<NC>(cur, xb, dot);
+        }
+        if (r + 2 >= ngrp) break;
     }
 }
 
@@ -203,16 +364,31 @@ __device__ __forceinline__ void row_of_thread(const FusedArgs& a, uint32_t tpb, 
     }
 }
 
-template <int FMT, int NC, int UNR = 2>
+// Row in range.  In line mode n is a multiple of 64 and tiles start at multiples of 64, so
+// a wave is wholly in or out: the test is made on a wave-uniform value, which keeps the
+// line reader out of divergent control flow (its record words stay scalar).
+template <int FMT>
+__device__ __forceinline__ bool row_live(const FusedArgs& a, uint32_t i) {
+    if constexpr (FMT == FMT_SELL_LINE)
+        return __builtin_amdgcn_readfirstlane(i & ~63u) < a.n;
+    else
+        return i < a.n;
+}
+
+// src: the spins (SoA) the field is evaluated at; gsrc: where neighbours are gathered from
+// (src itself, or with G3 its AoS3 copy).
+template <int FMT, int NC, int UNR = 2, bool G3 = false>
 __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32_t iref, const float* __restrict__ src,
-                                          const float* sdict, float s[3], float f[3]) {
+                                          const float* __restrict__ gsrc, const float* sdict, float s[3], float f[3]) {
     const uint32_t n = a.n;
     s[0] = src[i];
     s[1] = src[n + i];
     s[2] = src[2 * (size_t)n + i];
     float h[3];
     if constexpr (FMT == FMT_SELL_PACK)
-        exch_row_pack_fast<NC, UNR>(a.ex, i, src, sdict, h);
+        exch_row_pack_fast<NC, UNR, G3>(a.ex, i, gsrc, sdict, h);
+    else if constexpr (FMT == FMT_SELL_LINE)
+        exch_row_line<NC, G3>(a.ex, i, gsrc, h);
     else
         exch_any<FMT, NC, float, float, float>(a.ex, i, src, h);
     if (a.ex.fs_xadj) {
@@ -277,16 +453,23 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
         __syncthreads();                                                          \
     }
 
-template <int FMT, int NC, int TPB, int UNR>
-__global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S, float* __restrict__ Y,
-                                                    float* __restrict__ A) {
+// With G3 the kernels gather neighbours from the AoS3 copy (S3 / Y3, [3 i + c]) and keep it
+// current: the predictor writes Y3 beside Y, the corrector S3 beside S.
+struct Spin3 {
+    float x, y, z;
+};
+
+template <int FMT, int NC, int TPB, int UNR, bool G3>
+__global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S,
+                                                    const float* __restrict__ S3, float* __restrict__ Y,
+                                                    float* __restrict__ Y3, float* __restrict__ A) {
     SLLG_STAGE_DICT(FMT, NC, a)
     uint32_t i, iref;
     row_of_thread(a, TPB, i, iref);
-    if (i >= a.n) return;
+    if (!row_live<FMT>(a, i)) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC, UNR>(a, i, iref, S, sdict, s, f);
+    fused_rhs<FMT, NC, UNR, G3>(a, i, iref, S, G3 ? S3 : S, sdict, s, f);
     float y[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -296,24 +479,34 @@ __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __r
         Y[(size_t)c * n + i] = y[c] * r;
         A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
     }
+    if constexpr (G3) reinterpret_cast<Spin3*>(Y3)[i] = Spin3{y[0] * r, y[1] * r, y[2] * r};
 }
 
-template <int FMT, int NC, int TPB, int UNR>
+template <int FMT, int NC, int TPB, int UNR, bool G3>
 __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __restrict__ Y,
-                                                    const float* __restrict__ A, float* __restrict__ S) {
+                                                    const float* __restrict__ Y3, const float* __restrict__ A,
+                                                    float* __restrict__ S, float* __restrict__ S3) {
     SLLG_STAGE_DICT(FMT, NC, a)
     uint32_t i, iref;
     row_of_thread(a, TPB, i, iref);
-    if (i >= a.n) return;
+    if (!row_live<FMT>(a, i)) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC, UNR>(a, i, iref, Y, sdict, s, f);
+    fused_rhs<FMT, NC, UNR, G3>(a, i, iref, Y, G3 ? Y3 : Y, sdict, s, f);
     float p[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = A[(size_t)c * n + i] + a.half_rdt * f[c];
     const float r = rsqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) S[(size_t)c * n + i] = p[c] * r;
+    if constexpr (G3) reinterpret_cast<Spin3*>(S3)[i] = Spin3{p[0] * r, p[1] * r, p[2] * r};
+}
+
+// SoA -> AoS3 gather copy (after the spins changed outside the Heun kernels).
+__global__ __launch_bounds__(kTPB) void k_to_aos3(uint32_t n, const float* __restrict__ S, float* __restrict__ S3) {
+    const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
+    if (i >= n) return;
+    reinterpret_cast<Spin3*>(S3)[i] = Spin3{S[i], S[n + i], S[2 * (size_t)n + i]};
 }
 
 // Batched Heun: one launch covers the tiles of every member of a batch (independent spin
@@ -344,7 +537,7 @@ __global__ __launch_bounds__(kTPB) void k_heun_batch(const BatchMember* __restri
     const uint32_t n = a.n;
     float s[3], f[3];
     if constexpr (PHASE == 0) {
-        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.S, sdict, s, f);
+        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.S, bm.S, sdict, s, f);
         float y[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -355,7 +548,7 @@ __global__ __launch_bounds__(kTPB) void k_heun_batch(const BatchMember* __restri
             bm.A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
         }
     } else {
-        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.Y, sdict, s, f);
+        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.Y, bm.Y, sdict, s, f);
         float p[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[c] = bm.A[(size_t)c * n + i] + a.half_rdt * f[c];
@@ -376,7 +569,7 @@ __global__ __launch_bounds__(kTPB) void k_rk4(FusedArgs a, const float* __restri
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), M == 0 ? S : Yin, sdict, s, f);
+    fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), M == 0 ? S : Yin, M == 0 ? S : Yin, sdict, s, f);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const size_t o = (size_t)c * n + i;
@@ -418,7 +611,7 @@ __global__ __launch_bounds__(kTPB) void k_rk5(FusedArgs a, const float* __restri
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), M == 0 ? S : Yin, sdict, s, f);
+    fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), M == 0 ? S : Yin, M == 0 ? S : Yin, sdict, s, f);
     float* Ks[5] = {K0, K1, K2, K3, K4};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -524,6 +717,10 @@ static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
             if ((NC_VAL) == 1) { MACRO(FMT_SELL_PACK, 1); }                         \
             else if ((NC_VAL) == 3) { MACRO(FMT_SELL_PACK, 3); }                    \
             else { MACRO(FMT_SELL_PACK, 9); }                                       \
+        } else if ((FMT_VAL) == FMT_SELL_LINE) {                                    \
+            if ((NC_VAL) == 1) { MACRO(FMT_SELL_LINE, 1); }                         \
+            else if ((NC_VAL) == 3) { MACRO(FMT_SELL_LINE, 3); }                    \
+            else { MACRO(FMT_SELL_LINE, 9); }                                       \
         } else if ((FMT_VAL) == FMT_SELL_RAW) {                                     \
             if ((NC_VAL) == 1) { MACRO(FMT_SELL_RAW, 1); }                          \
             else if ((NC_VAL) == 3) { MACRO(FMT_SELL_RAW, 3); }                     \
@@ -539,50 +736,102 @@ static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
         }                                                                           \
     } while (0)
 
-// The Heun pair runs with the autotuned block size (256, 512 or 1024 threads).
-template <int F, int N, int TPB>
-static void launch_heun_pred(const FusedArgs& a, const float* S, float* Y, float* A, uint32_t n, int unr,
-                             hipStream_t st) {
-    if constexpr (F == FMT_SELL_PACK) {
-        if (unr == 4) {
-            k_heun_pred<F, N, TPB, 4><<<blocks_for(n, TPB), TPB, 0, st>>>(a, S, Y, A);
+// The Heun pair runs with the autotuned block size (256, 512 or 1024 threads), gather depth
+// (packed encoding) and gather copy (packed and line encodings).
+struct HeunBufs {
+    float *S, *S3, *Y, *Y3, *A;
+};
+
+template <int F, int N, int TPB, int UNR>
+static void launch_pred_g(const FusedArgs& a, const HeunBufs& b, uint32_t n, bool g3, hipStream_t st) {
+    if constexpr (F == FMT_SELL_PACK || F == FMT_SELL_LINE) {
+        if (g3) {
+            k_heun_pred<F, N, TPB, UNR, true><<<blocks_for(n, TPB), TPB, 0, st>>>(a, b.S, b.S3, b.Y, b.Y3, b.A);
             return;
         }
     }
-    k_heun_pred<F, N, TPB, 2><<<blocks_for(n, TPB), TPB, 0, st>>>(a, S, Y, A);
+    k_heun_pred<F, N, TPB, UNR, false><<<blocks_for(n, TPB), TPB, 0, st>>>(a, b.S, b.S3, b.Y, b.Y3, b.A);
+}
+template <int F, int N, int TPB, int UNR>
+static void launch_corr_g(const FusedArgs& a, const HeunBufs& b, uint32_t n, bool g3, hipStream_t st) {
+    if constexpr (F == FMT_SELL_PACK || F == FMT_SELL_LINE) {
+        if (g3) {
+            k_heun_corr<F, N, TPB, UNR, true><<<blocks_for(n, TPB), TPB, 0, st>>>(a, b.Y, b.Y3, b.A, b.S, b.S3);
+            return;
+        }
+    }
+    k_heun_corr<F, N, TPB, UNR, false><<<blocks_for(n, TPB), TPB, 0, st>>>(a, b.Y, b.Y3, b.A, b.S, b.S3);
 }
 
 template <int F, int N, int TPB>
-static void launch_heun_corr(const FusedArgs& a, const float* Y, const float* A, float* S, uint32_t n, int unr,
-                             hipStream_t st) {
+static void launch_heun_pred(const FusedArgs& a, const HeunBufs& b, uint32_t n, int unr, bool g3, hipStream_t st) {
     if constexpr (F == FMT_SELL_PACK) {
         if (unr == 4) {
-            k_heun_corr<F, N, TPB, 4><<<blocks_for(n, TPB), TPB, 0, st>>>(a, Y, A, S);
+            launch_pred_g<F, N, TPB, 4>(a, b, n, g3, st);
             return;
         }
     }
-    k_heun_corr<F, N, TPB, 2><<<blocks_for(n, TPB), TPB, 0, st>>>(a, Y, A, S);
+    launch_pred_g<F, N, TPB, 2>(a, b, n, g3, st);
+}
+
+template <int F, int N, int TPB>
+static void launch_heun_corr(const FusedArgs& a, const HeunBufs& b, uint32_t n, int unr, bool g3, hipStream_t st) {
+    if constexpr (F == FMT_SELL_PACK) {
+        if (unr == 4) {
+            launch_corr_g<F, N, TPB, 4>(a, b, n, g3, st);
+            return;
+        }
+    }
+    launch_corr_g<F, N, TPB, 2>(a, b, n, g3, st);
 }
 
 // The predictor stores the thermal field it draws for the corrector of the same step
 // (12 B written + 12 B read per spin instead of a second Philox + Box-Muller draw).
 static bool cache_thermal(const llg_ctx* c) { return c->tune_thc && c->T > 0.0 && c->noise_mode == LLG_NOISE_COUNTER; }
 
+// The AoS3 gather copy is used for the packed and line encodings when the tuner chose it.
+static bool use_g3(const llg_ctx* c) {
+    return c->tune_g3 && (c->ex32.fmt == FMT_SELL_PACK || c->ex32.fmt == FMT_SELL_LINE);
+}
+
+static int heun_bufs(llg_ctx* c, bool g3, HeunBufs& b) {
+    b.S = c->S.as<float>();
+    b.Y = c->Y.as<float>();
+    b.A = c->A.as<float>();
+    b.S3 = b.Y3 = nullptr;
+    if (g3) {
+        const size_t bytes = 3 * (size_t)c->n * sizeof(float);
+        if (c->S3.bytes < bytes) {
+            SLLG_TRY(c->S3.alloc(bytes));
+            c->g3_valid = false;
+        }
+        if (c->Y3.bytes < bytes) SLLG_TRY(c->Y3.alloc(bytes));
+        b.S3 = c->S3.as<float>();
+        b.Y3 = c->Y3.as<float>();
+    }
+    return 0;
+}
+
 int fused_heun_pred(llg_ctx* c, const StepSched& s) {
     FusedArgs a = fused_args(c, s);
-    float* S = c->S.as<float>();
-    float* Y = c->Y.as<float>();
-    float* A = c->A.as<float>();
+    const bool g3 = use_g3(c);
+    HeunBufs b;
+    SLLG_TRY(heun_bufs(c, g3, b));
+    if (g3 && !c->g3_valid) {  // the spins changed outside the Heun kernels
+        k_to_aos3<<<blocks_for(c->n, kTPB), kTPB, 0, c->stream>>>(c->n, b.S, b.S3);
+        SLLG_HIP(hipGetLastError());
+        c->g3_valid = true;
+    }
     if (cache_thermal(c)) {
         if (c->HT.bytes < 3 * (size_t)c->n * sizeof(float)) SLLG_TRY(c->HT.alloc(3 * (size_t)c->n * sizeof(float)));
         a.th_out = c->HT.as<float>();
     }
     const int tpb = c->tune_tpb, unr = c->tune_unr;
-#define SLLG_L(F, N)                                                                  \
-    do {                                                                              \
-        if (tpb == 1024) launch_heun_pred<F, N, 1024>(a, S, Y, A, c->n, unr, c->stream); \
-        else if (tpb == 512) launch_heun_pred<F, N, 512>(a, S, Y, A, c->n, unr, c->stream); \
-        else launch_heun_pred<F, N, 256>(a, S, Y, A, c->n, unr, c->stream);           \
+#define SLLG_L(F, N)                                                                          \
+    do {                                                                                      \
+        if (tpb == 1024) launch_heun_pred<F, N, 1024>(a, b, c->n, unr, g3, c->stream);        \
+        else if (tpb == 512) launch_heun_pred<F, N, 512>(a, b, c->n, unr, g3, c->stream);     \
+        else launch_heun_pred<F, N, 256>(a, b, c->n, unr, g3, c->stream);                     \
     } while (0)
     SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
@@ -592,20 +841,21 @@ int fused_heun_pred(llg_ctx* c, const StepSched& s) {
 
 int fused_heun_corr(llg_ctx* c, const StepSched& s) {
     FusedArgs a = fused_args(c, s);
-    float* S = c->S.as<float>();
-    float* Y = c->Y.as<float>();
-    float* A = c->A.as<float>();
+    const bool g3 = use_g3(c);
+    HeunBufs b;
+    SLLG_TRY(heun_bufs(c, g3, b));
     if (cache_thermal(c) && c->HT.p) a.th_in = c->HT.as<float>();
     const int tpb = c->tune_tpb, unr = c->tune_unr;
-#define SLLG_L(F, N)                                                                  \
-    do {                                                                              \
-        if (tpb == 1024) launch_heun_corr<F, N, 1024>(a, Y, A, S, c->n, unr, c->stream); \
-        else if (tpb == 512) launch_heun_corr<F, N, 512>(a, Y, A, S, c->n, unr, c->stream); \
-        else launch_heun_corr<F, N, 256>(a, Y, A, S, c->n, unr, c->stream);           \
+#define SLLG_L(F, N)                                                                          \
+    do {                                                                                      \
+        if (tpb == 1024) launch_heun_corr<F, N, 1024>(a, b, c->n, unr, g3, c->stream);        \
+        else if (tpb == 512) launch_heun_corr<F, N, 512>(a, b, c->n, unr, g3, c->stream);     \
+        else launch_heun_corr<F, N, 256>(a, b, c->n, unr, g3, c->stream);                     \
     } while (0)
     SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
     SLLG_HIP(hipGetLastError());
+    c->g3_valid = g3;  // S3 now matches S only if the corrector wrote it
     return 0;
 }
 
@@ -676,8 +926,12 @@ int fused_detfield(llg_ctx* c, const float* Sp, double* out) {
     a.an = c->an64;
     a.ex = c->ex64;
     const dim3 g(blocks_for(c->n, kTPB)), b(kTPB);
-#define SLLG_L(F, N) k_fused_detfield<F, N><<<g, b, 0, c->stream>>>(a, Sp, out)
-    SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+    // the FP64 view of the couplings is the packed or raw encoding (never line mode)
+#define SLLG_L(F, N)                                                                  \
+    do {                                                                              \
+        if constexpr (F != FMT_SELL_LINE) k_fused_detfield<F, N><<<g, b, 0, c->stream>>>(a, Sp, out); \
+    } while (0)
+    SLLG_DISPATCH_FMT_NC(c->ex64.fmt, c->ex64.ncomp, SLLG_L);
 #undef SLLG_L
     SLLG_HIP(hipGetLastError());
     return 0;

@@ -349,7 +349,7 @@ int reduce_energy(llg_ctx* c, const void* dem_now, double* dev_out) {
     a.an = c->an64;
     a.ex = c->ex64;
     double* part = c->red_part.as<double>();
-    const int fmt = a.f64 ? c->ex64.fmt : c->ex32.fmt;
+    const int fmt = c->ex64.fmt;  // the FP32 path's FP64 view is the packed or raw encoding
     const int nc = c->ex64.ncomp;
 #define SLLG_E(F, N)                                                                                 \
     do {                                                                                             \

@@ -42,12 +42,12 @@ def sc_ferro(L=32, precision="fp64", T=300.0, J=3.0e-21, K=1.0e-23, lam=0.1, dt=
 
 
 def fcc_3shell(L=64, precision="fp32", lam=0.05, dt=1e-16, Kc=5e-24, seed=42, update=100, noise="counter",
-               scheme="heun", ncomp=1, J=(5e-21, 1e-21, -4e-22)):
+               scheme="heun", ncomp=1, J=(5e-21, 1e-21, -4e-22), dims=None):
     uc = unit_cell("fcc")
     ints, _ = shells(uc, list(J))
-    dims = (L, L, L)
+    dims = tuple(dims) if dims is not None else (L, L, L)
     xadj, adj, vals = build_csr(uc, dims, ints, [MU] * 4, ncomp=ncomp)
-    n = L ** 3 * 4
+    n = int(np.prod(dims)) * 4
     sysd = System(n=n, species=[_species(lam=lam)], anis=dict(kc=Kc),
                   exch=dict(format=EXCH_CSR, xadj=xadj, adjncy=adj, vals=vals, n_atoms=4), dt=dt, scheme=scheme,
                   precision=precision, noise=noise, update=update, seed=seed)

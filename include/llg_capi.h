@@ -86,7 +86,8 @@ typedef enum llg_noise {
 /* Encoding switches of the FP32 path, for measurement; results do not depend on them. */
 #define LLG_FLAG_NO_XCD_REMAP 0x100u        /* tiles in launch order, no per-XCD row ranges */
 #define LLG_FLAG_NO_SUBLATTICE_ORDER 0x200u /* keep the reference spin order on the device */
-#define LLG_FLAG_NO_SLICE_SHARING 0x400u    /* store every 64-row slice's entries, even repeats */
+#define LLG_FLAG_NO_SLICE_SHARING 0x400u    /* store every 64-row slice's entries, even repeats (no line mode) */
+#define LLG_FLAG_NO_LINE_MODE 0x800u        /* keep per-lane gathers where slices are periodic lines */
 
 /* Per-species (unit-cell atom) constants; reference geom::ucm (inc/unitcell.h) and
  * the per-spin arrays filled in src/geom.cpp:239-247 and src/llg.cpp:591-592,626-631. */
@@ -298,8 +299,9 @@ int llg_profile_kernels(llg_ctx* ctx, uint64_t t0, uint32_t nsteps, int max_kind
 /* Launch-shape autotuner (no reference counterpart): times the FP32 Heun kernel pair for
  * every candidate block size (256, 512, 1024 threads) with and without the XCD-aware
  * tile remap, for the packed encoding 2 or 4 coupling chunks loaded ahead per wave, and
- * with the thermal field drawn in both passes or handed from predictor to corrector
- * (bit-identical either way), on this handle's own couplings, for nsteps steps each;
+ * with the thermal field drawn in both passes or handed from predictor to corrector, and
+ * with neighbours gathered from the SoA spins or from an AoS3 copy the kernels keep beside
+ * them (bit-identical either way), on this handle's own couplings, for nsteps steps each;
  * keeps the fastest.
  * The spin state is restored afterwards.  Results are cached per (device, encoding, n,
  * entries) in the process and, if the environment variable SLLG_TUNE_FILE names a file,
@@ -311,9 +313,11 @@ typedef struct llg_tune_result {
     int candidates, from_cache;
     int unroll;
     int thermal_cache;  /* 1: the predictor hands its thermal field to the corrector */
+    int gather_copy;    /* 1: neighbours are gathered from an AoS3 copy of the spins */
+    int pad_;
 } llg_tune_result;
 int llg_autotune(llg_ctx* ctx, uint32_t nsteps, llg_tune_result* out);
-int llg_set_launch(llg_ctx* ctx, int tpb, int xcd, int unroll, int thermal_cache);
+int llg_set_launch(llg_ctx* ctx, int tpb, int xcd, int unroll, int thermal_cache, int gather_copy);
 
 /* Algorithmic HBM bytes per spin-update of the configured path, split into the
  * kernels of one step, and the device bytes the handle holds. */

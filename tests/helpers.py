@@ -14,7 +14,7 @@ def full_anis():
                 k4par_dirs=ax)
 
 
-def tensor_csr_system(L=6, precision="fp64", ncomp=9, seed=5, fmt="csr", scheme="heun", noise="counter"):
+def tensor_csr_system(L=6, precision="fp64", ncomp=9, seed=5, fmt="csr", scheme="heun", noise="counter", dims=None):
     """sc lattice, NN + NNN exchange with random anisotropic/antisymmetric 3x3 tensors per
     interaction (J_ji = J_ij^T), two species on a checkerboard, all anisotropy terms."""
     rng = np.random.default_rng(seed)
@@ -34,11 +34,11 @@ def tensor_csr_system(L=6, precision="fp64", ncomp=9, seed=5, fmt="csr", scheme=
             J = np.diag(np.diag(J))
         dicts.append(dict(atom_i=0, atom_j=0, dc=dc, J=J))
     ints = sllg.interactions_from(dicts)
-    dims = (L, L, L)
+    dims = tuple(dims) if dims is not None else (L, L, L)
     xadj, adj, vals = sllg.build_csr(uc, dims, ints, [2.22], ncomp=ncomp)
-    n = L ** 3
+    n = int(np.prod(dims))
     idx = np.arange(n)
-    x, y, z = idx // (L * L), (idx // L) % L, idx % L
+    x, y, z = idx // (dims[1] * dims[2]), (idx // dims[2]) % dims[1], idx % dims[2]
     spec = ((x + y + z) % 2).astype(np.uint32)
     species = [dict(mu=2.22, lambda_=0.1, gamma_rel=1.0, k1u=1e-23, k1u_dir=(0, 0, 1), sublattice=0),
                dict(mu=2.22, lambda_=0.2, gamma_rel=1.3, k1u=-4e-24, k1u_dir=(0.6, 0.0, 0.8), sublattice=1)]

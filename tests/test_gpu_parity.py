@@ -273,9 +273,10 @@ def test_launch_shape_does_not_change_results(gpu):
     gives identical bits.  The autotuner keeps the state it was given."""
     sysd, s0 = _fcc_multi(L=4)
     res = []
-    for tpb, xcd, unr, thc in ((256, 1, 2, 1), (512, 0, 4, 0), (1024, 1, 4, 1), (256, 0, 2, 0), (256, 1, 4, 0)):
+    for tpb, xcd, unr, thc, g3 in ((256, 1, 2, 1, 0), (512, 0, 4, 0, 1), (1024, 1, 4, 1, 1), (256, 0, 2, 0, 0),
+                                    (256, 1, 4, 0, 1)):
         g = sllg.Context(sysd)
-        g.set_launch(tpb, xcd, unr, thc)
+        g.set_launch(tpb, xcd, unr, thc, g3)
         g.set_spins(s0)
         g.set_temperature(150.0)
         g.step(0, 40)
@@ -289,7 +290,7 @@ def test_launch_shape_does_not_change_results(gpu):
     before = g.get_spins()
     t = g.autotune(5)
     assert t["tpb"] in (256, 512, 1024) and t["xcd"] in (0, 1) and t["unroll"] in (2, 4)
-    assert t["candidates"] == 24 and t["thermal_cache"] in (0, 1)
+    assert t["candidates"] == 48 and t["thermal_cache"] in (0, 1) and t["gather_copy"] in (0, 1)
     assert np.array_equal(g.get_spins(), before)
     g.step(10, 30)
     assert np.array_equal(g.get_spins(), res[0])
@@ -385,3 +386,57 @@ def test_fourspin_sublattice_order(gpu):
     print("fourspin fcc sublattice-major", d)
     assert d <= 1e-4
     assert _maxdiff(g.field(), o.field()) <= 1e-4 * np.abs(o.field()).max()
+
+
+@pytest.mark.parametrize("case", ["sc", "fcc", "tensor"])
+def test_line_mode(gpu, case):
+    """Slices that are periodic 64-cell lines use the line reader (aligned line loads, wave
+    rotations for the +-1 neighbours, scalar coupling stream).  Same physics as the packed
+    reader to FP32 rounding (entries are summed per line, not in stored order) and as the
+    oracle; energies agree."""
+    if case == "sc":
+        sysd, s0, _ = W.sc_ferro(precision="fp32", dims=(4, 4, 64))
+    elif case == "fcc":
+        sysd, s0 = W.fcc_3shell(precision="fp32", dims=(2, 3, 64))
+        sysd.species = [dict(mu=2.22 if t != 2 else 1.5, lambda_=0.05, sublattice=t) for t in range(4)]
+        sysd.spin_species = (np.arange(sysd.n) % 4).astype(np.uint32)
+        sysd.n_sublattice = 4
+    else:
+        sysd, s0 = tensor_csr_system(precision="fp32", ncomp=9, dims=(3, 2, 64))
+    res, encs = [], []
+    for flags in (0, sllg.FLAG_NO_LINE_MODE):
+        sysd.flags = flags
+        g = sllg.Context(sysd)
+        encs.append(g.info()["exch_encoding"])
+        g.set_spins(s0)
+        g.set_temperature(150.0)
+        g.set_applied(np.array([0.1, 0.0, 0.4]))
+        g.step(0, 40)
+        res.append((g.get_spins(), g.reduce()[1], g.field()))
+    sysd.flags = 0
+    print(case, encs)
+    assert "line" in encs[0] and "line" not in encs[1]
+    assert _maxdiff(res[0][0], res[1][0]) <= 2e-5
+    assert abs(res[0][1] - res[1][1]) <= 1e-5 * abs(res[1][1])
+    assert _maxdiff(res[0][2], res[1][2]) <= 1e-5 * np.abs(res[1][2]).max()
+    g, o = run_pair(sysd, s0, 40, T=0.0, B=(0.0, 0.2, 0.5))
+    assert _maxdiff(g.get_spins(), o.get_spins()) <= 1e-4
+
+
+def test_line_mode_gather_copy(gpu):
+    """Line mode with and without the AoS3 gather copy gives identical bits, also across a
+    set_spins between steps (the copy is rebuilt from the new state)."""
+    sysd, s0 = W.fcc_3shell(precision="fp32", dims=(2, 2, 64))
+    res = []
+    for g3 in (0, 1):
+        g = sllg.Context(sysd)
+        assert "line" in g.info()["exch_encoding"]
+        g.set_launch(256, 1, 2, 1, g3)
+        g.set_spins(s0)
+        g.set_temperature(100.0)
+        g.step(0, 20)
+        mid = g.get_spins()
+        g.set_spins(mid[:, ::-1].copy())  # new state from the host: gather copy must follow
+        g.step(20, 20)
+        res.append(g.get_spins())
+    assert np.array_equal(res[0], res[1])
