@@ -159,11 +159,13 @@ __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_
 
 // Wave rotations by one lane (DPP): rot_next gives lane l the value of lane l+1, rot_prev
 // the value of lane l-1 (both wrap around the 64 lanes).
+// Every lane is written (a full-wave rotation with all rows and banks enabled), so no old
+// value is needed.
 __device__ __forceinline__ float rot_next(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
 }
 __device__ __forceinline__ float rot_prev(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x13C, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x13C, 0xF, 0xF, false));
 }
 
 // Line-mode group record, read with scalar loads by the whole wave (constant indices only,
@@ -266,25 +268,30 @@ __device__ __forceinline__ void exch_row_line(const Exch<float>& e, uint32_t i, 
     if (ngrp == 0) return;
     const SpinBuf sb = spin_buf(S, e.n);
     // Two line buffers used in turn (no register copies, so no early waits on the loads in
-    // flight); past the last group the record's next-line words name the slice's own line.
+    // flight).  Pairs of groups that both have a successor run in the loop; the last one or
+    // two groups follow it, and the last group issues no look-ahead loads.
     u32x3 xa[G], xb[G];
     {
         const RW r0 = load_line_rec<NC>(rp);
         line_loads<NC, G3>(sb, lane, base, r0.w, xa);
     }
-    for (uint32_t r = 0;; r += 2) {
-        {
-            const RW cur = load_line_rec<NC>(rp + (size_t)r * W);
-            line_loads<NC, G3>(sb, lane, base, cur.w + NX, xb);
-            line_group_sum<NC>(cur, xa, dot);
-        }
-        if (r + 1 >= ngrp) break;
-        {
-            const RW cur = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
-            line_loads<NC, G3>(sb, lane, base, cur.w + NX, xa);
-            line_group_sum<NC>(cur, xb, dot);
-        }
-        if (r + 2 >= ngrp) break;
+    uint32_t r = 0;
+    for (; r + 2 < ngrp; r += 2) {
+        const RW c0 = load_line_rec<NC>(rp + (size_t)r * W);
+        line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
+        line_group_sum<NC>(c0, xa, dot);
+        const RW c1 = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
+        line_loads<NC, G3>(sb, lane, base, c1.w + NX, xa);
+        line_group_sum<NC>(c1, xb, dot);
+    }
+    const RW c0 = load_line_rec<NC>(rp + (size_t)r * W);
+    if (r + 1 < ngrp) {
+        line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
+        line_group_sum<NC>(c0, xa, dot);
+        const RW c1 = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
+        line_group_sum<NC>(c1, xb, dot);
+    } else {
+        line_group_sum<NC>(c0, xa, dot);
     }
 }
 
