@@ -114,9 +114,11 @@ int upload_tables(llg_batch* b) {
 }
 
 int launch_chunk(llg_batch* b, uint32_t nsteps, bool dipolar) {
-    // the batch kernels write S without the members' AoS3 gather copies
     for (auto& m : b->m)
-        if (m.c) m.c->g3_valid = false;
+        if (m.c) {
+            m.c->g3_valid = false;    // the batch kernels write S and Y without AoS3 copies
+            m.c->y_soa_valid = true;
+        }
     for (uint32_t k = 0; k < nsteps; ++k) {
         if (dipolar)
             for (auto& mm : b->m)

@@ -81,6 +81,7 @@ int upload_state(llg_ctx* c, DevBuf& dst, const double* x, const double* y, cons
     const uint32_t n = c->n;
     const double* src[3] = {x, y, z};
     if (&dst == &c->S) c->g3_valid = false;  // the AoS3 gather copy no longer matches
+    if (&dst == &c->Y) c->y_soa_valid = true;
     if (c->prec == LLG_FP64_PLAIN) {
         for (int a = 0; a < 3; ++a)
             SLLG_HIP(hipMemcpyAsync(dst.as<double>() + (size_t)a * n, src[a], n * 8, hipMemcpyHostToDevice, c->stream));
@@ -214,7 +215,10 @@ int step_fused(llg_ctx* c, uint64_t t) {
 }
 
 int do_steps(llg_ctx* c, uint64_t t0, uint32_t nsteps) {
-    if (c->scheme != LLG_HEUN) c->g3_valid = false;  // RK stages write S without its gather copy
+    if (c->scheme != LLG_HEUN) {
+        c->g3_valid = false;  // RK stages write S without its gather copy
+        c->y_soa_valid = true;
+    }
     for (uint32_t k = 0; k < nsteps; ++k) {
         if (c->prec == LLG_FP64_PLAIN)
             SLLG_TRY(step_plain(c, t0 + k));
@@ -569,7 +573,10 @@ int llg_get_det_field(llg_ctx* c, double* hx, double* hy, double* hz) {
     if (!c || !hx || !hy || !hz) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
     SLLG_HIP(hipSetDevice(c->device));
     if (c->det_valid && c->prec == LLG_FP64_PLAIN) return download_state(c, c->DET.p, true, hx, hy, hz);
-    if (c->det_valid) return field_at(c, c->Y.p, hx, hy, hz);
+    if (c->det_valid) {
+        SLLG_TRY(fused_y_soa(c));
+        return field_at(c, c->Y.p, hx, hy, hz);
+    }
     return field_at(c, c->S.p, hx, hy, hz);
 }
 

@@ -185,7 +185,8 @@ struct llg_ctx {
     int tune_unr = 2;    // packed-exchange chunks loaded ahead per wave (llg_autotune)
     int tune_thc = 1;    // predictor hands its thermal field to the corrector (llg_autotune)
     int tune_g3 = 1;     // gather neighbours from the AoS3 copy of the spins (llg_autotune)
-    bool g3_valid = false;  // S3 holds the current S
+    bool g3_valid = false;   // S3 holds the current S
+    bool y_soa_valid = true;  // Y (SoA) holds the last predictor output (else only Y3 does)
 
     // device state
     sllg::DevBuf S, Y, A, H, FN, DET, XI, STG, STG0, DEM;  // SoA 3n (type per path)

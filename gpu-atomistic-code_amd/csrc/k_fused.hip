@@ -59,6 +59,10 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
 // Spins as one raw buffer resource: a neighbour's three components are three buffer loads
 // sharing one 32-bit byte offset (scalar component offsets 4n, 8n), so no 64-bit address
 // arithmetic per entry; a load outside the 12n bytes returns 0 instead of faulting.
+struct Spin3 {  // one spin of an AoS3 copy
+    float x, y, z;
+};
+
 struct SpinBuf {
     __amdgpu_buffer_rsrc_t r;
     uint32_t o1, o2;
@@ -383,14 +387,22 @@ __device__ __forceinline__ bool row_live(const FusedArgs& a, uint32_t i) {
 }
 
 // src: the spins (SoA) the field is evaluated at; gsrc: where neighbours are gathered from
-// (src itself, or with G3 its AoS3 copy).
+// (src itself, or with G3 its AoS3 copy, which then also gives the row's own spin; src is
+// then read only by the four-spin term).
 template <int FMT, int NC, int UNR = 2, bool G3 = false>
 __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32_t iref, const float* __restrict__ src,
                                           const float* __restrict__ gsrc, const float* sdict, float s[3], float f[3]) {
     const uint32_t n = a.n;
-    s[0] = src[i];
-    s[1] = src[n + i];
-    s[2] = src[2 * (size_t)n + i];
+    if constexpr (G3) {
+        const Spin3 v = reinterpret_cast<const Spin3*>(gsrc)[i];
+        s[0] = v.x;
+        s[1] = v.y;
+        s[2] = v.z;
+    } else {
+        s[0] = src[i];
+        s[1] = src[n + i];
+        s[2] = src[2 * (size_t)n + i];
+    }
     float h[3];
     if constexpr (FMT == FMT_SELL_PACK)
         exch_row_pack_fast<NC, UNR, G3>(a.ex, i, gsrc, sdict, h);
@@ -460,11 +472,10 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
         __syncthreads();                                                          \
     }
 
-// With G3 the kernels gather neighbours from the AoS3 copy (S3 / Y3, [3 i + c]) and keep it
-// current: the predictor writes Y3 beside Y, the corrector S3 beside S.
-struct Spin3 {
-    float x, y, z;
-};
+// With G3 the kernels gather neighbours (and read the row's own spin) from the AoS3 copy
+// (S3 / Y3, [3 i + c]) and keep it current: the predictor writes Y3 (and Y only when Y is
+// non-null: the four-spin term or a dipolar refresh at the predicted spins reads it), the
+// corrector S3 beside S.
 
 template <int FMT, int NC, int TPB, int UNR, bool G3>
 __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S,
@@ -482,9 +493,10 @@ __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __r
     for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
     const float r = rsqrtf(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        Y[(size_t)c * n + i] = y[c] * r;
-        A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
+    for (int c = 0; c < 3; ++c) A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
+    if (!G3 || Y) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Y[(size_t)c * n + i] = y[c] * r;
     }
     if constexpr (G3) reinterpret_cast<Spin3*>(Y3)[i] = Spin3{y[0] * r, y[1] * r, y[2] * r};
 }
@@ -498,11 +510,14 @@ __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __r
     row_of_thread(a, TPB, i, iref);
     if (!row_live<FMT>(a, i)) return;
     const uint32_t n = a.n;
+    float av[3];  // loaded before the field so its latency overlaps the neighbour gathers
+#pragma unroll
+    for (int c = 0; c < 3; ++c) av[c] = A[(size_t)c * n + i];
     float s[3], f[3];
     fused_rhs<FMT, NC, UNR, G3>(a, i, iref, Y, G3 ? Y3 : Y, sdict, s, f);
     float p[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) p[c] = A[(size_t)c * n + i] + a.half_rdt * f[c];
+    for (int c = 0; c < 3; ++c) p[c] = av[c] + a.half_rdt * f[c];
     const float r = rsqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) S[(size_t)c * n + i] = p[c] * r;
@@ -514,6 +529,16 @@ __global__ __launch_bounds__(kTPB) void k_to_aos3(uint32_t n, const float* __res
     const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
     if (i >= n) return;
     reinterpret_cast<Spin3*>(S3)[i] = Spin3{S[i], S[n + i], S[2 * (size_t)n + i]};
+}
+
+// AoS3 -> SoA (the predictor output when only its AoS3 copy was written).
+__global__ __launch_bounds__(kTPB) void k_from_aos3(uint32_t n, const float* __restrict__ S3, float* __restrict__ S) {
+    const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
+    if (i >= n) return;
+    const Spin3 v = reinterpret_cast<const Spin3*>(S3)[i];
+    S[i] = v.x;
+    S[n + i] = v.y;
+    S[2 * (size_t)n + i] = v.z;
 }
 
 // Batched Heun: one launch covers the tiles of every member of a batch (independent spin
@@ -819,6 +844,14 @@ static int heun_bufs(llg_ctx* c, bool g3, HeunBufs& b) {
     return 0;
 }
 
+int fused_y_soa(llg_ctx* c) {
+    if (c->y_soa_valid) return 0;
+    k_from_aos3<<<blocks_for(c->n, kTPB), kTPB, 0, c->stream>>>(c->n, c->Y3.as<float>(), c->Y.as<float>());
+    SLLG_HIP(hipGetLastError());
+    c->y_soa_valid = true;
+    return 0;
+}
+
 int fused_heun_pred(llg_ctx* c, const StepSched& s) {
     FusedArgs a = fused_args(c, s);
     const bool g3 = use_g3(c);
@@ -833,6 +866,11 @@ int fused_heun_pred(llg_ctx* c, const StepSched& s) {
         if (c->HT.bytes < 3 * (size_t)c->n * sizeof(float)) SLLG_TRY(c->HT.alloc(3 * (size_t)c->n * sizeof(float)));
         a.th_out = c->HT.as<float>();
     }
+    // With the AoS3 copy the corrector reads the predicted spins from Y3; the SoA Y is
+    // written only for the readers that need it (four-spin term, dipolar refresh at Y).
+    const bool ysoa = !g3 || c->ex32.fs_xadj || (c->dip && c->dip_every_stage);
+    if (!ysoa) b.Y = nullptr;
+    c->y_soa_valid = ysoa;
     const int tpb = c->tune_tpb, unr = c->tune_unr;
 #define SLLG_L(F, N)                                                                          \
     do {                                                                                      \

@@ -425,7 +425,8 @@ def test_line_mode(gpu, case):
 
 def test_line_mode_gather_copy(gpu):
     """Line mode with and without the AoS3 gather copy gives identical bits, also across a
-    set_spins between steps (the copy is rebuilt from the new state)."""
+    set_spins between steps (the copy is rebuilt from the new state); the deterministic field
+    at the predicted spins (kept only as the AoS3 copy with the gather copy on) is identical."""
     sysd, s0 = W.fcc_3shell(precision="fp32", dims=(2, 2, 64))
     res = []
     for g3 in (0, 1):
@@ -438,5 +439,6 @@ def test_line_mode_gather_copy(gpu):
         mid = g.get_spins()
         g.set_spins(mid[:, ::-1].copy())  # new state from the host: gather copy must follow
         g.step(20, 20)
-        res.append(g.get_spins())
-    assert np.array_equal(res[0], res[1])
+        res.append((g.get_spins(), g.det_field()))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
