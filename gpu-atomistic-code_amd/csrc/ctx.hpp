@@ -29,7 +29,7 @@ constexpr int kDictLds = 2048;  // floats of coupling dictionary staged in LDS (
 // Line-mode group records: line_group(nc) neighbour lines per record, line_words(nc) words
 // (this group's line words, 3 shifts x nc couplings per line, the next group's line words
 // last; padded to whole 16-byte units).
-constexpr int line_group(int nc) { return nc == 1 ? 4 : (nc == 3 ? 2 : 1); }
+constexpr int line_group(int nc) { return nc == 1 ? 3 : (nc == 3 ? 2 : 1); }
 constexpr int line_words(int nc) { return (2 * line_group(nc) + 3 * line_group(nc) * nc + 3) / 4 * 4; }
 
 enum ExchFmt : int {

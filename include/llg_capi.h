@@ -300,16 +300,16 @@ int llg_profile_kernels(llg_ctx* ctx, uint64_t t0, uint32_t nsteps, int max_kind
  * every candidate block size (256, 512, 1024 threads) with and without the XCD-aware
  * tile remap, for the packed encoding 2 or 4 coupling chunks loaded ahead per wave, and
  * with the thermal field drawn in both passes or handed from predictor to corrector, and
- * with neighbours gathered from the SoA spins or from an AoS3 copy the kernels keep beside
- * them (bit-identical either way), on this handle's own couplings, for nsteps steps each;
- * keeps the fastest.
+ * (packed and line encodings) with neighbours gathered from the SoA spins or from an AoS3
+ * copy the kernels keep beside them (bit-identical either way), on this handle's own
+ * couplings, for nsteps steps each; keeps the fastest.
  * The spin state is restored afterwards.  Results are cached per (device, encoding, n,
  * entries) in the process and, if the environment variable SLLG_TUNE_FILE names a file,
  * across processes.  Other paths return LLG_OK with the defaults. */
 typedef struct llg_tune_result {
     int tpb, xcd;
     double ms_per_step;     /* chosen */
-    double ms_per_step_default; /* 256 threads, remap on */
+    double ms_per_step_default; /* 256 threads, remap on, 2 chunks ahead, hand-off on, gather copy on */
     int candidates, from_cache;
     int unroll;
     int thermal_cache;  /* 1: the predictor hands its thermal field to the corrector */
