@@ -386,12 +386,13 @@ __device__ __forceinline__ bool row_live(const FusedArgs& a, uint32_t i) {
         return i < a.n;
 }
 
-// src: the spins (SoA) the field is evaluated at; gsrc: where neighbours are gathered from
+// t: the step the thermal field is drawn for.  src: the spins (SoA) the field is evaluated at; gsrc: where neighbours are gathered from
 // (src itself, or with G3 its AoS3 copy, which then also gives the row's own spin; src is
 // then read only by the four-spin term).
 template <int FMT, int NC, int UNR = 2, bool G3 = false>
 __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32_t iref, const float* __restrict__ src,
-                                          const float* __restrict__ gsrc, const float* sdict, float s[3], float f[3]) {
+                                          const float* __restrict__ gsrc, const float* sdict, float s[3], float f[3],
+                                          uint64_t t) {
     const uint32_t n = a.n;
     if constexpr (G3) {
         const Spin3 v = reinterpret_cast<const Spin3*>(gsrc)[i];
@@ -436,7 +437,7 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
             th[2] = a.th_in[2 * (size_t)n + i];
         } else {
             float z[3];
-            normals3_f32(a.seed, a.stream, iref, a.t, z);
+            normals3_f32(a.seed, a.stream, iref, t, z);
             const float TP = a.sqrtT * sp.sigma;
             th[0] = z[0] * TP;
             th[1] = z[1] * TP;
@@ -487,7 +488,7 @@ __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __r
     if (!row_live<FMT>(a, i)) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC, UNR, G3>(a, i, iref, S, G3 ? S3 : S, sdict, s, f);
+    fused_rhs<FMT, NC, UNR, G3>(a, i, iref, S, G3 ? S3 : S, sdict, s, f, a.t);
     float y[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -514,7 +515,7 @@ __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __r
 #pragma unroll
     for (int c = 0; c < 3; ++c) av[c] = A[(size_t)c * n + i];
     float s[3], f[3];
-    fused_rhs<FMT, NC, UNR, G3>(a, i, iref, Y, G3 ? Y3 : Y, sdict, s, f);
+    fused_rhs<FMT, NC, UNR, G3>(a, i, iref, Y, G3 ? Y3 : Y, sdict, s, f, a.t);
     float p[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = av[c] + a.half_rdt * f[c];
@@ -561,15 +562,15 @@ __global__ __launch_bounds__(kTPB) void k_heun_batch(const BatchMember* __restri
     const uint32_t tile = tile_index(xcd);
     const uint32_t m = __builtin_amdgcn_readfirstlane(tile_member[tile]);
     const BatchMember& bm = mem[m];
-    FusedArgs a = bm.a;
-    a.t = bm.t0 + step_off;
+    const FusedArgs& a = bm.a;  // read in place (uniform scalar loads); a local copy lands in scratch
+    const uint64_t t = bm.t0 + step_off;
     SLLG_STAGE_DICT(FMT, NC, a)
     const uint32_t i = (tile - bm.tile0) * kTPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
     if constexpr (PHASE == 0) {
-        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.S, bm.S, sdict, s, f);
+        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.S, bm.S, sdict, s, f, t);
         float y[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -580,7 +581,7 @@ __global__ __launch_bounds__(kTPB) void k_heun_batch(const BatchMember* __restri
             bm.A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
         }
     } else {
-        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.Y, bm.Y, sdict, s, f);
+        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.Y, bm.Y, sdict, s, f, t);
         float p[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[c] = bm.A[(size_t)c * n + i] + a.half_rdt * f[c];
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(kTPB) void k_rk4(FusedArgs a, const float* __restri
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), M == 0 ? S : Yin, M == 0 ? S : Yin, sdict, s, f);
+    fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), M == 0 ? S : Yin, M == 0 ? S : Yin, sdict, s, f, a.t);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const size_t o = (size_t)c * n + i;
@@ -643,7 +644,7 @@ __global__ __launch_bounds__(kTPB) void k_rk5(FusedArgs a, const float* __restri
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), M == 0 ? S : Yin, M == 0 ? S : Yin, sdict, s, f);
+    fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), M == 0 ? S : Yin, M == 0 ? S : Yin, sdict, s, f, a.t);
     float* Ks[5] = {K0, K1, K2, K3, K4};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
