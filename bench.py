@@ -197,6 +197,7 @@ def accuracy(args):
             g.step(t, c - t)
             t = c
             out[c] = (g.get_spins(), g.reduce(energy=False)[0][0])
+            print(f"accuracy: {prec} step {c}", file=sys.stderr, flush=True)
         g.close()
         return out
 
@@ -287,6 +288,7 @@ def batched(args):
     setup = time.perf_counter() - t
     B.step(10)
     B.reduce(ids)
+    tune = B.autotune(10) if not args.no_tune else None
     steps = args.steps
     t = time.perf_counter()
     done = 0
@@ -300,11 +302,13 @@ def batched(args):
     print(json.dumps({"mode": "batched", "replicas": R, "spins_total": info["spins"], "steps": steps,
                       "spin_updates_per_s": info["spins"] * steps / sec, "replica_steps_per_s": R * steps / sec,
                       "ms_per_step": sec * 1e3 / steps, "setup_s": setup, "device_bytes": info["device_bytes"],
+                      "launch_tune": tune,
                       "m_z_lowest_T": float(m[0][2]), "m_z_highest_T": float(m[-1][2])}), flush=True)
 
 
 def serve_mode(args):
-    """Poisson stream of simple-cubic M(T) and field-ramp requests, L log-uniform in
+    """Poisson stream of simple-cubic M(T), field-ramp and spin-wave requests (one third
+    each; spin-wave requests sample S(k) at (q, 0, 0) every `update` steps), L log-uniform in
     [10, 100], T uniform in [0, 1.5 Tc], steps uniform in [smin, smax]; batched FP32 loop vs
     sequential FP64 plain solves, p50/p99 latency."""
     import sllg
@@ -317,7 +321,7 @@ def serve_mode(args):
     for k in range(args.requests):
         t += rng.exponential(1.0 / args.rate)
         L = int(round(10 ** rng.uniform(np.log10(args.lmin), np.log10(args.lmax))))
-        typ = SB.REQ_RAMP if rng.random() < 0.5 else SB.REQ_MT
+        typ = (SB.REQ_MT, SB.REQ_RAMP, SB.REQ_SW)[int(rng.integers(3))]
         specs.append(dict(type=typ, L=L, steps=int(rng.integers(args.smin, args.smax + 1)) // 100 * 100,
                           update=100, T=float(rng.uniform(0, 1.5 * Tc)), arrival_s=t, seed=1000 + k,
                           B0=(0, 0, -0.5) if typ == SB.REQ_RAMP else (0, 0, 0), B1=(0, 0, 0.5) if typ == SB.REQ_RAMP else (0, 0, 0)))
@@ -356,7 +360,7 @@ def main():
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--T", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="headline: skip the launch-shape autotuner")
+    ap.add_argument("--no-tune", action="store_true", help="headline, batched: skip the launch-shape autotuner")
     ap.add_argument("--no-unshared", action="store_true",
                     help="headline: skip the comparison run with repeated slices stored separately")
     ap.add_argument("--cpu-budget", type=float, default=15.0)

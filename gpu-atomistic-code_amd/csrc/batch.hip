@@ -1,7 +1,8 @@
 // batch.hip - batches of independent spin systems and the continuous-batching request
 // loop (include/llg_batch.h).  No reference counterpart: the reference runs one system
-// per process; its request types (M(T), field ramp) are reproduced by llg_serve_run with
-// the sampling semantics of src/mvt.cpp and src/ramp_field.cpp.
+// per process; its request types (M(T), field ramp, spin-wave time series) are reproduced by
+// llg_serve_run with the sampling semantics of src/mvt.cpp, src/ramp_field.cpp and
+// src/timeseries.cpp.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -47,6 +48,8 @@ struct llg_batch {
     hipGraph_t graph = nullptr;
     uint32_t gsteps = 0;
     uint32_t graphs_built = 0;
+    uint32_t tpb = 256;  // threads per tile (one block per tile)
+    int xcd = 1;         // XCD-aware block order
     DevBuf red;
     std::vector<std::shared_ptr<ExchStore>> pinned;  // coupling stores kept for future members
 };
@@ -108,7 +111,7 @@ int upload_tables(llg_batch* b) {
             cs.push_back(b->m[k].c);
             t0.push_back(b->m[k].t);
         }
-        SLLG_TRY(batch_build_table(cs, t0, g->dmem, g->dmap, g->ntiles, b->stream));
+        SLLG_TRY(batch_build_table(cs, t0, b->tpb, g->dmem, g->dmap, g->ntiles, b->stream));
     }
     return 0;
 }
@@ -125,13 +128,13 @@ int launch_chunk(llg_batch* b, uint32_t nsteps, bool dipolar) {
                 if (mm.live && mm.c->dip && (mm.t + k) % mm.c->dip_refresh == 0)
                     SLLG_TRY(dipolar_refresh(mm.c, mm.c->S.p, mm.c->DEM.p));
         for (auto& g : b->groups)
-            SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 0, 1));
+            SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 0, b->tpb, b->xcd));
         if (dipolar)
             for (auto& mm : b->m)
                 if (mm.live && mm.c->dip && mm.c->dip_every_stage)
                     SLLG_TRY(dipolar_refresh(mm.c, mm.c->Y.p, mm.c->DEM.p));
         for (auto& g : b->groups)
-            SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 1, 1));
+            SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 1, b->tpb, b->xcd));
     }
     return 0;
 }
@@ -336,6 +339,82 @@ int llg_batch_step(llg_batch* b, uint32_t nsteps) {
     return 0;
 }
 
+int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd) {
+    if (!b) SLLG_FAIL(LLG_E_INVALID, "NULL batch");
+    if (tpb != 128 && tpb != 256 && tpb != 512) SLLG_FAIL(LLG_E_INVALID, "tile of %u threads (128, 256, 512)", tpb);
+    if (tpb != b->tpb || (xcd ? 1 : 0) != b->xcd) drop_graph(b);  // the captured launches have the old shape
+    b->tpb = tpb;
+    b->xcd = xcd ? 1 : 0;
+    return 0;
+}
+
+int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
+    if (!b || !out || nsteps == 0) SLLG_FAIL(LLG_E_INVALID, "NULL argument or nsteps = 0");
+    std::memset(out, 0, sizeof(*out));
+    SLLG_HIP(hipSetDevice(b->device));
+    // save every live member's spins and step counter; the timed steps are undone afterwards
+    std::vector<DevBuf> save(b->m.size());
+    std::vector<uint64_t> t_save(b->m.size());
+    for (size_t k = 0; k < b->m.size(); ++k) {
+        if (!b->m[k].live) continue;
+        llg_ctx* c = b->m[k].c;
+        SLLG_TRY(save[k].alloc(c->S.bytes));
+        SLLG_HIP(hipMemcpyAsync(save[k].p, c->S.p, c->S.bytes, hipMemcpyDeviceToDevice, b->stream));
+        t_save[k] = b->m[k].t;
+    }
+    const uint32_t tpb0 = b->tpb;
+    const int xcd0 = b->xcd;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    SLLG_HIP(hipEventCreate(&e0));
+    SLLG_HIP(hipEventCreate(&e1));
+    const uint32_t tpbs[3] = {128, 256, 512};
+    double best = 1e300;
+    uint32_t btpb = 256;
+    int bxcd = 1, rc = 0;
+    for (int ti = 0; ti < 3 && !rc; ++ti)
+        for (int xcd = 1; xcd >= 0 && !rc; --xcd) {
+            rc = llg_batch_set_launch(b, tpbs[ti], xcd);
+            if (!rc) rc = llg_batch_step(b, 2);  // warm-up (and graph capture when enabled)
+            if (!rc) rc = llg_batch_step(b, nsteps);  // a graph of this length is captured here
+            if (rc) break;
+            if (hipEventRecord(e0, b->stream) != hipSuccess) rc = LLG_E_DEVICE;
+            if (!rc) rc = llg_batch_step(b, nsteps);
+            if (!rc && hipEventRecord(e1, b->stream) != hipSuccess) rc = LLG_E_DEVICE;
+            if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = LLG_E_DEVICE;
+            float ms = 0.f;
+            if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = LLG_E_DEVICE;
+            if (rc) break;
+            const double per = (double)ms / nsteps;
+            if (tpbs[ti] == 256 && xcd == 1) out->ms_per_step_default = per;
+            out->candidates++;
+            if (per < best) {
+                best = per;
+                btpb = tpbs[ti];
+                bxcd = xcd;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // restore the members' spins and step counters
+    for (size_t k = 0; k < b->m.size(); ++k) {
+        if (!b->m[k].live) continue;
+        llg_ctx* c = b->m[k].c;
+        const hipError_t e = hipMemcpyAsync(c->S.p, save[k].p, c->S.bytes, hipMemcpyDeviceToDevice, b->stream);
+        if (e != hipSuccess && !rc) rc = LLG_E_DEVICE;
+        b->m[k].t = t_save[k];
+    }
+    if (hipStreamSynchronize(b->stream) != hipSuccess && !rc) rc = LLG_E_DEVICE;
+    if (rc) {
+        (void)llg_batch_set_launch(b, tpb0, xcd0);
+        return rc;
+    }
+    SLLG_TRY(llg_batch_set_launch(b, btpb, bxcd));
+    out->tpb = (int)btpb;
+    out->xcd = bxcd;
+    out->ms_per_step = best;
+    return 0;
+}
+
 int llg_batch_reduce(llg_batch* b, const int* members, uint32_t count, uint32_t stride, double* m) {
     if (!b || (count && (!members || !m))) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
     if (count == 0) return 0;
@@ -454,11 +533,42 @@ void record_sample(const llg_request& r, Live& lv, const double m[3], llg_result
 }  // namespace
 
 extern "C" int llg_serve_run(const llg_serve_config* cfg, const llg_request* reqs, uint32_t nreq, llg_result* results,
-                             double* m_series, const uint64_t* series_offset, llg_serve_stats* stats) {
+                             double* m_series, const uint64_t* series_offset, double* sq_series,
+                             const uint64_t* sq_offset, llg_serve_stats* stats) {
     if (!cfg || (nreq && (!reqs || !results)) || !stats) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
-    for (uint32_t i = 0; i < nreq; ++i)
+    for (uint32_t i = 0; i < nreq; ++i) {
         if (reqs[i].L == 0 || reqs[i].update == 0 || reqs[i].steps == 0 || !(reqs[i].dt > 0) || !(reqs[i].mu > 0))
             SLLG_FAIL(LLG_E_INVALID, "request %u: L, steps, update, dt and mu must be positive", i);
+        if (reqs[i].type > LLG_REQ_SW) SLLG_FAIL(LLG_E_INVALID, "request %u: unknown type %u", i, reqs[i].type);
+    }
+    if (sq_series && !sq_offset) SLLG_FAIL(LLG_E_INVALID, "sq_series needs sq_offset");
+    // spin-wave requests: S(k) at (q, 0, 0), q = 0 .. L/2, on the L^3 mesh (spin (x L + y) L + z)
+    std::map<uint32_t, std::vector<uint32_t>> meshes;
+    auto sw_setup = [&](const llg_request& r, llg_ctx* c) -> int {
+        if (r.type != LLG_REQ_SW) return 0;
+        std::vector<uint32_t>& mesh = meshes[r.L];
+        const uint64_t n = (uint64_t)r.L * r.L * r.L;
+        if (mesh.empty()) {
+            mesh.resize(3 * n);
+            for (uint64_t i = 0; i < n; ++i) {
+                mesh[3 * i] = (uint32_t)(i / ((uint64_t)r.L * r.L));
+                mesh[3 * i + 1] = (uint32_t)((i / r.L) % r.L);
+                mesh[3 * i + 2] = (uint32_t)(i % r.L);
+            }
+        }
+        const uint32_t nk = r.L / 2 + 1, grid[3] = {r.L, r.L, r.L};
+        std::vector<uint32_t> kp(3 * nk, 0u);
+        for (uint32_t q = 0; q < nk; ++q) kp[3 * q] = q;
+        return llg_set_sq(c, grid, mesh.data(), nk, kp.data(), 2, nullptr, nullptr, 1, nullptr);
+    };
+    auto sw_sample = [&](const llg_request& r, uint32_t i, uint32_t sample, llg_ctx* c) -> int {
+        if (r.type != LLG_REQ_SW) return 0;
+        const uint32_t nk = r.L / 2 + 1;
+        std::vector<double> v(2 * nk);
+        SLLG_TRY(llg_sample_sq(c, v.data()));
+        if (sq_series) std::memcpy(sq_series + sq_offset[i] + (uint64_t)sample * 2 * nk, v.data(), v.size() * 8);
+        return 0;
+    };
     std::memset(stats, 0, sizeof(*stats));
     std::memset(results, 0, sizeof(llg_result) * nreq);
     std::vector<uint32_t> order(nreq);
@@ -527,14 +637,16 @@ extern "C" int llg_serve_run(const llg_serve_config* cfg, const llg_request* req
             std::vector<double> zx(d.n, 0.0), zz(d.n, 1.0);
             int rc = llg_set_spins(c, zx.data(), zx.data(), zz.data());
             if (!rc) rc = llg_set_temperature(c, r.T);
+            if (!rc) rc = sw_setup(r, c);
             Live lv;
             lv.req = (int)i;
             for (uint32_t s = 0; !rc && s < r.steps; s += r.update) {
                 double B[3];
                 ramp_field(r, s, B);
-                if (r.type == LLG_REQ_MT) std::memcpy(B, r.B0, sizeof B);
+                if (r.type != LLG_REQ_RAMP) std::memcpy(B, r.B0, sizeof B);
                 double m[3];
                 rc = llg_reduce(c, m, nullptr);
+                if (!rc) rc = sw_sample(r, i, lv.samples, c);
                 if (rc) break;
                 record_sample(r, lv, m, res, m_series, series_offset);
                 rc = llg_set_applied(c, B);
@@ -596,6 +708,7 @@ extern "C" int llg_serve_run(const llg_serve_config* cfg, const llg_request* req
                     break;                        // wait for memory to free up
                 }
                 if (rc) return rc;
+                SLLG_TRY(sw_setup(r, b->m[member].c));
                 Live lv;
                 lv.req = (int)i;
                 lv.member = member;
@@ -622,6 +735,8 @@ extern "C" int llg_serve_run(const llg_serve_config* cfg, const llg_request* req
                 size_t k = 0;
                 for (auto& lv : live)
                     if (lv.done % reqs[lv.req].update == 0) {
+                        // the batch's launches are complete (the reduction synchronised its stream)
+                        SLLG_TRY(sw_sample(reqs[lv.req], (uint32_t)lv.req, lv.samples, b->m[lv.member].c));
                         record_sample(reqs[lv.req], lv, &m[3 * k], results[lv.req], m_series, series_offset);
                         ++k;
                         if (reqs[lv.req].type == LLG_REQ_RAMP) {

@@ -555,17 +555,17 @@ struct BatchMember {
     uint64_t t0;  // step of the first step of the current launch chunk
 };
 
-template <int PHASE, int FMT, int NC>
-__global__ __launch_bounds__(kTPB) void k_heun_batch(const BatchMember* __restrict__ mem,
-                                                     const uint32_t* __restrict__ tile_member, uint32_t step_off,
-                                                     int xcd) {
+template <int PHASE, int FMT, int NC, int TPB>
+__global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restrict__ mem,
+                                                    const uint32_t* __restrict__ tile_member, uint32_t step_off,
+                                                    int xcd) {
     const uint32_t tile = tile_index(xcd);
     const uint32_t m = __builtin_amdgcn_readfirstlane(tile_member[tile]);
     const BatchMember& bm = mem[m];
     const FusedArgs& a = bm.a;  // read in place (uniform scalar loads); a local copy lands in scratch
     const uint64_t t = bm.t0 + step_off;
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = (tile - bm.tile0) * kTPB + threadIdx.x;
+    const uint32_t i = (tile - bm.tile0) * TPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
@@ -987,8 +987,8 @@ int fused_detfield(llg_ctx* c, const float* Sp, double* out) {
 
 size_t batch_member_size() { return sizeof(BatchMember); }
 
-int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<uint64_t>& t0, DevBuf& dmem,
-                      DevBuf& dmap, uint32_t& ntiles, hipStream_t s) {
+int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<uint64_t>& t0, uint32_t tpb,
+                      DevBuf& dmem, DevBuf& dmap, uint32_t& ntiles, hipStream_t s) {
     std::vector<BatchMember> hm(members.size());
     std::vector<uint32_t> map;
     uint32_t tile = 0;
@@ -1000,7 +1000,7 @@ int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<ui
         hm[k].Y = c->Y.as<float>();
         hm[k].A = c->A.as<float>();
         hm[k].tile0 = tile;
-        hm[k].ntiles = blocks_for(c->n, kTPB);
+        hm[k].ntiles = blocks_for(c->n, tpb);
         hm[k].t0 = t0[k];
         map.insert(map.end(), hm[k].ntiles, (uint32_t)k);
         tile += hm[k].ntiles;
@@ -1015,15 +1015,24 @@ int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<ui
     return 0;
 }
 
+template <int PHASE, int F, int N>
+static void launch_batch_tpb(hipStream_t s, const BatchMember* m, const uint32_t* map, uint32_t ntiles,
+                             uint32_t step_off, uint32_t tpb, int xcd) {
+    if (tpb == 128) k_heun_batch<PHASE, F, N, 128><<<ntiles, 128, 0, s>>>(m, map, step_off, xcd);
+    else if (tpb == 512) k_heun_batch<PHASE, F, N, 512><<<ntiles, 512, 0, s>>>(m, map, step_off, xcd);
+    else k_heun_batch<PHASE, F, N, 256><<<ntiles, 256, 0, s>>>(m, map, step_off, xcd);
+}
+
 int batch_heun_launch(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,
-                      uint32_t step_off, int phase, int xcd) {
+                      uint32_t step_off, int phase, uint32_t tpb, int xcd) {
     if (ntiles == 0) return 0;
+    if (tpb != 128 && tpb != 256 && tpb != 512) SLLG_FAIL(LLG_E_INVALID, "batch tile of %u threads", tpb);
     const BatchMember* m = static_cast<const BatchMember*>(dmem);
     const uint32_t* map = static_cast<const uint32_t*>(dmap);
-#define SLLG_L(F, N)                                                                          \
-    do {                                                                                      \
-        if (phase == 0) k_heun_batch<0, F, N><<<ntiles, kTPB, 0, s>>>(m, map, step_off, xcd); \
-        else k_heun_batch<1, F, N><<<ntiles, kTPB, 0, s>>>(m, map, step_off, xcd);            \
+#define SLLG_L(F, N)                                                                   \
+    do {                                                                               \
+        if (phase == 0) launch_batch_tpb<0, F, N>(s, m, map, ntiles, step_off, tpb, xcd); \
+        else launch_batch_tpb<1, F, N>(s, m, map, ntiles, step_off, tpb, xcd);            \
     } while (0)
     SLLG_DISPATCH_FMT_NC(fmt, nc, SLLG_L);
 #undef SLLG_L

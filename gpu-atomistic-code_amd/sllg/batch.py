@@ -9,7 +9,7 @@ import numpy as np
 from . import Desc, LLGError, _check, _p, lib
 
 BATCH_GRAPH = 0x1
-REQ_MT, REQ_RAMP = 0, 1
+REQ_MT, REQ_RAMP, REQ_SW = 0, 1, 2
 
 
 class BatchInfo(C.Structure):
@@ -41,6 +41,11 @@ class ServeStats(C.Structure):
                 ("completed", C.c_uint32), ("max_live", C.c_uint32), ("spin_updates", C.c_uint64)]
 
 
+class BatchTune(C.Structure):
+    _fields_ = [("tpb", C.c_int), ("xcd", C.c_int), ("ms_per_step", C.c_double),
+                ("ms_per_step_default", C.c_double), ("candidates", C.c_int), ("pad_", C.c_int)]
+
+
 _bound = False
 
 
@@ -65,8 +70,10 @@ def _bind():
     L.llg_batch_get_spins.argtypes = [P, C.c_int, dp, dp, dp]
     L.llg_batch_get_step.argtypes = [P, C.c_int, C.POINTER(C.c_uint64)]
     L.llg_batch_get_info.argtypes = [P, C.POINTER(BatchInfo)]
+    L.llg_batch_set_launch.argtypes = [P, C.c_uint32, C.c_int]
+    L.llg_batch_autotune.argtypes = [P, C.c_uint32, C.POINTER(BatchTune)]
     L.llg_serve_run.argtypes = [C.POINTER(ServeConfig), C.POINTER(Request), C.c_uint32, C.POINTER(Result), dp,
-                                C.POINTER(C.c_uint64), C.POINTER(ServeStats)]
+                                C.POINTER(C.c_uint64), dp, C.POINTER(C.c_uint64), C.POINTER(ServeStats)]
     _bound = True
     return L
 
@@ -120,6 +127,16 @@ class Batch:
     def step(self, nsteps):
         _check(self.L.llg_batch_step(self.h, int(nsteps)), self.L)
 
+    def set_launch(self, tpb, xcd=1):
+        _check(self.L.llg_batch_set_launch(self.h, int(tpb), int(xcd)), self.L)
+
+    def autotune(self, nsteps=10):
+        """Time every tile size and block order on the live members (their spins and step
+        counters are restored) and keep the fastest; returns the choice."""
+        r = BatchTune()
+        _check(self.L.llg_batch_autotune(self.h, int(nsteps), C.byref(r)), self.L)
+        return {f: getattr(r, f) for f, _ in BatchTune._fields_ if f != "pad_"}
+
     def reduce(self, members, stride=3):
         ms = np.ascontiguousarray(members, dtype=np.int32)
         out = np.zeros(len(ms) * stride)
@@ -169,23 +186,30 @@ def make_requests(specs):
 
 def serve(requests, device=0, budget_bytes=0, max_members=0, chunk=0, sequential=False, precision=1, graph=False,
           want_series=False):
-    """Run llg_serve_run; returns (results list of dicts, stats dict, series list or None)."""
+    """Run llg_serve_run; returns (results list of dicts, stats dict, series list or None).
+
+    With want_series, series[i] is request i's <M> samples (ceil(steps/update) x 3) and, for
+    a spin-wave request, results[i]["sq"] its S(k) samples (ceil(steps/update) x (L/2+1))
+    as complex numbers at k = (q, 0, 0)."""
     L = _bind()
     n = len(requests)
     cfg = ServeConfig(device, BATCH_GRAPH if graph else 0, int(budget_bytes), max_members, chunk,
                       1 if sequential else 0, precision)
     res = (Result * n)()
     st = ServeStats()
-    series = None
-    offs = None
+    series = offs = sq = sq_offs = None
+    counts = [(r.steps + r.update - 1) // r.update for r in requests]
     if want_series:
-        counts = [(r.steps + r.update - 1) // r.update for r in requests]
-        offs_np = np.zeros(n, np.uint64)
-        offs_np[1:] = np.cumsum(counts)[:-1]
+        offs = np.zeros(n, np.uint64)
+        offs[1:] = np.cumsum(counts)[:-1]
         series = np.zeros(int(sum(counts)) * 3)
-        offs = offs_np
+        sqn = [counts[k] * (r.L // 2 + 1) * 2 if r.type == REQ_SW else 0 for k, r in enumerate(requests)]
+        sq_offs = np.zeros(n, np.uint64)
+        sq_offs[1:] = np.cumsum(sqn)[:-1]
+        sq = np.zeros(max(int(sum(sqn)), 1))
+    u64 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint64)) if a is not None else None
     _check(L.llg_serve_run(C.byref(cfg), requests, n, res, _p(series, C.c_double) if series is not None else None,
-                           offs.ctypes.data_as(C.POINTER(C.c_uint64)) if offs is not None else None, C.byref(st)), L)
+                           u64(offs), _p(sq, C.c_double) if sq is not None else None, u64(sq_offs), C.byref(st)), L)
     out = [dict(arrival_s=r.arrival_s, admit_s=r.admit_s, done_s=r.done_s, latency_s=r.latency_s,
                 m_final=list(r.m_final), m_mean=list(r.m_mean), samples=r.samples) for r in res]
     stats = {f: getattr(st, f) for f, _ in ServeStats._fields_}
@@ -193,7 +217,12 @@ def serve(requests, device=0, budget_bytes=0, max_members=0, chunk=0, sequential
     if series is not None:
         ser = []
         for k, r in enumerate(requests):
-            c = (r.steps + r.update - 1) // r.update
+            c = counts[k]
             o = int(offs[k])
             ser.append(series[3 * o:3 * (o + c)].reshape(c, 3))
+            if r.type == REQ_SW:
+                nk = r.L // 2 + 1
+                q = int(sq_offs[k])
+                v = sq[q:q + 2 * nk * c].reshape(c, nk, 2)
+                out[k]["sq"] = v[..., 0] + 1j * v[..., 1]
     return out, stats, ser

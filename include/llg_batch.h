@@ -57,6 +57,25 @@ int llg_batch_set_applied(llg_batch* b, int member, const double B[3]);
 /* Advance every member nsteps steps (each from its own step counter). */
 int llg_batch_step(llg_batch* b, uint32_t nsteps);
 
+/* Launch shape of the batched kernels: tpb threads per tile (one block per tile: 128, 256
+ * or 512) and the XCD-aware block order (1) or plain order (0).  Results do not depend on
+ * it (each thread owns one spin of one member). */
+int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd);
+
+/* The batch's own autotuner (no reference counterpart; the reference takes
+ * threadsperblock from its config, src/cuda.cu:322-324): times nsteps steps of the live
+ * members for every tile size and block order (with hipGraph chunks of nsteps steps when
+ * the batch replays graphs), keeps the fastest, and restores every member's spins and
+ * step counter. */
+typedef struct llg_batch_tune {
+    int tpb, xcd;
+    double ms_per_step;          /* chosen */
+    double ms_per_step_default;  /* 256 threads, XCD order */
+    int candidates;
+    int pad_;
+} llg_batch_tune;
+int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out);
+
 /* FP64 sublattice magnetisation of every member listed, m[k*nsub_max*3 ...]: each member
  * writes n_sublattice*3 values at offset k*stride. */
 int llg_batch_reduce(llg_batch* b, const int* members, uint32_t count, uint32_t stride, double* m);
@@ -66,11 +85,16 @@ int llg_batch_get_info(llg_batch* b, llg_batch_info* info);
 
 /* ---------------------------------------------------------------- request loop */
 
-/* Request types: the semantics of the reference drivers (src/mvt.cpp, src/ramp_field.cpp):
+/* Request types: the semantics of the reference drivers (src/mvt.cpp, src/ramp_field.cpp,
+ * src/timeseries.cpp):
  *   LLG_REQ_MT    fixed T and B: run `steps`, sample <M> every `update` steps;
  *   LLG_REQ_RAMP  field ramp B(t) = B0 + (B1 - B0) t/steps, B updated only on sampling
- *                 steps (src/ramp_field.cpp:209-224), T fixed. */
-enum { LLG_REQ_MT = 0, LLG_REQ_RAMP = 1 };
+ *                 steps (src/ramp_field.cpp:209-224), T fixed;
+ *   LLG_REQ_SW    spin-wave time series at fixed T and B0: every `update` steps, <M> and the
+ *                 transverse structure factor S(k) (quantisation axis z, llg_set_sq with
+ *                 ua = ub = 1) at k = (q, 0, 0) mesh points, q = 0 .. L/2, as the
+ *                 time-series driver samples (src/timeseries.cpp:256-378). */
+enum { LLG_REQ_MT = 0, LLG_REQ_RAMP = 1, LLG_REQ_SW = 2 };
 
 typedef struct llg_request {
     uint32_t type;
@@ -119,9 +143,12 @@ typedef struct llg_serve_stats {
  * and memory is free, advance in shared launches, and leave when done.  results[i]
  * belongs to reqs[i]; m_series (may be NULL) receives for request i its samples
  * (3 doubles each) at series_offset[i] (computed by the caller as the prefix sum of
- * ceil(steps/update)). */
+ * ceil(steps/update)).  sq_series (may be NULL) receives for spin-wave request i, sample s,
+ * the L/2 + 1 complex S(k) values (re, im) at doubles sq_offset[i] + s (L/2 + 1) 2
+ * (the caller's prefix sum of ceil(steps/update) (L/2 + 1) 2 over spin-wave requests). */
 int llg_serve_run(const llg_serve_config* cfg, const llg_request* reqs, uint32_t nreq, llg_result* results,
-                  double* m_series, const uint64_t* series_offset, llg_serve_stats* stats);
+                  double* m_series, const uint64_t* series_offset, double* sq_series, const uint64_t* sq_offset,
+                  llg_serve_stats* stats);
 
 #ifdef __cplusplus
 }

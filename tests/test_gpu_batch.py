@@ -3,6 +3,8 @@
 Bars: a member's trajectory is bit-identical whether it runs alone or with other members
 (same kernel, no cross-member data); batch vs single-handle kernels agree to 1e-6 (FP32,
 same arithmetic in two kernels); graph replay is bit-identical to eager launches."""
+import math
+
 import numpy as np
 import pytest
 
@@ -121,6 +123,94 @@ def test_serve_results_same_alone_and_in_mix(gpu):
         assert np.array_equal(ser[k], ser1[0]), k
         assert alone[0]["m_final"] == mix[k]["m_final"]
     print(st)
+
+
+def test_batch_autotune_restores_and_shapes_agree(gpu):
+    """The batch autotuner restores every member's spins and step counter, and every tile
+    size / block order gives bit-identical trajectories (one thread per spin)."""
+    res = []
+    for shape in (None, (128, 0), (512, 1), "tune"):
+        bt = SB.Batch(flags=SB.BATCH_GRAPH)
+        ids = [bt.add(s, s0, T, B) for s, s0, T, B in _members()]
+        before = [bt.get_spins(k) for k in ids]
+        if shape == "tune":
+            t = bt.autotune(5)
+            assert t["candidates"] == 6 and t["tpb"] in (128, 256, 512)
+            for k, s0 in zip(ids, before):
+                assert np.array_equal(bt.get_spins(k), s0)
+                assert bt.get_step(k) == 0
+        elif shape is not None:
+            bt.set_launch(*shape)
+        bt.step(30)
+        res.append([bt.get_spins(k) for k in ids])
+        bt.close()
+    for r in res[1:]:
+        for a, b in zip(res[0], r):
+            assert np.array_equal(a, b)
+
+
+def test_batch_ensemble_langevin(gpu):
+    """Ensemble M(T) from one batch: replicas of n free spins in B at four temperatures
+    (FP32, counter noise, hipGraph chunks).  <S_z> must equal the Langevin function
+    coth x - 1/x, x = mu mu_B B / (k_B T), within five standard errors: per-spin variance
+    1 - 2 L(x)/x - L(x)^2, n spins, and one independent sample per two relaxation times
+    (1 + lambda^2)/(lambda gamma B) of the sampling window."""
+    n, B, lam, dt = 4096, 10.0, 1.0, 2e-15
+    xs = (0.5, 1.0, 2.0, 4.0)
+    bt = SB.Batch(flags=SB.BATCH_GRAPH)
+    ids = []
+    for k, x in enumerate(xs):
+        T = 2.22 * sllg.MU_B * B / (sllg.K_B * x)
+        sysd = sllg.System(n=n, species=[dict(mu=2.22, lambda_=lam)], dt=dt, precision="fp32", seed=40 + k)
+        s0 = np.zeros((3, n))
+        s0[2] = 1.0
+        ids.append(bt.add(sysd, s0, T=T, B=(0.0, 0.0, B)))
+    for _ in range(60):
+        bt.step(100)
+    acc = []
+    for _ in range(60):
+        bt.step(50)
+        acc.append(bt.reduce(ids)[:, 2])
+    m = np.mean(np.array(acc), axis=0)
+    tau = (1 + lam ** 2) / (lam * 1.76e11 * B) / dt  # relaxation time in steps
+    n_t = max(1.0, 60 * 50 / (2 * tau))
+    for x, mz in zip(xs, m):
+        L = 1 / math.tanh(x) - 1 / x
+        tol = 5 * math.sqrt(1 - 2 * L / x - L * L) / math.sqrt(n * n_t)
+        print(f"x={x} <Sz>={mz:.4f} Langevin={L:.4f} tol={tol:.4f}")
+        assert abs(mz - L) < tol, (x, mz, L, tol)
+    bt.close()
+
+
+def test_serve_spin_wave_requests(gpu):
+    """Spin-wave requests sample S(k) at (q, 0, 0) beside <M>: S(0)/n is the transverse
+    magnetisation of the same sample; T = 0 from the +z start gives S(k) = 0; each request's
+    S(k) series is the same alone and in a mix with M(T) and ramp requests; the sequential
+    FP64 plain comparator samples the same quantities."""
+    specs = [dict(type=SB.REQ_SW, L=8, steps=500, update=50, T=300.0, seed=31, arrival_s=0.0),
+             dict(type=SB.REQ_MT, L=10, steps=400, update=100, T=100.0, seed=32, arrival_s=0.0),
+             dict(type=SB.REQ_SW, L=12, steps=300, update=100, T=0.0, seed=33, arrival_s=0.005,
+                  B0=(0.0, 0.0, 0.5)),
+             dict(type=SB.REQ_RAMP, L=6, steps=300, update=100, T=50.0, seed=34, arrival_s=0.01,
+                  B0=(0, 0, -1.0), B1=(0, 0, 1.0))]
+    reqs = SB.make_requests(specs)
+    mix, st, ser = SB.serve(reqs, want_series=True)
+    assert st["completed"] == 4
+    sq0 = mix[0]["sq"]
+    n0 = 8 ** 3
+    assert sq0.shape == (10, 5)
+    mt = ser[0][:, 0] + 1j * ser[0][:, 1]
+    assert np.abs(sq0[:, 0] / n0 - mt).max() <= 1e-9
+    assert np.abs(sq0[1:]).max() > 0  # thermal spin waves after the first sample
+    assert np.abs(mix[2]["sq"]).max() == 0.0
+    assert "sq" not in mix[1] and "sq" not in mix[3]
+    for k in (0, 2):
+        alone, _, _ = SB.serve(SB.make_requests([dict(specs[k], arrival_s=0.0)]), want_series=True)
+        assert np.array_equal(alone[0]["sq"], mix[k]["sq"]), k
+    seq, _, sers = SB.serve(SB.make_requests([dict(specs[0], arrival_s=0.0)]), want_series=True, sequential=True,
+                            precision=sllg.FP64_PLAIN)
+    assert seq[0]["sq"].shape == sq0.shape
+    assert np.abs(seq[0]["sq"][:, 0] / n0 - (sers[0][:, 0] + 1j * sers[0][:, 1])).max() <= 1e-9
 
 
 def test_serve_sequential_plain_matches_physics(gpu):
