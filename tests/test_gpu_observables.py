@@ -105,3 +105,25 @@ def test_structure_factor_matches_fft(gpu, prec, axis):
     assert np.allclose(per.sum(axis=0), tot, rtol=0, atol=1e-9)
     with pytest.raises(sllg.LLGError):
         g.set_sq(grid, mesh, np.array([[grid[0], 0, 0]]), axis=axis)
+
+
+def test_mvt_driver_on_device(tmp_path):
+    """The M(T) driver on an 8^3 sc ferromagnet (FP32, lambda = 0.5, ordered start): one
+    MvT.dat line per temperature and |m| falling with T (Tc ~ 313 K for J = 3e-21 J)."""
+    from sllg import drivers
+
+    sysd, _, _ = W.sc_ferro(L=8, precision="fp32", lam=0.5, dt=1e-15)
+    g = sllg.Context(sysd)
+    s = np.zeros((3, sysd.n))
+    s[2] = 1.0
+    g.set_spins(s)
+    out = tmp_path / "MvT.dat"
+    rows = drivers.mvt(g, 0.0, 400.0, 100.0, dt=1e-15, update=50, min_run_time=2e-12, max_run_time=8e-12,
+                       mean_tol=1e-4, var_tol=1e-5, path=str(out))
+    lines = out.read_text().splitlines()
+    assert lines[0] == "#Temperature\tMean" and len(lines) == 5
+    m = [r["modm"][0] for r in rows]
+    print(rows)
+    assert abs(m[0] - 1.0) < 1e-5  # T = 0 stays ordered (no noise, anisotropy along z)
+    assert m[0] > m[1] > m[2] > m[3]
+    assert all(r["steps"] >= int(2e-12 / 1e-15) for r in rows)  # 1999: int() truncates as the reference does
