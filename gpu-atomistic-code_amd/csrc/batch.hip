@@ -50,6 +50,7 @@ struct llg_batch {
     uint32_t graphs_built = 0;
     uint32_t tpb = 256;  // threads per tile (one block per tile)
     int xcd = 1;         // XCD-aware block order
+    int line = 1;        // line reader for members with a line encoding (else their packed entries)
     DevBuf red;
     std::vector<std::shared_ptr<ExchStore>> pinned;  // coupling stores kept for future members
 };
@@ -89,7 +90,8 @@ int regroup(llg_batch* b) {
     for (size_t k = 0; k < b->m.size(); ++k) {
         if (!b->m[k].live) continue;
         const llg_ctx* c = b->m[k].c;
-        const auto key = std::make_pair(c->ex32.fmt, c->ex32.ncomp);
+        const int f = (is_line_fmt(c->ex32.fmt) && !b->line) ? (int)FMT_SELL_PACK : c->ex32.fmt;
+        const auto key = std::make_pair(f, c->ex32.ncomp);
         if (!by.count(key)) {
             b->groups.emplace_back(new Group());
             by[key] = b->groups.back().get();
@@ -339,12 +341,14 @@ int llg_batch_step(llg_batch* b, uint32_t nsteps) {
     return 0;
 }
 
-int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd) {
+int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd, int line_mode) {
     if (!b) SLLG_FAIL(LLG_E_INVALID, "NULL batch");
     if (tpb != 128 && tpb != 256 && tpb != 512) SLLG_FAIL(LLG_E_INVALID, "tile of %u threads (128, 256, 512)", tpb);
     if (tpb != b->tpb || (xcd ? 1 : 0) != b->xcd) drop_graph(b);  // the captured launches have the old shape
+    if ((line_mode ? 1 : 0) != b->line) b->dirty = true;        // members regroup by the reader they run
     b->tpb = tpb;
     b->xcd = xcd ? 1 : 0;
+    b->line = line_mode ? 1 : 0;
     return 0;
 }
 
@@ -363,17 +367,22 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
         t_save[k] = b->m[k].t;
     }
     const uint32_t tpb0 = b->tpb;
-    const int xcd0 = b->xcd;
+    const int xcd0 = b->xcd, line0 = b->line;
+    bool any_line = false;
+    for (auto& mm : b->m) any_line = any_line || (mm.live && is_line_fmt(mm.c->ex32.fmt));
+    const int nline = any_line ? 2 : 1;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     SLLG_HIP(hipEventCreate(&e0));
     SLLG_HIP(hipEventCreate(&e1));
     const uint32_t tpbs[3] = {128, 256, 512};
     double best = 1e300;
     uint32_t btpb = 256;
-    int bxcd = 1, rc = 0;
+    int bxcd = 1, bline = line0, rc = 0;
+    for (int li = 0; li < nline && !rc; ++li)
     for (int ti = 0; ti < 3 && !rc; ++ti)
         for (int xcd = 1; xcd >= 0 && !rc; --xcd) {
-            rc = llg_batch_set_launch(b, tpbs[ti], xcd);
+            const int line = nline == 2 ? 1 - li : line0;
+            rc = llg_batch_set_launch(b, tpbs[ti], xcd, line);
             if (!rc) rc = llg_batch_step(b, 2);  // warm-up (and graph capture when enabled)
             if (!rc) rc = llg_batch_step(b, nsteps);  // a graph of this length is captured here
             if (rc) break;
@@ -385,12 +394,13 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
             if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = LLG_E_DEVICE;
             if (rc) break;
             const double per = (double)ms / nsteps;
-            if (tpbs[ti] == 256 && xcd == 1) out->ms_per_step_default = per;
+            if (tpbs[ti] == 256 && xcd == 1 && li == 0) out->ms_per_step_default = per;
             out->candidates++;
             if (per < best) {
                 best = per;
                 btpb = tpbs[ti];
                 bxcd = xcd;
+                bline = line;
             }
         }
     (void)hipEventDestroy(e0);
@@ -405,12 +415,13 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
     }
     if (hipStreamSynchronize(b->stream) != hipSuccess && !rc) rc = LLG_E_DEVICE;
     if (rc) {
-        (void)llg_batch_set_launch(b, tpb0, xcd0);
+        (void)llg_batch_set_launch(b, tpb0, xcd0, line0);
         return rc;
     }
-    SLLG_TRY(llg_batch_set_launch(b, btpb, bxcd));
+    SLLG_TRY(llg_batch_set_launch(b, btpb, bxcd, bline));
     out->tpb = (int)btpb;
     out->xcd = bxcd;
+    out->line_mode = bline;
     out->ms_per_step = best;
     return 0;
 }

@@ -740,7 +740,7 @@ int llg_get_info(llg_ctx* c, llg_info* info) {
         else if (h.fmt == LLG_EXCH_STENCIL && h.per_row) M = (double)h.sk * 8.0 * nc;
     } else {
         const Exch<float>& x = c->ex32;
-        if (x.fmt == FMT_SELL_PACK || x.fmt == FMT_SELL_LINE) M = (double)x.nent / n * 4.0;
+        if (x.fmt == FMT_SELL_PACK || is_line_fmt(x.fmt)) M = (double)x.nent / n * 4.0;
         else if (x.fmt == FMT_SELL_RAW) M = (double)x.nent / n * (4.0 + 4.0 * nc);
         else if (x.fmt == FMT_DIA) M = (double)h.ndiag * 4.0 * nc;
         else if (x.fmt == FMT_STENCIL && h.per_row) M = (double)h.sk * 4.0 * nc;
@@ -800,7 +800,7 @@ int create_with_store(const llg_desc* d, const std::shared_ptr<ExchStore>& store
 namespace {
 std::mutex g_tune_mu;
 struct TuneChoice {
-    int tpb, xcd, unr, thc, g3;
+    int tpb, xcd, unr, thc, g3, line;
 };
 std::map<std::string, TuneChoice> g_tune_cache;
 
@@ -820,7 +820,8 @@ void tune_file_load() {
     if (!fp) return;
     char key[256];
     TuneChoice t{};
-    while (std::fscanf(fp, "%255s %d %d %d %d %d", key, &t.tpb, &t.xcd, &t.unr, &t.thc, &t.g3) == 6) g_tune_cache[key] = t;
+    while (std::fscanf(fp, "%255s %d %d %d %d %d %d", key, &t.tpb, &t.xcd, &t.unr, &t.thc, &t.g3, &t.line) == 7)
+        g_tune_cache[key] = t;
     std::fclose(fp);
 }
 
@@ -829,12 +830,13 @@ void tune_file_append(const std::string& key, const TuneChoice& t) {
     if (!f) return;
     FILE* fp = std::fopen(f, "a");
     if (!fp) return;
-    std::fprintf(fp, "%s %d %d %d %d %d\n", key.c_str(), t.tpb, t.xcd, t.unr, t.thc, t.g3);
+    std::fprintf(fp, "%s %d %d %d %d %d %d\n", key.c_str(), t.tpb, t.xcd, t.unr, t.thc, t.g3, t.line);
     std::fclose(fp);
 }
 }  // namespace
 
-extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd, int unroll, int thermal_cache, int gather_copy) {
+extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd, int unroll, int thermal_cache, int gather_copy,
+                              int line_mode) {
     if (!c || (tpb != 256 && tpb != 512 && tpb != 1024)) SLLG_FAIL(LLG_E_INVALID, "block size must be 256, 512 or 1024");
     if (unroll != 2 && unroll != 4) SLLG_FAIL(LLG_E_INVALID, "unroll must be 2 or 4");
     c->tune_tpb = tpb;
@@ -842,6 +844,7 @@ extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd, int unroll, int ther
     c->tune_unr = unroll;
     c->tune_thc = thermal_cache ? 1 : 0;
     c->tune_g3 = gather_copy ? 1 : 0;
+    c->tune_line = line_mode ? 1 : 0;
     return 0;
 }
 
@@ -853,6 +856,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     out->unroll = c->tune_unr;
     out->thermal_cache = c->tune_thc;
     out->gather_copy = c->tune_g3;
+    out->line_mode = c->tune_line;
     if (c->prec != LLG_FP32 || c->scheme != LLG_HEUN) return 0;
     SLLG_HIP(hipSetDevice(c->device));
     const std::string key = tune_key(c);
@@ -870,6 +874,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
             c->tune_unr = out->unroll = it->second.unr;
             c->tune_thc = out->thermal_cache = it->second.thc;
             c->tune_g3 = out->gather_copy = it->second.g3;
+            c->tune_line = out->line_mode = it->second.line;
             out->from_cache = 1;
             return 0;
         }
@@ -879,23 +884,26 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     SLLG_TRY(save.alloc(c->S.bytes));
     SLLG_HIP(hipMemcpyAsync(save.p, c->S.p, c->S.bytes, hipMemcpyDeviceToDevice, c->stream));
     const int tpbs[3] = {256, 512, 1024};
-    const int nunr = c->ex32.fmt == FMT_SELL_PACK ? 2 : 1;
+    // line encodings: line reader or the packed entries kept beside it
+    const int nline = is_line_fmt(c->ex32.fmt) ? 2 : 1;
     const int unrs[2] = {2, 4};
     // handing the thermal field over only matters at T > 0 with counter noise
     const int nthc = (c->T > 0.0 && c->noise_mode == LLG_NOISE_COUNTER) ? 2 : 1;
     // the AoS3 gather copy applies to the packed and line encodings
-    const int ng3 = (c->ex32.fmt == FMT_SELL_PACK || c->ex32.fmt == FMT_SELL_LINE) ? 2 : 1;
+    const int ng3 = (c->ex32.fmt == FMT_SELL_PACK || is_line_fmt(c->ex32.fmt)) ? 2 : 1;
+    const int line0 = c->tune_line;
     double best = 1e300;
-    TuneChoice b{256, 1, 2, c->tune_thc, c->tune_g3};
+    TuneChoice b{256, 1, 2, c->tune_thc, c->tune_g3, c->tune_line};
     hipEvent_t e0, e1;
     SLLG_HIP(hipEventCreate(&e0));
     SLLG_HIP(hipEventCreate(&e1));
     const int noise = c->noise_mode;
     c->noise_mode = LLG_NOISE_COUNTER;  // no host queue traffic while timing
     int rc = 0;
+    for (int li = 0; li < nline && !rc; ++li)
     for (int gi = 0; gi < ng3 && !rc; ++gi)
     for (int hi = 0; hi < nthc && !rc; ++hi)
-    for (int ui = 0; ui < nunr && !rc; ++ui)
+    for (int ui = 0; ui < ((nline == 2 ? li == 1 : c->ex32.fmt == FMT_SELL_PACK) ? 2 : 1) && !rc; ++ui)
         for (int ti = 0; ti < 3 && !rc; ++ti)
             for (int xcd = 1; xcd >= 0 && !rc; --xcd) {
                 c->tune_tpb = tpbs[ti];
@@ -903,6 +911,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
                 c->tune_unr = unrs[ui];
                 if (nthc == 2) c->tune_thc = 1 - hi;
                 if (ng3 == 2) c->tune_g3 = 1 - gi;
+                c->tune_line = nline == 2 ? 1 - li : line0;
                 rc = do_steps(c, 0, 2);
                 if (rc) break;
                 (void)hipEventRecord(e0, c->stream);
@@ -912,11 +921,12 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 const double per = ms / nsteps;
-                if (tpbs[ti] == 256 && xcd == 1 && unrs[ui] == 2 && hi == 0 && gi == 0) out->ms_per_step_default = per;
+                if (tpbs[ti] == 256 && xcd == 1 && unrs[ui] == 2 && hi == 0 && gi == 0 && li == 0)
+                    out->ms_per_step_default = per;
                 out->candidates++;
                 if (per < best) {
                     best = per;
-                    b = TuneChoice{tpbs[ti], xcd, unrs[ui], c->tune_thc, c->tune_g3};
+                    b = TuneChoice{tpbs[ti], xcd, unrs[ui], c->tune_thc, c->tune_g3, c->tune_line};
                 }
             }
     c->noise_mode = noise;
@@ -932,6 +942,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     c->tune_unr = out->unroll = b.unr;
     c->tune_thc = out->thermal_cache = b.thc;
     c->tune_g3 = out->gather_copy = b.g3;
+    c->tune_line = out->line_mode = b.line;
     out->ms_per_step = best;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = b;

@@ -29,6 +29,7 @@ constexpr int kDictLds = 2048;  // floats of coupling dictionary staged in LDS (
 // Line-mode group records: line_group(nc) neighbour lines per record, line_words(nc) words
 // (this group's line words, 3 shifts x nc couplings per line, the next group's line words
 // last; padded to whole 16-byte units).
+constexpr uint32_t kLineFallback = 0xFFFFFFFFu;  // line-mode slice read from the packed entries
 constexpr int line_group(int nc) { return nc == 1 ? 3 : (nc == 3 ? 2 : 1); }
 constexpr int line_words(int nc) { return (2 * line_group(nc) + 3 * line_group(nc) * nc + 3) / 4 * 4; }
 
@@ -39,8 +40,17 @@ enum ExchFmt : int {
     FMT_STENCIL = 3,    // periodic stencil (shared or per-row values)
     FMT_SELL_PACK = 4,  // FP32: sliced ELL, u32 = (row offset << dbits) | dictionary index
     FMT_SELL_RAW = 5,   // FP32: sliced ELL, u32 col + FP32 values [ncomp][entries]
-    FMT_SELL_LINE = 6   // FP32: slices that are periodic lines, lane-uniform scalar coupling stream
+    FMT_SELL_LINE = 6,   // FP32: slices that are periodic 64-cell lines, lane-uniform scalar coupling stream
+    FMT_SELL_LINE16 = 7,      // FP32: slices of four periodic 16-cell lines, same stream (rotations within 16 lanes)
+    FMT_SELL_LINE_OPEN = 8,   // FP32: open 64-cell lines (zero-filling shifts instead of rotations)
+    FMT_SELL_LINE16_OPEN = 9  // FP32: four open 16-cell lines per slice
 };
+
+constexpr bool is_line_fmt(int f) {
+    return f == FMT_SELL_LINE || f == FMT_SELL_LINE16 || f == FMT_SELL_LINE_OPEN || f == FMT_SELL_LINE16_OPEN;
+}
+constexpr int line_len(int f) { return (f == FMT_SELL_LINE || f == FMT_SELL_LINE_OPEN) ? 64 : 16; }
+constexpr bool line_open(int f) { return f == FMT_SELL_LINE_OPEN || f == FMT_SELL_LINE16_OPEN; }
 
 // Per-species constants, precision R.
 template <class R> struct Species {
@@ -81,6 +91,11 @@ template <class V> struct Exch {
     uint64_t nent = 0;                    // total entries (SELL_RAW value stride)
     uint32_t ncell = 0;                   // rows per sublattice block of the device order
     uint32_t dbits = 0;                   // dictionary-index bits of a packed entry
+    // line mode (FMT_SELL_LINE / LINE16): per slice {first word, groups}; groups ==
+    // kLineFallback sends the slice to the packed entries above
+    const uint2* lslice = nullptr;
+    const uint32_t* lent = nullptr;       // group records
+    uint32_t lfall = 0;                   // slices that use the packed entries
     // four-spin quartets of each row (same row order as the spins of this path)
     const uint32_t* fs_xadj = nullptr;    // [n+1]; null = term off
     const uint4* fs_jkl = nullptr;        // (j, k, l, 0)
@@ -185,6 +200,7 @@ struct llg_ctx {
     int tune_unr = 2;    // packed-exchange chunks loaded ahead per wave (llg_autotune)
     int tune_thc = 1;    // predictor hands its thermal field to the corrector (llg_autotune)
     int tune_g3 = 1;     // gather neighbours from the AoS3 copy of the spins (llg_autotune)
+    int tune_line = 1;   // use the line reader when the couplings have a line encoding (llg_autotune)
     bool g3_valid = false;   // S3 holds the current S
     bool y_soa_valid = true;  // Y (SoA) holds the last predictor output (else only Y3 does)
 
@@ -220,3 +236,12 @@ struct llg_ctx {
     // timing
     sllg::KernelTimer* timer = nullptr;
 };
+
+namespace sllg {
+// Encoding the FP32 kernels run with: a line encoding keeps the packed entries beside its
+// line stream, so the packed reader can run instead (tuned per handle).
+inline int exec_fmt(const llg_ctx* c) {
+    const int f = c->ex32.fmt;
+    return (is_line_fmt(f) && !c->tune_line) ? FMT_SELL_PACK : f;
+}
+}  // namespace sllg

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -177,13 +178,19 @@ struct TupleHash {
     }
 };
 
-// Line mode of the FP32 path.  When every 64-row slice is one periodic line of cells along
-// the fastest cell index (so neighbour j of lane l is lane (l + dz) mod 64 of an aligned
-// line starting at device row L, dz in {0, +1, -1}), the couplings of a slice are
-// lane-uniform: per line, one aligned load of the spins, and the shifted neighbours come
-// from a wave rotation.  A slice becomes a list of group records read with scalar loads,
+// Line mode of the FP32 path.  When the lattice's fastest cell index runs along lines of
+// len = 64 or 16 cells, a 64-row slice is one line (len 64) or four consecutive lines
+// (len 16), and neighbour j of lane l is at row
+//   L + (l & ~(len - 1)) + ((l + dz) mod len),   dz in {0, +1, -1},
+// for periodic lines, or at L + l + dz with no neighbour past a line's ends for open lines,
+// of a 64-row window starting at row L (a multiple of len).  Where that holds for every
+// lane with one coupling tuple, the couplings of a slice are lane-uniform: per window, one
+// load of the spins, and the shifted neighbours come from a rotation (periodic) or a
+// zero-filling shift (open) across the wave (len 64) or within each 16-lane row (len 16).  Slices where it does not hold (a window
+// that would wrap a periodic boundary inside the slice) keep the packed entries
+// (kLineFallback).  A slice becomes a list of group records read with scalar loads,
 // G = line_group(ncomp) lines each, line_words(ncomp) words:
-//   [0, G)           line words: (L - slice base) | mask, L a multiple of 64, mask bit 0/1/2
+//   [0, G)           line words: (L - slice base) | mask, L a multiple of len, mask bit 0/1/2
 //                    = couplings present for dz = 0 / +1 / -1 (0 for an unused slot)
 //   [G, G + 3 G nc)  couplings [line][dz 0, +1, -1][component], FP32, zero where absent
 //   [W - G, W)       the next group's line words (zero after the last group); W is
@@ -193,7 +200,8 @@ struct TupleHash {
 // Identical record lists are stored once; the first record is a zero record for empty
 // slices.  Entries are summed line by line, not in stored CSR order.
 bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vector<uint32_t>& idx,
-                 const std::vector<TupleKey>& tuples, uint32_t& stored, uint32_t& lines_row) {
+                 const std::vector<TupleKey>& tuples, uint32_t len, bool open, uint32_t& stored,
+                 uint32_t& lines_row, uint32_t& nfall) {
     ExchStore& X = *c->xs;
     const HostExch& h = X.h;
     const uint32_t n = c->n;
@@ -207,55 +215,88 @@ bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vec
     std::unordered_map<uint64_t, std::vector<uint32_t>> seen;  // record-list hash -> stored slices
     std::vector<uint32_t> st;
     stored = 0;
-    uint64_t lines_total = 0;
+    nfall = 0;
+    uint64_t lines_total = 0, lined = 0;
+    const int32_t lm = (int32_t)len - 1;
     struct Line {
         int32_t lrel;
         uint32_t mask;
         double v[3][9];
     };
-    for (uint32_t s = 0; s < nsl; ++s) {
-        const uint32_t d0 = s * kSlice;
-        const uint32_t r0 = ref_of[d0];
-        const uint32_t cnt = h.xadj[r0 + 1] - h.xadj[r0];
-        std::vector<Line> lines;
-        for (uint32_t q = 0; q < cnt; ++q) {
-            int dzq = 2;  // none found
-            int64_t Lq = 0;
-            uint32_t tq = 0;
+    // key of one coupling of row d0 + l: window start relative to d0, shift dz, tuple
+    using LKey = std::tuple<int64_t, int, uint32_t>;
+    // keys of row d0 + l in CSR order (csr, optional) and sorted (out, for comparing lanes)
+    auto lane_keys = [&](uint32_t d0, int32_t l, std::vector<LKey>& out, std::vector<LKey>* csr) -> bool {
+        out.clear();
+        const uint32_t r = ref_of[d0 + (uint32_t)l];
+        for (uint32_t qq = h.xadj[r]; qq < h.xadj[r + 1]; ++qq) {
+            const int64_t j = P.to_dev(h.adj[qq]);
+            bool found = false;
             for (int dz : {0, 1, -1}) {
-                bool ok = true;
-                int64_t L0 = -1;
-                for (uint32_t l = 0; l < kSlice && ok; ++l) {
-                    const uint32_t r = ref_of[d0 + l];
-                    if (h.xadj[r + 1] - h.xadj[r] != cnt) return false;
-                    const uint32_t qq = h.xadj[r] + q;
-                    const int64_t j = P.to_dev(h.adj[qq]);
-                    const int64_t L = j - (int64_t)((l + (uint32_t)(dz + 64)) & 63u);
-                    if (l == 0) {
-                        L0 = L;
-                        tq = idx[qq];
-                    }
-                    ok = (L == L0) && (L % kSlice == 0) && (idx[qq] == tq);
+                int32_t pos = (l & lm) + dz;
+                if (open) {
+                    if (pos < 0 || pos > lm) continue;
+                } else {
+                    pos &= lm;
                 }
-                if (ok) {
-                    dzq = dz;
-                    Lq = L0;
+                const int64_t L = j - (int64_t)((l & ~lm) + pos);
+                if (L % (int64_t)len == 0 && L >= 0 && L + kSlice <= (int64_t)n) {
+                    out.emplace_back(L - (int64_t)d0, dz, idx[qq]);
+                    found = true;
                     break;
                 }
             }
-            if (dzq == 2) return false;
-            const int32_t lrel = (int32_t)(Lq - (int64_t)d0);
-            Line* g = nullptr;
-            for (Line& x : lines)
-                if (x.lrel == lrel) g = &x;
-            if (!g) {
-                lines.push_back(Line{lrel, 0u, {}});
-                g = &lines.back();
-            }
-            const int k = dzq == 0 ? 0 : (dzq == 1 ? 1 : 2);
-            g->mask |= 1u << k;
-            for (int a = 0; a < nc; ++a) g->v[k][a] += tuples[tq].v[a];
+            if (!found) return false;
         }
+        if (csr) *csr = out;
+        std::sort(out.begin(), out.end());
+        return true;
+    };
+    std::vector<LKey> ref, ref_csr, lk, first, last;
+    for (uint32_t s = 0; s < nsl; ++s) {
+        const uint32_t d0 = s * kSlice;
+        std::vector<Line> lines;
+        // periodic lines: every lane has the same keys; open lines: every interior lane
+        // does, and a line's first (last) lane lacks the keys that shift past its start (end)
+        bool fits = lane_keys(d0, open ? 1 : 0, ref, &ref_csr);
+        if (fits && open) {
+            first.clear();
+            last.clear();
+            for (const LKey& k : ref) {
+                if (std::get<1>(k) != -1) first.push_back(k);
+                if (std::get<1>(k) != 1) last.push_back(k);
+            }
+        }
+        for (int32_t l = 0; l < kSlice && fits; ++l) {
+            fits = lane_keys(d0, l, lk, nullptr);
+            if (!fits) break;
+            const int32_t p = l & lm;
+            const std::vector<LKey>& want = !open ? ref : (p == 0 ? first : (p == lm ? last : ref));
+            fits = lk == want;
+        }
+        if (fits) {
+            // windows in the order the CSR row names them; repeated neighbours add up
+            for (const LKey& key : ref_csr) {
+                const int32_t lrel = (int32_t)std::get<0>(key);
+                const int dz = std::get<1>(key);
+                Line* g = nullptr;
+                for (Line& x : lines)
+                    if (x.lrel == lrel) g = &x;
+                if (!g) {
+                    lines.push_back(Line{lrel, 0u, {}});
+                    g = &lines.back();
+                }
+                const int k = dz == 0 ? 0 : (dz == 1 ? 1 : 2);
+                g->mask |= 1u << k;
+                for (int a = 0; a < nc; ++a) g->v[k][a] += tuples[std::get<2>(key)].v[a];
+            }
+        }
+        if (!fits) {  // this slice keeps its packed entries
+            table[s] = make_uint2(0u, kLineFallback);
+            ++nfall;
+            continue;
+        }
+        ++lined;
         st.clear();
         const size_t ng = (lines.size() + (size_t)G - 1) / (size_t)G;
         auto line_word = [&](size_t l) -> uint32_t {
@@ -299,14 +340,16 @@ bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vec
             ++stored;
         }
     }
+    // worth it only when most slices are lines
+    if (lined == 0 || 4u * nfall > nsl) return false;
     if (upload(X.d, words.data(), words.size(), c->stream)) return false;
     if (upload(X.e, table.data(), table.size(), c->stream)) return false;
     Exch<float>& x = X.ex32;
-    x.fmt = FMT_SELL_LINE;
-    x.slice = X.e.as<uint2>();
-    x.ent = X.d.as<uint32_t>();
-    x.nent = words.size();
-    lines_row = (uint32_t)(lines_total / nsl);
+    x.fmt = len == 64 ? (open ? FMT_SELL_LINE_OPEN : FMT_SELL_LINE) : (open ? FMT_SELL_LINE16_OPEN : FMT_SELL_LINE16);
+    x.lslice = X.e.as<uint2>();
+    x.lent = X.d.as<uint32_t>();
+    x.lfall = nfall;
+    lines_row = (uint32_t)(lines_total / lined);
     return true;
 }
 
@@ -507,11 +550,17 @@ int build_sell(llg_ctx* c) {
     // field) keeps the packed encoding set up above.
     if (pack && !(c->flags & (LLG_FLAG_NO_LINE_MODE | LLG_FLAG_NO_SLICE_SHARING))) {
         uint32_t lstored = 0, lines_row = 0;
-        if (build_lines(c, ref_of, idx, tuples, lstored, lines_row)) {
-            char buf[96];
-            std::snprintf(buf, sizeof buf, "sell64-line-t%u-slices%uof%u-lines%u", P.na, lstored, nsl, lines_row);
-            X.encoding = buf;
-        }
+        uint32_t nfall = 0;
+        bool done = false;
+        for (uint32_t len : {64u, 16u})
+            for (bool open : {false, true})
+                if (!done && build_lines(c, ref_of, idx, tuples, len, open, lstored, lines_row, nfall)) {
+                    char buf[128];
+                    std::snprintf(buf, sizeof buf, "line%u%s-t%u-slices%uof%u-lines%u-packed%u", len, open ? "o" : "",
+                                  P.na, lstored, nsl, lines_row, nfall);
+                    X.encoding = buf;
+                    done = true;
+                }
     }
     return 0;
 }

@@ -174,8 +174,9 @@ __device__ __forceinline__ float rot_prev(float v) {
 
 // Line-mode group record, read with scalar loads by the whole wave (constant indices only,
 // so it stays in SGPRs):
-//   w[0, G)            this group's line words: start row relative to the slice's first row
-//                      (int32, multiple of 64) | mask (bit 0: unshifted, 1: +1 lane, 2: -1 lane)
+//   w[0, G)            this group's line words: start row of the 64-row window relative to
+//                      the slice's first row (int32, a multiple of the line length, 64 or 16)
+//                      | mask (bit 0: unshifted, 1: next along the line, 2: previous)
 //   w[G, G + 3 G NC)   couplings [line][shift][component], FP32 (then zero padding)
 //   w[W - G, W)        the next group's line words (zero after the last group)
 template <int NC>
@@ -232,11 +233,27 @@ __device__ __forceinline__ void line_loads(const SpinBuf& sb, uint32_t lane, int
                                            u32x3* x) {
 #pragma unroll
     for (int u = 0; u < line_group(NC); ++u)
-        x[u] = line_v<G3>(sb, lane, __builtin_amdgcn_readfirstlane((uint32_t)(base + (int32_t)(lw[u] & ~63u))));
+        x[u] = line_v<G3>(sb, lane, __builtin_amdgcn_readfirstlane((uint32_t)(base + (int32_t)(lw[u] & ~7u))));
+}
+
+// Neighbours along lines of LEN cells: lane l takes lane l + 1 (next) or l - 1 (prev) of
+// its line.  Periodic lines wrap within the line: across the whole wave (wave_rol:1 /
+// wave_ror:1) for 64-cell lines, within each 16-lane row (row_ror:15 / row_ror:1) for
+// 16-cell lines.  Open lines shift instead (wave_shl:1 / wave_shr:1, row_shl:1 /
+// row_shr:1) and the lane past a line's end reads 0 (bound_ctrl): no neighbour there.
+template <int LEN, bool OPEN>
+__device__ __forceinline__ float line_next(float v) {
+    constexpr int ctrl = OPEN ? (LEN == 64 ? 0x130 : 0x101) : (LEN == 64 ? 0x134 : 0x12F);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, OPEN));
+}
+template <int LEN, bool OPEN>
+__device__ __forceinline__ float line_prev(float v) {
+    constexpr int ctrl = OPEN ? (LEN == 64 ? 0x138 : 0x111) : (LEN == 64 ? 0x13C : 0x121);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, OPEN));
 }
 
 // One group's arithmetic: per line, the unshifted and the two rotated neighbours.
-template <int NC>
+template <int NC, int LEN, bool OPEN>
 __device__ __forceinline__ void line_group_sum(const LineRecW<NC>& r, const u32x3* x, float dot[3]) {
 #pragma unroll
     for (int u = 0; u < LineRecW<NC>::G; ++u) {
@@ -244,29 +261,38 @@ __device__ __forceinline__ void line_group_sum(const LineRecW<NC>& r, const u32x
         const float s0[3] = {__uint_as_float(x[u].x), __uint_as_float(x[u].y), __uint_as_float(x[u].z)};
         if (m & 1u) line_entry<NC>(r, u, 0, s0, dot);
         if (m & 2u) {
-            const float xs[3] = {rot_next(s0[0]), rot_next(s0[1]), rot_next(s0[2])};
+            const float xs[3] = {line_next<LEN, OPEN>(s0[0]), line_next<LEN, OPEN>(s0[1]), line_next<LEN, OPEN>(s0[2])};
             line_entry<NC>(r, u, 1, xs, dot);
         }
         if (m & 4u) {
-            const float xs[3] = {rot_prev(s0[0]), rot_prev(s0[1]), rot_prev(s0[2])};
+            const float xs[3] = {line_prev<LEN, OPEN>(s0[0]), line_prev<LEN, OPEN>(s0[1]), line_prev<LEN, OPEN>(s0[2])};
             line_entry<NC>(r, u, 2, xs, dot);
         }
     }
 }
 
-// Line-mode reader (FMT_SELL_LINE): per group of G neighbour lines, one aligned line load
-// each, the +-1 shifted neighbours as wave rotations of the loaded line.  Software
-// pipelined: a group's record names the next group's lines, whose loads are issued before
-// the current group's arithmetic.
-template <int NC, bool G3>
+// Line-mode reader (FMT_SELL_LINE / LINE16): per group of G neighbour windows, one
+// 64-row window load each, the +-1 shifted neighbours as rotations along the lines of the
+// loaded window.  Software pipelined: a group's record names the next group's windows,
+// whose loads are issued before the current group's arithmetic.  A slice marked
+// kLineFallback is read from the packed entries.
+// LFB: the encoding has slices marked kLineFallback (compiled in only then: the packed
+// reader's registers would otherwise cost the line kernels occupancy).
+template <int NC, bool G3, int LEN, bool OPEN, bool LFB>
 __device__ __forceinline__ void exch_row_line(const Exch<float>& e, uint32_t i, const float* __restrict__ S,
-                                              float dot[3]) {
+                                              const float* sdict, float dot[3]) {
     using RW = LineRecW<NC>;
     constexpr int G = RW::G, W = RW::W, NX = RW::NX;
-    dot[0] = dot[1] = dot[2] = 0.0f;
     const uint32_t sl = __builtin_amdgcn_readfirstlane(i >> 6);
-    const uint2 sd = e.slice[sl];
-    const uint32_t* __restrict__ rp = e.ent + sd.x;
+    const uint2 sd = e.lslice[sl];
+    if constexpr (LFB) {
+        if (sd.y == kLineFallback) {
+            exch_row_pack_fast<NC, 2, G3>(e, i, S, sdict, dot);
+            return;
+        }
+    }
+    dot[0] = dot[1] = dot[2] = 0.0f;
+    const uint32_t* __restrict__ rp = e.lent + sd.x;
     const uint32_t ngrp = sd.y, lane = i & 63u;
     const int32_t base = (int32_t)(sl << 6);
     if (ngrp == 0) return;
@@ -283,19 +309,19 @@ __device__ __forceinline__ void exch_row_line(const Exch<float>& e, uint32_t i, 
     for (; r + 2 < ngrp; r += 2) {
         const RW c0 = load_line_rec<NC>(rp + (size_t)r * W);
         line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
-        line_group_sum<NC>(c0, xa, dot);
+        line_group_sum<NC, LEN, OPEN>(c0, xa, dot);
         const RW c1 = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
         line_loads<NC, G3>(sb, lane, base, c1.w + NX, xa);
-        line_group_sum<NC>(c1, xb, dot);
+        line_group_sum<NC, LEN, OPEN>(c1, xb, dot);
     }
     const RW c0 = load_line_rec<NC>(rp + (size_t)r * W);
     if (r + 1 < ngrp) {
         line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
-        line_group_sum<NC>(c0, xa, dot);
+        line_group_sum<NC, LEN, OPEN>(c0, xa, dot);
         const RW c1 = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
-        line_group_sum<NC>(c1, xb, dot);
+        line_group_sum<NC, LEN, OPEN>(c1, xb, dot);
     } else {
-        line_group_sum<NC>(c0, xa, dot);
+        line_group_sum<NC, LEN, OPEN>(c0, xa, dot);
     }
 }
 
@@ -380,7 +406,7 @@ __device__ __forceinline__ void row_of_thread(const FusedArgs& a, uint32_t tpb, 
 // line reader out of divergent control flow (its record words stay scalar).
 template <int FMT>
 __device__ __forceinline__ bool row_live(const FusedArgs& a, uint32_t i) {
-    if constexpr (FMT == FMT_SELL_LINE)
+    if constexpr (is_line_fmt(FMT))
         return __builtin_amdgcn_readfirstlane(i & ~63u) < a.n;
     else
         return i < a.n;
@@ -389,7 +415,7 @@ __device__ __forceinline__ bool row_live(const FusedArgs& a, uint32_t i) {
 // t: the step the thermal field is drawn for.  src: the spins (SoA) the field is evaluated at; gsrc: where neighbours are gathered from
 // (src itself, or with G3 its AoS3 copy, which then also gives the row's own spin; src is
 // then read only by the four-spin term).
-template <int FMT, int NC, int UNR = 2, bool G3 = false>
+template <int FMT, int NC, int UNR = 2, bool G3 = false, bool LFB = true>
 __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32_t iref, const float* __restrict__ src,
                                           const float* __restrict__ gsrc, const float* sdict, float s[3], float f[3],
                                           uint64_t t) {
@@ -407,8 +433,8 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
     float h[3];
     if constexpr (FMT == FMT_SELL_PACK)
         exch_row_pack_fast<NC, UNR, G3>(a.ex, i, gsrc, sdict, h);
-    else if constexpr (FMT == FMT_SELL_LINE)
-        exch_row_line<NC, G3>(a.ex, i, gsrc, h);
+    else if constexpr (is_line_fmt(FMT))
+        exch_row_line<NC, G3, line_len(FMT), line_open(FMT), LFB>(a.ex, i, gsrc, sdict, h);
     else
         exch_any<FMT, NC, float, float, float>(a.ex, i, src, h);
     if (a.ex.fs_xadj) {
@@ -465,20 +491,35 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
 }
 
 // Stage the coupling dictionary of the packed encoding in LDS (before any early exit).
+// The packed reader's coupling dictionary in LDS.  Packed format: a static array.  Line
+// formats: dynamic LDS, allocated by the launch (line_lds_bytes) only when some slices fall
+// back to packed entries; the condition is the same for every thread of a block (one
+// handle, or one batch member per block), so every thread reaches the barrier.
 #define SLLG_STAGE_DICT(FMT_, NC_, a_)                                              \
-    __shared__ float sdict[FMT_ == FMT_SELL_PACK ? kDictLds : 1];                 \
-    if constexpr (FMT_ == FMT_SELL_PACK) {                                        \
-        const uint32_t m_ = (a_).ex.ndict * NC_;                                  \
-        for (uint32_t k_ = threadIdx.x; k_ < m_; k_ += blockDim.x) sdict[k_] = (a_).ex.dict[k_]; \
-        __syncthreads();                                                          \
+    __shared__ float sdict_s_[FMT_ == FMT_SELL_PACK ? kDictLds : 1];              \
+    extern __shared__ float sdict_d_[];                                           \
+    float* const sdict = is_line_fmt(FMT_) ? sdict_d_ : sdict_s_;                 \
+    if constexpr (FMT_ == FMT_SELL_PACK || is_line_fmt(FMT_)) {                   \
+        if (FMT_ == FMT_SELL_PACK || (a_).ex.lfall) {                             \
+            const uint32_t m_ = (a_).ex.ndict * NC_;                              \
+            for (uint32_t k_ = threadIdx.x; k_ < m_; k_ += blockDim.x) sdict[k_] = (a_).ex.dict[k_]; \
+            __syncthreads();                                                      \
+        }                                                                         \
     }
+
+// Dynamic LDS bytes of a launch of format F (the dictionary, for line encodings with
+// slices read from the packed entries).
+template <int F>
+inline size_t line_lds_bytes(const Exch<float>& e) {
+    return (is_line_fmt(F) && e.lfall) ? kDictLds * sizeof(float) : 0;
+}
 
 // With G3 the kernels gather neighbours (and read the row's own spin) from the AoS3 copy
 // (S3 / Y3, [3 i + c]) and keep it current: the predictor writes Y3 (and Y only when Y is
 // non-null: the four-spin term or a dipolar refresh at the predicted spins reads it), the
 // corrector S3 beside S.
 
-template <int FMT, int NC, int TPB, int UNR, bool G3>
+template <int FMT, int NC, int TPB, int UNR, bool G3, bool LFB>
 __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S,
                                                     const float* __restrict__ S3, float* __restrict__ Y,
                                                     float* __restrict__ Y3, float* __restrict__ A) {
@@ -488,7 +529,7 @@ __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __r
     if (!row_live<FMT>(a, i)) return;
     const uint32_t n = a.n;
     float s[3], f[3];
-    fused_rhs<FMT, NC, UNR, G3>(a, i, iref, S, G3 ? S3 : S, sdict, s, f, a.t);
+    fused_rhs<FMT, NC, UNR, G3, LFB>(a, i, iref, S, G3 ? S3 : S, sdict, s, f, a.t);
     float y[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -502,7 +543,7 @@ __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __r
     if constexpr (G3) reinterpret_cast<Spin3*>(Y3)[i] = Spin3{y[0] * r, y[1] * r, y[2] * r};
 }
 
-template <int FMT, int NC, int TPB, int UNR, bool G3>
+template <int FMT, int NC, int TPB, int UNR, bool G3, bool LFB>
 __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __restrict__ Y,
                                                     const float* __restrict__ Y3, const float* __restrict__ A,
                                                     float* __restrict__ S, float* __restrict__ S3) {
@@ -515,7 +556,7 @@ __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __r
 #pragma unroll
     for (int c = 0; c < 3; ++c) av[c] = A[(size_t)c * n + i];
     float s[3], f[3];
-    fused_rhs<FMT, NC, UNR, G3>(a, i, iref, Y, G3 ? Y3 : Y, sdict, s, f, a.t);
+    fused_rhs<FMT, NC, UNR, G3, LFB>(a, i, iref, Y, G3 ? Y3 : Y, sdict, s, f, a.t);
     float p[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = av[c] + a.half_rdt * f[c];
@@ -566,7 +607,7 @@ __global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restric
     const uint64_t t = bm.t0 + step_off;
     SLLG_STAGE_DICT(FMT, NC, a)
     const uint32_t i = (tile - bm.tile0) * TPB + threadIdx.x;
-    if (i >= a.n) return;
+    if (!row_live<FMT>(a, i)) return;
     const uint32_t n = a.n;
     float s[3], f[3];
     if constexpr (PHASE == 0) {
@@ -754,6 +795,18 @@ static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
             if ((NC_VAL) == 1) { MACRO(FMT_SELL_LINE, 1); }                         \
             else if ((NC_VAL) == 3) { MACRO(FMT_SELL_LINE, 3); }                    \
             else { MACRO(FMT_SELL_LINE, 9); }                                       \
+        } else if ((FMT_VAL) == FMT_SELL_LINE16) {                                  \
+            if ((NC_VAL) == 1) { MACRO(FMT_SELL_LINE16, 1); }                       \
+            else if ((NC_VAL) == 3) { MACRO(FMT_SELL_LINE16, 3); }                  \
+            else { MACRO(FMT_SELL_LINE16, 9); }                                     \
+        } else if ((FMT_VAL) == FMT_SELL_LINE_OPEN) {                               \
+            if ((NC_VAL) == 1) { MACRO(FMT_SELL_LINE_OPEN, 1); }                    \
+            else if ((NC_VAL) == 3) { MACRO(FMT_SELL_LINE_OPEN, 3); }               \
+            else { MACRO(FMT_SELL_LINE_OPEN, 9); }                                  \
+        } else if ((FMT_VAL) == FMT_SELL_LINE16_OPEN) {                             \
+            if ((NC_VAL) == 1) { MACRO(FMT_SELL_LINE16_OPEN, 1); }                  \
+            else if ((NC_VAL) == 3) { MACRO(FMT_SELL_LINE16_OPEN, 3); }             \
+            else { MACRO(FMT_SELL_LINE16_OPEN, 9); }                                \
         } else if ((FMT_VAL) == FMT_SELL_RAW) {                                     \
             if ((NC_VAL) == 1) { MACRO(FMT_SELL_RAW, 1); }                          \
             else if ((NC_VAL) == 3) { MACRO(FMT_SELL_RAW, 3); }                     \
@@ -775,25 +828,49 @@ struct HeunBufs {
     float *S, *S3, *Y, *Y3, *A;
 };
 
-template <int F, int N, int TPB, int UNR>
-static void launch_pred_g(const FusedArgs& a, const HeunBufs& b, uint32_t n, bool g3, hipStream_t st) {
-    if constexpr (F == FMT_SELL_PACK || F == FMT_SELL_LINE) {
+template <int F, int N, int TPB, int UNR, bool LFB>
+static void launch_pred_gf(const FusedArgs& a, const HeunBufs& b, uint32_t n, bool g3, hipStream_t st) {
+    const size_t lds = line_lds_bytes<F>(a.ex);
+    if constexpr (F == FMT_SELL_PACK || is_line_fmt(F)) {
         if (g3) {
-            k_heun_pred<F, N, TPB, UNR, true><<<blocks_for(n, TPB), TPB, 0, st>>>(a, b.S, b.S3, b.Y, b.Y3, b.A);
+            k_heun_pred<F, N, TPB, UNR, true, LFB><<<blocks_for(n, TPB), TPB, lds, st>>>(a, b.S, b.S3, b.Y, b.Y3, b.A);
             return;
         }
     }
-    k_heun_pred<F, N, TPB, UNR, false><<<blocks_for(n, TPB), TPB, 0, st>>>(a, b.S, b.S3, b.Y, b.Y3, b.A);
+    k_heun_pred<F, N, TPB, UNR, false, LFB><<<blocks_for(n, TPB), TPB, lds, st>>>(a, b.S, b.S3, b.Y, b.Y3, b.A);
+}
+template <int F, int N, int TPB, int UNR, bool LFB>
+static void launch_corr_gf(const FusedArgs& a, const HeunBufs& b, uint32_t n, bool g3, hipStream_t st) {
+    const size_t lds = line_lds_bytes<F>(a.ex);
+    if constexpr (F == FMT_SELL_PACK || is_line_fmt(F)) {
+        if (g3) {
+            k_heun_corr<F, N, TPB, UNR, true, LFB><<<blocks_for(n, TPB), TPB, lds, st>>>(a, b.Y, b.Y3, b.A, b.S, b.S3);
+            return;
+        }
+    }
+    k_heun_corr<F, N, TPB, UNR, false, LFB><<<blocks_for(n, TPB), TPB, lds, st>>>(a, b.Y, b.Y3, b.A, b.S, b.S3);
+}
+// Line encodings with slices read from the packed entries run the kernels that carry the
+// packed reader; all others run without it.
+template <int F, int N, int TPB, int UNR>
+static void launch_pred_g(const FusedArgs& a, const HeunBufs& b, uint32_t n, bool g3, hipStream_t st) {
+    if constexpr (is_line_fmt(F)) {
+        if (a.ex.lfall) {
+            launch_pred_gf<F, N, TPB, UNR, true>(a, b, n, g3, st);
+            return;
+        }
+    }
+    launch_pred_gf<F, N, TPB, UNR, false>(a, b, n, g3, st);
 }
 template <int F, int N, int TPB, int UNR>
 static void launch_corr_g(const FusedArgs& a, const HeunBufs& b, uint32_t n, bool g3, hipStream_t st) {
-    if constexpr (F == FMT_SELL_PACK || F == FMT_SELL_LINE) {
-        if (g3) {
-            k_heun_corr<F, N, TPB, UNR, true><<<blocks_for(n, TPB), TPB, 0, st>>>(a, b.Y, b.Y3, b.A, b.S, b.S3);
+    if constexpr (is_line_fmt(F)) {
+        if (a.ex.lfall) {
+            launch_corr_gf<F, N, TPB, UNR, true>(a, b, n, g3, st);
             return;
         }
     }
-    k_heun_corr<F, N, TPB, UNR, false><<<blocks_for(n, TPB), TPB, 0, st>>>(a, b.Y, b.Y3, b.A, b.S, b.S3);
+    launch_corr_gf<F, N, TPB, UNR, false>(a, b, n, g3, st);
 }
 
 template <int F, int N, int TPB>
@@ -824,7 +901,8 @@ static bool cache_thermal(const llg_ctx* c) { return c->tune_thc && c->T > 0.0 &
 
 // The AoS3 gather copy is used for the packed and line encodings when the tuner chose it.
 static bool use_g3(const llg_ctx* c) {
-    return c->tune_g3 && (c->ex32.fmt == FMT_SELL_PACK || c->ex32.fmt == FMT_SELL_LINE);
+    const int f = exec_fmt(c);
+    return c->tune_g3 && (f == FMT_SELL_PACK || is_line_fmt(f));
 }
 
 static int heun_bufs(llg_ctx* c, bool g3, HeunBufs& b) {
@@ -879,7 +957,7 @@ int fused_heun_pred(llg_ctx* c, const StepSched& s) {
         else if (tpb == 512) launch_heun_pred<F, N, 512>(a, b, c->n, unr, g3, c->stream);     \
         else launch_heun_pred<F, N, 256>(a, b, c->n, unr, g3, c->stream);                     \
     } while (0)
-    SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+    SLLG_DISPATCH_FMT_NC(exec_fmt(c), c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
     SLLG_HIP(hipGetLastError());
     return 0;
@@ -898,7 +976,7 @@ int fused_heun_corr(llg_ctx* c, const StepSched& s) {
         else if (tpb == 512) launch_heun_corr<F, N, 512>(a, b, c->n, unr, g3, c->stream);     \
         else launch_heun_corr<F, N, 256>(a, b, c->n, unr, g3, c->stream);                     \
     } while (0)
-    SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+    SLLG_DISPATCH_FMT_NC(exec_fmt(c), c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
     SLLG_HIP(hipGetLastError());
     c->g3_valid = g3;  // S3 now matches S only if the corrector wrote it
@@ -936,24 +1014,24 @@ int fused_rk_stage(llg_ctx* c, const StepSched& s, int m) {
         float* ACC = K[0];
 #define SLLG_L(F, N)                                                                              \
     switch (m) {                                                                                  \
-        case 0: k_rk4<F, N, 0><<<g, b, 0, c->stream>>>(a, Yin, Yout, ACC, S); break;              \
-        case 1: k_rk4<F, N, 1><<<g, b, 0, c->stream>>>(a, Yin, Yout, ACC, S); break;              \
-        case 2: k_rk4<F, N, 2><<<g, b, 0, c->stream>>>(a, Yin, Yout, ACC, S); break;              \
-        default: k_rk4<F, N, 3><<<g, b, 0, c->stream>>>(a, Yin, Yout, ACC, S); break;             \
+        case 0: k_rk4<F, N, 0><<<g, b, line_lds_bytes<F>(a.ex), c->stream>>>(a, Yin, Yout, ACC, S); break;              \
+        case 1: k_rk4<F, N, 1><<<g, b, line_lds_bytes<F>(a.ex), c->stream>>>(a, Yin, Yout, ACC, S); break;              \
+        case 2: k_rk4<F, N, 2><<<g, b, line_lds_bytes<F>(a.ex), c->stream>>>(a, Yin, Yout, ACC, S); break;              \
+        default: k_rk4<F, N, 3><<<g, b, line_lds_bytes<F>(a.ex), c->stream>>>(a, Yin, Yout, ACC, S); break;             \
     }
-        SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+        SLLG_DISPATCH_FMT_NC(exec_fmt(c), c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
     } else {
 #define SLLG_L(F, N)                                                                                    \
     switch (m) {                                                                                        \
-        case 0: k_rk5<F, N, 0><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
-        case 1: k_rk5<F, N, 1><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
-        case 2: k_rk5<F, N, 2><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
-        case 3: k_rk5<F, N, 3><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
-        case 4: k_rk5<F, N, 4><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
-        default: k_rk5<F, N, 5><<<g, b, 0, c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        case 0: k_rk5<F, N, 0><<<g, b, line_lds_bytes<F>(a.ex), c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        case 1: k_rk5<F, N, 1><<<g, b, line_lds_bytes<F>(a.ex), c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        case 2: k_rk5<F, N, 2><<<g, b, line_lds_bytes<F>(a.ex), c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        case 3: k_rk5<F, N, 3><<<g, b, line_lds_bytes<F>(a.ex), c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        case 4: k_rk5<F, N, 4><<<g, b, line_lds_bytes<F>(a.ex), c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
+        default: k_rk5<F, N, 5><<<g, b, line_lds_bytes<F>(a.ex), c->stream>>>(a, Yin, Yout, S, K[0], K[1], K[2], K[3], K[4]); break; \
     }
-        SLLG_DISPATCH_FMT_NC(c->ex32.fmt, c->ex32.ncomp, SLLG_L);
+        SLLG_DISPATCH_FMT_NC(exec_fmt(c), c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
     }
     SLLG_HIP(hipGetLastError());
@@ -975,7 +1053,7 @@ int fused_detfield(llg_ctx* c, const float* Sp, double* out) {
     // the FP64 view of the couplings is the packed or raw encoding (never line mode)
 #define SLLG_L(F, N)                                                                  \
     do {                                                                              \
-        if constexpr (F != FMT_SELL_LINE) k_fused_detfield<F, N><<<g, b, 0, c->stream>>>(a, Sp, out); \
+        if constexpr (!is_line_fmt(F)) k_fused_detfield<F, N><<<g, b, 0, c->stream>>>(a, Sp, out); \
     } while (0)
     SLLG_DISPATCH_FMT_NC(c->ex64.fmt, c->ex64.ncomp, SLLG_L);
 #undef SLLG_L
@@ -1018,9 +1096,11 @@ int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<ui
 template <int PHASE, int F, int N>
 static void launch_batch_tpb(hipStream_t s, const BatchMember* m, const uint32_t* map, uint32_t ntiles,
                              uint32_t step_off, uint32_t tpb, int xcd) {
-    if (tpb == 128) k_heun_batch<PHASE, F, N, 128><<<ntiles, 128, 0, s>>>(m, map, step_off, xcd);
-    else if (tpb == 512) k_heun_batch<PHASE, F, N, 512><<<ntiles, 512, 0, s>>>(m, map, step_off, xcd);
-    else k_heun_batch<PHASE, F, N, 256><<<ntiles, 256, 0, s>>>(m, map, step_off, xcd);
+    // members of a line format may have slices read from packed entries: reserve the dictionary
+    const size_t lds = is_line_fmt(F) ? kDictLds * sizeof(float) : 0;
+    if (tpb == 128) k_heun_batch<PHASE, F, N, 128><<<ntiles, 128, lds, s>>>(m, map, step_off, xcd);
+    else if (tpb == 512) k_heun_batch<PHASE, F, N, 512><<<ntiles, 512, lds, s>>>(m, map, step_off, xcd);
+    else k_heun_batch<PHASE, F, N, 256><<<ntiles, 256, lds, s>>>(m, map, step_off, xcd);
 }
 
 int batch_heun_launch(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,

@@ -91,7 +91,7 @@ class Info(C.Structure):
 class TuneResult(C.Structure):
     _fields_ = [("tpb", C.c_int), ("xcd", C.c_int), ("ms_per_step", C.c_double),
                 ("ms_per_step_default", C.c_double), ("candidates", C.c_int), ("from_cache", C.c_int),
-                ("unroll", C.c_int), ("thermal_cache", C.c_int), ("gather_copy", C.c_int), ("pad_", C.c_int)]
+                ("unroll", C.c_int), ("thermal_cache", C.c_int), ("gather_copy", C.c_int), ("line_mode", C.c_int)]
 
 
 class UnitCell(C.Structure):
@@ -148,7 +148,7 @@ def lib():
         L.llg_autotune.argtypes = [P, C.c_uint32, C.POINTER(TuneResult)]
         L.llg_set_bins.argtypes = [P, C.c_uint32, C.POINTER(C.c_uint32), dp]
         L.llg_reduce_bins.argtypes = [P, dp]
-        L.llg_set_launch.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.llg_set_launch.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         up = C.POINTER(C.c_uint32)
         L.llg_set_sq.argtypes = [P, up, up, C.c_uint32, up, C.c_int, dp, dp, C.c_uint32, up]
         L.llg_sample_sq.argtypes = [P, dp]
@@ -577,14 +577,14 @@ class Context:
         r = TuneResult()
         _check(self.L.llg_autotune(self.h, int(nsteps), C.byref(r)), self.L)
         return {"tpb": r.tpb, "xcd": r.xcd, "unroll": r.unroll, "thermal_cache": r.thermal_cache,
-                "gather_copy": r.gather_copy,
+                "gather_copy": r.gather_copy, "line_mode": r.line_mode,
                 "ms_per_step": r.ms_per_step,
                 "ms_per_step_default": r.ms_per_step_default, "candidates": r.candidates,
                 "from_cache": bool(r.from_cache)}
 
-    def set_launch(self, tpb, xcd=1, unroll=2, thermal_cache=1, gather_copy=1):
-        _check(self.L.llg_set_launch(self.h, int(tpb), int(xcd), int(unroll), int(thermal_cache), int(gather_copy)),
-               self.L)
+    def set_launch(self, tpb, xcd=1, unroll=2, thermal_cache=1, gather_copy=1, line_mode=1):
+        _check(self.L.llg_set_launch(self.h, int(tpb), int(xcd), int(unroll), int(thermal_cache), int(gather_copy),
+                                     int(line_mode)), self.L)
 
     def info(self):
         i = Info()

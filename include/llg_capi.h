@@ -87,7 +87,7 @@ typedef enum llg_noise {
 #define LLG_FLAG_NO_XCD_REMAP 0x100u        /* tiles in launch order, no per-XCD row ranges */
 #define LLG_FLAG_NO_SUBLATTICE_ORDER 0x200u /* keep the reference spin order on the device */
 #define LLG_FLAG_NO_SLICE_SHARING 0x400u    /* store every 64-row slice's entries, even repeats (no line mode) */
-#define LLG_FLAG_NO_LINE_MODE 0x800u        /* keep per-lane gathers where slices are periodic lines */
+#define LLG_FLAG_NO_LINE_MODE 0x800u        /* no line encoding: per-lane gathers only */
 
 /* Per-species (unit-cell atom) constants; reference geom::ucm (inc/unitcell.h) and
  * the per-spin arrays filled in src/geom.cpp:239-247 and src/llg.cpp:591-592,626-631. */
@@ -299,25 +299,27 @@ int llg_profile_kernels(llg_ctx* ctx, uint64_t t0, uint32_t nsteps, int max_kind
 /* Launch-shape autotuner (no reference counterpart): times the FP32 Heun kernel pair for
  * every candidate block size (256, 512, 1024 threads) with and without the XCD-aware
  * tile remap, for the packed encoding 2 or 4 coupling chunks loaded ahead per wave, and
- * with the thermal field drawn in both passes or handed from predictor to corrector, and
+ * with the thermal field drawn in both passes or handed from predictor to corrector,
  * (packed and line encodings) with neighbours gathered from the SoA spins or from an AoS3
- * copy the kernels keep beside them (bit-identical either way), on this handle's own
- * couplings, for nsteps steps each; keeps the fastest.
+ * copy the kernels keep beside them (bit-identical either way), and (line encodings) with
+ * the line reader or the packed entries kept beside the line stream (same couplings, sums
+ * in another order), on this handle's own couplings, for nsteps steps each; keeps the
+ * fastest.
  * The spin state is restored afterwards.  Results are cached per (device, encoding, n,
  * entries) in the process and, if the environment variable SLLG_TUNE_FILE names a file,
  * across processes.  Other paths return LLG_OK with the defaults. */
 typedef struct llg_tune_result {
     int tpb, xcd;
     double ms_per_step;     /* chosen */
-    double ms_per_step_default; /* 256 threads, remap on, 2 chunks ahead, hand-off on, gather copy on */
+    double ms_per_step_default; /* 256 threads, remap on, 2 chunks ahead, hand-off on, gather copy on, line reader */
     int candidates, from_cache;
     int unroll;
     int thermal_cache;  /* 1: the predictor hands its thermal field to the corrector */
     int gather_copy;    /* 1: neighbours are gathered from an AoS3 copy of the spins */
-    int pad_;
+    int line_mode;      /* 1: line reader (line encodings), 0: the packed entries kept beside it */
 } llg_tune_result;
 int llg_autotune(llg_ctx* ctx, uint32_t nsteps, llg_tune_result* out);
-int llg_set_launch(llg_ctx* ctx, int tpb, int xcd, int unroll, int thermal_cache, int gather_copy);
+int llg_set_launch(llg_ctx* ctx, int tpb, int xcd, int unroll, int thermal_cache, int gather_copy, int line_mode);
 
 /* Algorithmic HBM bytes per spin-update of the configured path, split into the
  * kernels of one step, and the device bytes the handle holds. */

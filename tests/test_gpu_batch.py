@@ -40,6 +40,30 @@ def test_batch_member_equals_single_handle(gpu):
         assert B.get_step(k) == 100
 
 
+def test_batch_line_members_equal_single_handle(gpu):
+    """Members whose couplings use the line reader (64-cell lines, 16-cell lines with packed
+    boundary slices, open 16-cell lines) give the single handle's trajectory in a batch."""
+    mem = []
+    a, sa, _ = W.sc_ferro(precision="fp32", dims=(2, 2, 64), seed=21)
+    b, sb, _ = W.sc_ferro(precision="fp32", dims=(2, 32, 16), seed=22)
+    c, sc, _ = W.sc_ferro(precision="fp32", dims=(2, 32, 16), pbc=(1, 1, 0), seed=23)  # open lines
+    for s, s0, T in ((a, sa, 200.0), (b, sb, 300.0), (c, sc, 100.0)):
+        mem.append((s, s0, T, (0.0, 0.0, 0.3)))
+    bt = SB.Batch()
+    ids = [bt.add(s, s0, T, B) for s, s0, T, B in mem]
+    bt.step(25)
+    for k, (s, s0, T, B) in zip(ids, mem):
+        g = sllg.Context(s)
+        assert "line" in g.info()["exch_encoding"]
+        g.set_spins(s0)
+        g.set_temperature(T)
+        g.set_applied(np.array(B))
+        g.step(0, 25)
+        assert float(np.abs(bt.get_spins(k) - g.get_spins()).max()) <= 1e-6
+        g.close()
+    bt.close()
+
+
 def test_batch_alone_vs_mix_bitwise(gpu):
     ms = _members()
     solo = SB.Batch()

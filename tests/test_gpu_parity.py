@@ -388,19 +388,35 @@ def test_fourspin_sublattice_order(gpu):
     assert _maxdiff(g.field(), o.field()) <= 1e-4 * np.abs(o.field()).max()
 
 
-@pytest.mark.parametrize("case", ["sc", "fcc", "tensor"])
+@pytest.mark.parametrize("case", ["sc", "fcc", "diag", "tensor", "sc16", "fcc16", "sc64open", "sc16open"])
 def test_line_mode(gpu, case):
-    """Slices that are periodic 64-cell lines use the line reader (aligned line loads, wave
-    rotations for the +-1 neighbours, scalar coupling stream).  Same physics as the packed
-    reader to FP32 rounding (entries are summed per line, not in stored order) and as the
-    oracle; energies agree."""
-    if case == "sc":
+    """Slices made of periodic 64-cell lines (or four 16-cell lines) use the line reader
+    (window loads, rotations across the wave or within 16-lane rows for the +-1
+    neighbours, scalar coupling stream); 16-cell lattices keep the packed entries for the
+    slices at the periodic y boundary.  Same physics as the packed reader to FP32 rounding
+    (entries are summed per line, not in stored order) and as the oracle; energies agree."""
+    want = "line64"
+    if case == "sc64open":  # open along the lines: zero-filling shifts at the line ends
+        sysd, s0, _ = W.sc_ferro(precision="fp32", dims=(4, 4, 64), pbc=(1, 1, 0))
+        want = "line64o"
+    elif case == "sc16open":
+        sysd, s0, _ = W.sc_ferro(precision="fp32", dims=(2, 32, 16), pbc=(1, 1, 0))
+        want = "line16o"
+    elif case == "sc16":
+        sysd, s0, _ = W.sc_ferro(precision="fp32", dims=(2, 32, 16))
+        want = "line16"
+    elif case == "fcc16":
+        sysd, s0 = W.fcc_3shell(precision="fp32", dims=(2, 32, 16))
+        want = "line16"
+    elif case == "sc":
         sysd, s0, _ = W.sc_ferro(precision="fp32", dims=(4, 4, 64))
     elif case == "fcc":
         sysd, s0 = W.fcc_3shell(precision="fp32", dims=(2, 3, 64))
         sysd.species = [dict(mu=2.22 if t != 2 else 1.5, lambda_=0.05, sublattice=t) for t in range(4)]
         sysd.spin_species = (np.arange(sysd.n) % 4).astype(np.uint32)
         sysd.n_sublattice = 4
+    elif case == "diag":  # three diagonal components per entry (groups of 2 lines)
+        sysd, s0 = tensor_csr_system(precision="fp32", ncomp=3, dims=(2, 3, 64))
     else:
         sysd, s0 = tensor_csr_system(precision="fp32", ncomp=9, dims=(3, 2, 64))
     res, encs = [], []
@@ -414,8 +430,19 @@ def test_line_mode(gpu, case):
         g.step(0, 40)
         res.append((g.get_spins(), g.reduce()[1], g.field()))
     sysd.flags = 0
+    # the line encoding keeps the packed entries: with the line reader off, the handle
+    # reproduces the packed-only handle bit for bit
+    g = sllg.Context(sysd)
+    g.set_launch(256, 1, 2, 1, 1, 0)
+    g.set_spins(s0)
+    g.set_temperature(150.0)
+    g.set_applied(np.array([0.1, 0.0, 0.4]))
+    g.step(0, 40)
+    assert np.array_equal(g.get_spins(), res[1][0])
     print(case, encs)
-    assert "line" in encs[0] and "line" not in encs[1]
+    assert want in encs[0] and "line" not in encs[1]
+    if want.startswith("line16"):
+        assert "-packed0" not in encs[0]  # boundary slices read from the packed entries
     assert _maxdiff(res[0][0], res[1][0]) <= 2e-5
     assert abs(res[0][1] - res[1][1]) <= 1e-5 * abs(res[1][1])
     assert _maxdiff(res[0][2], res[1][2]) <= 1e-5 * np.abs(res[1][2]).max()
