@@ -11,7 +11,7 @@ Also measured in the same run: the FP64 plain path (reference kernel structure, 
 the per-kernel mean launch time from HIP events on the handle's stream (roofline), and
 the FP64 oracle on one host core over a bounded sample (cpu_baseline).
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|copy-probe]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|copy-probe|accuracy|rk4|batched|serve|fft-exchange]
 """
 import argparse
 import json
@@ -336,6 +336,36 @@ def serve_mode(args):
     print(json.dumps(out), flush=True)
 
 
+def fft_exchange(args):
+    """FFT exchange (the reference's Exchange_method "fft"): open 128x128x16 sc film, NN
+    exchange and dipolar in one species-resolved convolution evaluated every stage, FP32
+    Heun, against the same physics as CSR exchange plus the dipolar convolution refreshed
+    every stage; spin-updates/s of both and their trajectory difference at T = 0."""
+    import sllg
+    from sllg import workloads as W
+
+    out = {"mode": "fft_exchange"}
+    res = {}
+    for name, csr in (("fft_exchange", False), ("csr_plus_dipolar", True)):
+        sysd, s0 = W.sc_film_fftx(Lx=args.film[0], Ly=args.film[1], Lz=args.film[2], precision="fp32", as_csr=csr)
+        g = sllg.Context(sysd)
+        g.set_spins(s0)
+        g.set_temperature(0.0)
+        g.step(0, 20)
+        g.sync()
+        t = time.perf_counter()
+        g.step(20, args.steps)
+        g.sync()
+        sec = time.perf_counter() - t
+        res[name] = g.get_spins()
+        out[name] = {"spin_updates_per_s": sysd.n * args.steps / sec, "ms_per_step": sec * 1e3 / args.steps,
+                     "kernels_ms": g.profile_kernels(20 + args.steps, 10)}
+        g.close()
+    out["n_spins"] = int(np.prod(args.film))
+    out["max_abs_dS_after_steps"] = float(np.abs(res["fft_exchange"] - res["csr_plus_dipolar"]).max())
+    print(json.dumps(out), flush=True)
+
+
 def copy_probe(args):
     """Device copy of a known size, for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
     import torch
@@ -356,7 +386,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--mode", default="headline",
-                    choices=["headline", "copy-probe", "accuracy", "rk4", "batched", "serve"])
+                    choices=["headline", "copy-probe", "accuracy", "rk4", "batched", "serve", "fft-exchange"])
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--T", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -383,7 +413,7 @@ def main():
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     modes = {"copy-probe": copy_probe, "accuracy": accuracy, "rk4": rk4_dmi, "batched": batched,
-             "serve": serve_mode, "headline": headline}
+             "serve": serve_mode, "headline": headline, "fft-exchange": fft_exchange}
     modes[args.mode](args)
 
 

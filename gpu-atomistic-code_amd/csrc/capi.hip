@@ -425,14 +425,25 @@ static int create_impl(const llg_desc* d, const std::shared_ptr<ExchStore>& shar
         SLLG_TRY(c->sub.alloc(n));
         SLLG_HIP(hipMemcpy(c->sub.p, u8.data(), n, hipMemcpyHostToDevice));
     }
-    if (d->dip.enabled) {
+    const bool fftx = d->fftx.n_int > 0;
+    if (fftx) {
+        if (d->exch.format != LLG_EXCH_NONE)
+            SLLG_FAIL(LLG_E_INVALID, "FFT exchange replaces the exchange SpMV: exch.format must be LLG_EXCH_NONE");
+        if (!d->fftx.ints) SLLG_FAIL(LLG_E_INVALID, "FFT exchange without interactions");
+        if (!d->dip.mesh) SLLG_FAIL(LLG_E_INVALID, "FFT exchange needs the mesh geometry of desc.dip (dim, nk, abc, mesh)");
+        for (uint32_t q = 0; q < d->fftx.n_int; ++q)
+            if (d->fftx.ints[q].s1 >= c->ns || d->fftx.ints[q].s2 >= c->ns)
+                SLLG_FAIL(LLG_E_INVALID, "FFT exchange interaction %u: species out of range", q);
+    }
+    if (d->dip.enabled || fftx) {
         if (!d->dip.mesh) SLLG_FAIL(LLG_E_INVALID, "dipolar enabled without mesh coordinates");
         c->dip = true;
-        c->dip_every_stage = d->dip.every_stage ? 1 : 0;
-        c->dip_refresh = d->dip.refresh ? d->dip.refresh : d->update;
+        // FFT exchange: the convolution is the exchange field, evaluated at every stage
+        c->dip_every_stage = (fftx || d->dip.every_stage) ? 1 : 0;
+        c->dip_refresh = fftx ? 1 : (d->dip.refresh ? d->dip.refresh : d->update);
         SLLG_TRY(c->DEM.alloc(v3));
         SLLG_HIP(hipMemset(c->DEM.p, 0, v3));
-        SLLG_TRY(dipolar_create(c, d->dip, d->spin_species));
+        SLLG_TRY(dipolar_create(c, d->dip, d->spin_species, fftx ? &d->fftx : nullptr, d->dip.enabled != 0));
     }
     SLLG_HIP(hipMemset(c->STG.p, 0, v3));
     // initial state: all spins along +z
@@ -562,6 +573,9 @@ int llg_sync(llg_ctx* c) {
 static int field_at(llg_ctx* c, const void* Sp, double* hx, double* hy, double* hz) {
     DevBuf out;
     SLLG_TRY(out.alloc(3 * (size_t)c->n * 8));
+    // FFT exchange is the exchange field: evaluate it at these spins (every step refreshes it
+    // before use, so the buffer may be overwritten); a dipolar field keeps its refresh schedule
+    if (c->dip && dipolar_is_exchange(c)) SLLG_TRY(dipolar_refresh(c, Sp, c->DEM.p));
     if (c->prec == LLG_FP64_PLAIN)
         SLLG_TRY(plain_detfield(c, (const double*)Sp, out.as<double>()));
     else

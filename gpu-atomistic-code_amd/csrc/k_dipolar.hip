@@ -16,6 +16,14 @@
 // (empty points stay zero, so no per-refresh zero fill), R2C (3 batched transforms),
 // H_k = N_k S_k (6 real x complex per k-point), C2R, gather / P.  FP32 path: FP32 grids
 // and transforms with the FP64-built kernel cast to FP32; FP64 plain path: FP64 throughout.
+//
+// FFT exchange (the reference's exchm = 0, src/exch_intmat.cpp:28-160, src/intmat.cpp:29-262,
+// applied per stage src/cuda.cu:47-60,162-175): one grid per species s and component, the
+// kernel N(s1, s2, a, b, r) complex (exchange tensors need not be symmetric, so N is not
+// even in r): J_ab / (mu_s1 mu_B) at r = -d for the interaction (s1, s2, d, J), plus the
+// dipolar 1e-7 (3 e_a e_b - delta_ab) mu_s2 mu_B / r^3 when enabled.  Each evaluation:
+// scatter S of every spin into its species' grids, R2C (3 nms transforms), H(s1, a) =
+// sum_{s2, b} N(s1, s2, a, b) S(s2, b) per k-point, C2R, gather / P.
 #include <hipfft/hipfft.h>
 
 #include <cmath>
@@ -41,6 +49,12 @@ struct DipolarPlan {
     DevBuf hgrid;  // [3][P] real output grid
     hipfftHandle fwd = 0, inv = 0;
     bool have_fwd = false, have_inv = false;
+    // FFT exchange (general species-resolved kernel)
+    bool general = false;
+    uint32_t nms = 1;
+    DevBuf sp;     // u32 species of each spin (device order)
+    DevBuf Nkc;    // [nms][nms][9][Ph] complex, path precision
+    DevBuf hspec;  // [nms][3][Ph] complex
     ~DipolarPlan() {
         if (have_fwd) hipfftDestroy(fwd);
         if (have_inv) hipfftDestroy(inv);
@@ -111,6 +125,68 @@ __global__ __launch_bounds__(kTPB) void k_dip_gather(uint32_t n, size_t P, R inv
     out[2 * (size_t)n + i] = hgrid[2 * P + g] * invP;
 }
 
+// FFT exchange: S of spin i into the grids of its species, [s][c][P].
+template <class R>
+__global__ __launch_bounds__(kTPB) void k_fx_scatter(uint32_t n, size_t P, const uint32_t* __restrict__ lin,
+                                                     const uint32_t* __restrict__ sp, const R* __restrict__ S,
+                                                     R* __restrict__ grid) {
+    const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
+    if (i >= n) return;
+    R* const gs = grid + (size_t)sp[i] * 3 * P;
+    const size_t g = lin[i];
+    gs[g] = S[i];
+    gs[P + g] = S[(size_t)n + i];
+    gs[2 * P + g] = S[2 * (size_t)n + i];
+}
+
+// H(s1, a)(k) = sum_{s2, b} N(s1, s2, a, b)(k) S(s2, b)(k), complex.
+template <class R, class C>
+__global__ __launch_bounds__(kTPB) void k_fx_mult(size_t Ph, uint32_t nms, const C* __restrict__ Nk,
+                                                  const C* __restrict__ spec, C* __restrict__ hspec) {
+    const size_t k = (size_t)blockIdx.x * kTPB + threadIdx.x;
+    if (k >= Ph) return;
+    for (uint32_t s1 = 0; s1 < nms; ++s1)
+        for (int a = 0; a < 3; ++a) {
+            R hr = 0, hi = 0;
+            for (uint32_t s2 = 0; s2 < nms; ++s2)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const C nv = Nk[(((size_t)s1 * nms + s2) * 9 + a * 3 + b) * Ph + k];
+                    const C sv = spec[((size_t)s2 * 3 + b) * Ph + k];
+                    hr += nv.x * sv.x - nv.y * sv.y;
+                    hi += nv.x * sv.y + nv.y * sv.x;
+                }
+            C h;
+            h.x = hr;
+            h.y = hi;
+            hspec[((size_t)s1 * 3 + a) * Ph + k] = h;
+        }
+}
+
+template <class R>
+__global__ __launch_bounds__(kTPB) void k_fx_gather(uint32_t n, size_t P, R invP, const uint32_t* __restrict__ lin,
+                                                    const uint32_t* __restrict__ sp, const R* __restrict__ hgrid,
+                                                    R* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kTPB + threadIdx.x;
+    if (i >= n) return;
+    const R* const hs = hgrid + (size_t)sp[i] * 3 * P;
+    const size_t g = lin[i];
+    out[i] = hs[g] * invP;
+    out[(size_t)n + i] = hs[P + g] * invP;
+    out[2 * (size_t)n + i] = hs[2 * P + g] * invP;
+}
+
+template <class C>
+__global__ __launch_bounds__(kTPB) void k_take_complex(size_t m, const hipfftDoubleComplex* __restrict__ in,
+                                                       C* __restrict__ out) {
+    const size_t k = (size_t)blockIdx.x * kTPB + threadIdx.x;
+    if (k >= m) return;
+    C v;
+    v.x = in[k].x;
+    v.y = in[k].y;
+    out[k] = v;
+}
+
 template <class R>
 __global__ __launch_bounds__(kTPB) void k_take_real(size_t m, const hipfftDoubleComplex* __restrict__ in, R* __restrict__ out) {
     const size_t k = (size_t)blockIdx.x * kTPB + threadIdx.x;
@@ -150,7 +226,131 @@ static void fill_kernel(const int G[3], const uint32_t dim[3], const uint32_t nk
     (void)dim;
 }
 
-int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_species) {
+// FP64 FFT-exchange kernel on the padded grid, [(s1 nms + s2) 9 + 3 a + b][P].
+static void fill_general(const int G[3], const uint32_t nk[3], const double abc[3], uint32_t nms,
+                         const std::vector<double>& mu_sp, const llg_fftexch& fx, bool dip, std::vector<double>& N) {
+    const size_t P = (size_t)G[0] * G[1] * G[2];
+    N.assign((size_t)nms * nms * 9 * P, 0.0);
+    const int D[3] = {G[0] / 2, G[1] / 2, G[2] / 2};
+    auto at = [&](uint32_t s1, uint32_t s2, int ab, size_t lin) -> double& {
+        return N[(((size_t)s1 * nms + s2) * 9 + ab) * P + lin];
+    };
+    // exchange: neighbour at mesh(i) + d, so the convolution index x - y is -d; an offset
+    // of D_a or more along an axis never joins two spins of the system
+    for (uint32_t q = 0; q < fx.n_int; ++q) {
+        const llg_fft_interaction& it = fx.ints[q];
+        int g[3];
+        bool far = false;
+        for (int a = 0; a < 3; ++a) {
+            if (std::abs(it.d[a]) >= D[a]) far = true;
+            g[a] = ((-it.d[a]) % G[a] + G[a]) % G[a];
+        }
+        if (far) continue;
+        const size_t lin = ((size_t)g[0] * G[1] + g[1]) * G[2] + g[2];
+        const double inv = 1.0 / (mu_sp[it.s1] * LLG_MU_B);
+        for (int ab = 0; ab < 9; ++ab) at(it.s1, it.s2, ab, lin) += it.J[ab] * inv;
+    }
+    if (!dip) return;
+    for (int i = 0; i < G[0]; ++i)
+        for (int j = 0; j < G[1]; ++j)
+            for (int k = 0; k < G[2]; ++k) {
+                if (i == 0 && j == 0 && k == 0) continue;
+                const int g[3] = {i, j, k};
+                bool skip = false;
+                double r[3];
+                for (int a = 0; a < 3; ++a) {
+                    if (g[a] == D[a] && D[a] > 0) { skip = true; break; }
+                    const int dd = g[a] < D[a] ? g[a] : g[a] - G[a];
+                    r[a] = static_cast<double>(dd) * abc[a] / static_cast<double>(nk[a]);
+                }
+                if (skip) continue;
+                const double mr = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                const double oo = 1. / (mr * mr * mr);
+                const double e[3] = {r[0] / mr, r[1] / mr, r[2] / mr};
+                const size_t lin = ((size_t)i * G[1] + j) * G[2] + k;
+                for (uint32_t s2 = 0; s2 < nms; ++s2) {
+                    const double m2 = mu_sp[s2] * LLG_MU_B;
+                    for (int a = 0; a < 3; ++a)
+                        for (int b = 0; b < 3; ++b) {
+                            const double v = 1e-7 * ((3.0 * e[a] * e[b]) - (a == b ? 1.0 : 0.0)) * oo * m2;
+                            for (uint32_t s1 = 0; s1 < nms; ++s1) at(s1, s2, a * 3 + b, lin) += v;
+                        }
+                }
+            }
+}
+
+// FFT exchange plan: species grids, general complex kernel, transforms.
+static int fftexch_create(llg_ctx* c, DipolarPlan* p, const llg_dipolar& d, const uint32_t* spin_species,
+                          const llg_fftexch& fx, bool dip) {
+    p->general = true;
+    p->nms = c->ns;
+    const uint32_t n = c->n, nms = p->nms;
+    std::vector<uint32_t> lin(n), sp(n);
+    std::vector<uint8_t> used((size_t)nms * p->P, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t m[3];
+        for (int a = 0; a < 3; ++a) {
+            m[a] = d.mesh[(size_t)i * 3 + a];
+            if (m[a] >= d.dim[a] * d.nk[a]) SLLG_FAIL(LLG_E_INVALID, "mesh coordinate of spin %u outside dim*nk", i);
+        }
+        const uint32_t s = spin_species ? spin_species[i] : (i % c->ns);
+        const size_t g = ((size_t)m[0] * p->G[1] + m[1]) * p->G[2] + m[2];
+        if (used[(size_t)s * p->P + g]) SLLG_FAIL(LLG_E_INVALID, "two spins of species %u share a mesh point (spin %u)", s, i);
+        used[(size_t)s * p->P + g] = 1;
+        lin[c->perm.to_dev(i)] = (uint32_t)g;
+        sp[c->perm.to_dev(i)] = s;
+    }
+    SLLG_TRY(p->lin.alloc(n * sizeof(uint32_t)));
+    SLLG_HIP(hipMemcpy(p->lin.p, lin.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    SLLG_TRY(p->sp.alloc(n * sizeof(uint32_t)));
+    SLLG_HIP(hipMemcpy(p->sp.p, sp.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    // kernel: FP64 real space -> FP64 D2Z on the device -> path precision complex
+    std::vector<double> N;
+    fill_general(p->G, d.nk, d.abc, nms, c->mu_sp, fx, dip, N);
+    const int howmany = (int)(nms * nms * 9);
+    const size_t rs = p->f64 ? sizeof(double) : sizeof(float);
+    {
+        DevBuf dN, dNk;
+        SLLG_TRY(dN.alloc(N.size() * sizeof(double)));
+        SLLG_TRY(dNk.alloc((size_t)howmany * p->Ph * sizeof(hipfftDoubleComplex)));
+        SLLG_HIP(hipMemcpy(dN.p, N.data(), N.size() * sizeof(double), hipMemcpyHostToDevice));
+        hipfftHandle h;
+        int dims[3] = {p->G[0], p->G[1], p->G[2]};
+        SLLG_FFT(hipfftPlanMany(&h, 3, dims, nullptr, 1, (int)p->P, nullptr, 1, (int)p->Ph, HIPFFT_D2Z, howmany));
+        hipfftResult r = hipfftExecD2Z(h, dN.as<double>(), dNk.as<hipfftDoubleComplex>());
+        hipfftDestroy(h);
+        if (r != HIPFFT_SUCCESS) SLLG_FAIL(LLG_E_DEVICE, "FFT-exchange kernel D2Z failed: %d", (int)r);
+        const size_t m = (size_t)howmany * p->Ph;
+        SLLG_TRY(p->Nkc.alloc(m * 2 * rs));
+        if (p->f64)
+            k_take_complex<hipfftDoubleComplex><<<blocks_for(m, kTPB), kTPB>>>(m, dNk.as<hipfftDoubleComplex>(),
+                                                                              p->Nkc.as<hipfftDoubleComplex>());
+        else
+            k_take_complex<hipfftComplex><<<blocks_for(m, kTPB), kTPB>>>(m, dNk.as<hipfftDoubleComplex>(),
+                                                                        p->Nkc.as<hipfftComplex>());
+        SLLG_HIP(hipGetLastError());
+        SLLG_HIP(hipDeviceSynchronize());
+    }
+    SLLG_TRY(p->grid.alloc((size_t)3 * nms * p->P * rs));
+    SLLG_TRY(p->hgrid.alloc((size_t)3 * nms * p->P * rs));
+    SLLG_TRY(p->spec.alloc((size_t)3 * nms * p->Ph * 2 * rs));
+    SLLG_TRY(p->hspec.alloc((size_t)3 * nms * p->Ph * 2 * rs));
+    SLLG_HIP(hipMemset(p->grid.p, 0, (size_t)3 * nms * p->P * rs));
+    int dims[3] = {p->G[0], p->G[1], p->G[2]};
+    const int nb = (int)(3 * nms);
+    SLLG_FFT(hipfftPlanMany(&p->fwd, 3, dims, nullptr, 1, (int)p->P, nullptr, 1, (int)p->Ph,
+                            p->f64 ? HIPFFT_D2Z : HIPFFT_R2C, nb));
+    p->have_fwd = true;
+    SLLG_FFT(hipfftPlanMany(&p->inv, 3, dims, nullptr, 1, (int)p->Ph, nullptr, 1, (int)p->P,
+                            p->f64 ? HIPFFT_Z2D : HIPFFT_C2R, nb));
+    p->have_inv = true;
+    SLLG_FFT(hipfftSetStream(p->fwd, c->stream));
+    SLLG_FFT(hipfftSetStream(p->inv, c->stream));
+    return 0;
+}
+
+int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_species, const llg_fftexch* fx,
+                   bool dip_terms) {
     DipolarPlan* p = new DipolarPlan();
     c->dplan = p;
     p->f64 = (c->prec == LLG_FP64_PLAIN);
@@ -160,6 +360,7 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
     }
     p->P = (size_t)p->G[0] * p->G[1] * p->G[2];
     p->Ph = (size_t)p->G[0] * p->G[1] * (p->G[2] / 2 + 1);
+    if (fx) return fftexch_create(c, p, d, spin_species, *fx, dip_terms);
     const uint32_t n = c->n;
     // grid point of each spin, indexed in the handle's device order
     std::vector<uint32_t> lin(n);
@@ -249,6 +450,8 @@ int dipolar_set_stream(llg_ctx* c) {
     return 0;
 }
 
+bool dipolar_is_exchange(const llg_ctx* c) { return c->dplan && c->dplan->general; }
+
 void dipolar_destroy(llg_ctx* c) {
     delete c->dplan;
     c->dplan = nullptr;
@@ -259,6 +462,35 @@ int dipolar_refresh(llg_ctx* c, const void* S, void* out) {
     if (!p) SLLG_FAIL(LLG_E_STATE, "dipolar plan missing");
     const uint32_t n = c->n;
     const dim3 gn(blocks_for(n, kTPB)), gk(blocks_for(p->Ph, kTPB)), b(kTPB);
+    if (p->general) {
+        const uint32_t* lin = p->lin.as<uint32_t>();
+        const uint32_t* sp = p->sp.as<uint32_t>();
+        if (p->f64) {
+            k_fx_scatter<double><<<gn, b, 0, c->stream>>>(n, p->P, lin, sp, (const double*)S, p->grid.as<double>());
+            SLLG_HIP(hipGetLastError());
+            SLLG_FFT(hipfftExecD2Z(p->fwd, p->grid.as<double>(), p->spec.as<hipfftDoubleComplex>()));
+            k_fx_mult<double, hipfftDoubleComplex><<<gk, b, 0, c->stream>>>(
+                p->Ph, p->nms, p->Nkc.as<hipfftDoubleComplex>(), p->spec.as<hipfftDoubleComplex>(),
+                p->hspec.as<hipfftDoubleComplex>());
+            SLLG_HIP(hipGetLastError());
+            SLLG_FFT(hipfftExecZ2D(p->inv, p->hspec.as<hipfftDoubleComplex>(), p->hgrid.as<double>()));
+            k_fx_gather<double><<<gn, b, 0, c->stream>>>(n, p->P, 1.0 / (double)p->P, lin, sp, p->hgrid.as<double>(),
+                                                         (double*)out);
+        } else {
+            k_fx_scatter<float><<<gn, b, 0, c->stream>>>(n, p->P, lin, sp, (const float*)S, p->grid.as<float>());
+            SLLG_HIP(hipGetLastError());
+            SLLG_FFT(hipfftExecR2C(p->fwd, p->grid.as<float>(), p->spec.as<hipfftComplex>()));
+            k_fx_mult<float, hipfftComplex><<<gk, b, 0, c->stream>>>(p->Ph, p->nms, p->Nkc.as<hipfftComplex>(),
+                                                                     p->spec.as<hipfftComplex>(),
+                                                                     p->hspec.as<hipfftComplex>());
+            SLLG_HIP(hipGetLastError());
+            SLLG_FFT(hipfftExecC2R(p->inv, p->hspec.as<hipfftComplex>(), p->hgrid.as<float>()));
+            k_fx_gather<float><<<gn, b, 0, c->stream>>>(n, p->P, 1.0f / (float)p->P, lin, sp, p->hgrid.as<float>(),
+                                                        (float*)out);
+        }
+        SLLG_HIP(hipGetLastError());
+        return 0;
+    }
     if (p->f64) {
         k_dip_scatter<double><<<gn, b, 0, c->stream>>>(n, p->P, p->lin.as<uint32_t>(), p->mu.as<double>(),
                                                        (const double*)S, p->grid.as<double>());

@@ -48,8 +48,11 @@ int set_sq(llg_ctx* c, const uint32_t grid[3], const uint32_t* mesh, uint32_t nk
 int sample_sq(llg_ctx* c, double* dev_out /* ngroups*nk*2 */);
 
 // Dipolar convolution (k_dipolar.hip)
-int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_species);
+// fx != null: FFT exchange (species-resolved 3x3 kernel; dipolar terms added when dip_terms)
+int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_species, const llg_fftexch* fx,
+                   bool dip_terms);
 void dipolar_destroy(llg_ctx* c);
+bool dipolar_is_exchange(const llg_ctx* c);  // the plan is an FFT-exchange plan
 int dipolar_refresh(llg_ctx* c, const void* S, void* out);  // spins of the path's precision -> out (3n)
 
 // Encodings (encode.cpp)

@@ -68,6 +68,15 @@ class Dipolar(C.Structure):
                 ("mesh", C.POINTER(C.c_uint32))]
 
 
+class FftInteraction(C.Structure):
+    _fields_ = [("s1", C.c_uint32), ("s2", C.c_uint32), ("d", C.c_int32 * 3), ("pad_", C.c_uint32),
+                ("J", C.c_double * 9)]
+
+
+class FftExch(C.Structure):
+    _fields_ = [("n_int", C.c_uint32), ("pad_", C.c_uint32), ("ints", C.POINTER(FftInteraction))]
+
+
 class FourSpin(C.Structure):
     _fields_ = [("nq", C.c_uint64), ("xadj", C.POINTER(C.c_uint32)), ("jkl", C.POINTER(C.c_uint32)),
                 ("jq", C.c_double), ("cpu_form", C.c_int32), ("pad_", C.c_uint32)]
@@ -79,7 +88,7 @@ class Desc(C.Structure):
                 ("spin_species", C.POINTER(C.c_uint32)), ("anis", Anisotropy), ("exch", Exchange),
                 ("dip", Dipolar), ("dt", C.c_double), ("scheme", C.c_int32), ("precision", C.c_int32),
                 ("noise", C.c_int32), ("update", C.c_uint32), ("seed", C.c_uint64), ("stream", C.c_uint64),
-                ("flags", C.c_uint32), ("device", C.c_int32), ("four", FourSpin)]
+                ("flags", C.c_uint32), ("device", C.c_int32), ("four", FourSpin), ("fftx", FftExch)]
 
 
 class Info(C.Structure):
@@ -342,6 +351,9 @@ class System:
     flags: int = 0
     device: int = 0
     four: dict = field(default_factory=dict)  # xadj [n+1], jkl [nq,3], jq (T), cpu_form
+    # FFT exchange: list of (s1, s2, (dx, dy, dz), J 3x3 in joules); needs the mesh geometry
+    # in dip (dim, nk, abc, mesh; dip["enabled"] adds the dipolar terms to the same kernel)
+    fftx: list = field(default_factory=list)
 
     def desc(self) -> Desc:
         d = Desc()
@@ -401,9 +413,9 @@ class System:
             d.exch.stencil_vals = _p(sv, C.c_double)
             d.exch.stencil_vals_per_row = int(e.get("per_row", 0))
         dp = self.dip
-        if dp.get("enabled"):
+        if dp.get("enabled") or self.fftx:
             self._mesh = np.ascontiguousarray(dp["mesh"], dtype=np.uint32)
-            d.dip.enabled = 1
+            d.dip.enabled = 1 if dp.get("enabled") else 0
             d.dip.every_stage = int(dp.get("every_stage", 0))
             d.dip.refresh = int(dp.get("refresh", 0))
             d.dip.dim[:] = list(dp["dim"])
@@ -429,6 +441,15 @@ class System:
             d.four.jkl = _p(fj, C.c_uint32)
             d.four.jq = float(f["jq"])
             d.four.cpu_form = int(f.get("cpu_form", 0))
+        if self.fftx:
+            arr = (FftInteraction * len(self.fftx))()
+            for k, (s1, s2, dv, J) in enumerate(self.fftx):
+                arr[k].s1, arr[k].s2 = int(s1), int(s2)
+                arr[k].d[:] = [int(x) for x in dv]
+                arr[k].J[:] = [float(x) for x in np.asarray(J, dtype=float).reshape(9)]
+            self._keep.append(arr)
+            d.fftx.n_int = len(self.fftx)
+            d.fftx.ints = C.cast(arr, C.POINTER(FftInteraction))
         return d
 
 

@@ -139,6 +139,30 @@ def sc_film_dip(Lx=128, Ly=128, Lz=16, precision="fp32", J=3e-21, a=3e-10, lam=0
     return sysd, random_spins(n, seed, seed + 100)
 
 
+def sc_film_fftx(Lx=128, Ly=128, Lz=16, precision="fp32", J=3e-21, a=3e-10, lam=0.1, dt=1e-16, seed=3, update=100,
+                 dipolar=True, as_csr=False):
+    """Open sc film with nearest-neighbour exchange J as FFT exchange (the reference's
+    Exchange_method "fft": open boundaries, convolution every stage; dipolar terms in the
+    same kernel when `dipolar`), or with as_csr the same physics as CSR exchange plus the
+    dipolar convolution refreshed every stage."""
+    uc = unit_cell("sc")
+    dims = (Lx, Ly, Lz)
+    n = Lx * Ly * Lz
+    m = mesh(uc, dims)
+    dip = dict(enabled=1 if dipolar else 0, mesh=m, dim=dims, nk=(1, 1, 1), abc=(a, a, a), every_stage=1, refresh=1)
+    common = dict(n=n, species=[_species(lam=lam)], dip=dip, dt=dt, precision=precision, noise="counter",
+                  update=update, seed=seed)
+    if as_csr:
+        ints, _ = shells(uc, [J])
+        xadj, adj, vals = build_csr(uc, dims, ints, [MU], pbc=(0, 0, 0))
+        sysd = System(exch=dict(format=EXCH_CSR, xadj=xadj, adjncy=adj, vals=vals), **common)
+    else:
+        Jt = np.eye(3) * J
+        vecs = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)]
+        sysd = System(fftx=[(0, 0, v, Jt) for v in vecs], **common)
+    return sysd, random_spins(n, seed, seed + 100)
+
+
 def plaquette_quartets(dims, pbc=(1, 1, 1)):
     """Four-spin quartets of a simple-cubic lattice (spin index (x*d1 + y)*d2 + z): every
     elementary square plaquette of the xy, yz and zx planes, listed once from the point of

@@ -178,6 +178,30 @@ typedef struct llg_fourspin {
     uint32_t pad_;
 } llg_fourspin;
 
+/* FFT exchange, the reference's Exchange_method "fft" (exchm = 0): the exchange (and, with
+ * dip.enabled, the dipole-dipole interaction) as one species-resolved 3x3 convolution on the
+ * zero-padded mesh of desc.dip (dim, nk, abc, mesh; open boundaries), applied before every
+ * field evaluation (src/exch_intmat.cpp:28-160 fills N(s1, s2, a, b, r) with
+ * J_ab / (mu_s1 mu_B); src/intmat.cpp:188-262 adds 1e-7 (3 e_a e_b - delta_ab) mu_s2 mu_B / r^3;
+ * src/cuda.cu:47-60,162-175 and src/cufields.cu:248-292,361-408,432-446 apply it per stage).
+ * Interaction q: a spin i of species s1 at mesh m receives
+ *   H_a(i) += sum_b J[q][3 a + b] S_b(j) / (mu_s1 mu_B)
+ * from the spin j of species s2 at mesh m + d[q] when there is one (J in joules; the caller
+ * lists every vector, e.g. all symmetry images of a shell).  exch.format must be
+ * LLG_EXCH_NONE; n_int = 0 turns it off. */
+typedef struct llg_fft_interaction {
+    uint32_t s1, s2;
+    int32_t d[3];
+    uint32_t pad_;
+    double J[9];
+} llg_fft_interaction;
+
+typedef struct llg_fftexch {
+    uint32_t n_int;
+    uint32_t pad_;
+    const llg_fft_interaction* ints;
+} llg_fftexch;
+
 typedef struct llg_desc {
     uint32_t abi_version; /* LLG_ABI_VERSION */
     uint32_t n;           /* number of spins */
@@ -198,6 +222,7 @@ typedef struct llg_desc {
     uint32_t flags;
     int32_t device;       /* HIP device ordinal */
     llg_fourspin four;    /* optional four-spin term (zeroed by llg_desc_init) */
+    llg_fftexch fftx;     /* optional FFT exchange (zeroed by llg_desc_init) */
 } llg_desc;
 
 typedef struct llg_ctx llg_ctx;

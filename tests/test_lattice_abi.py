@@ -152,6 +152,8 @@ int main(void) {
          offsetof(llg_dipolar, mesh));
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(llg_ucf_atom), offsetof(llg_ucf_atom, element), sizeof(llg_tune_result),
          offsetof(llg_tune_result, candidates), sizeof(llg_fourspin), offsetof(llg_desc, four));
+  printf("%zu %zu %zu %zu\n", sizeof(llg_fft_interaction), offsetof(llg_fft_interaction, J), sizeof(llg_fftexch),
+         offsetof(llg_desc, fftx));
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(llg_batch_tune), offsetof(llg_batch_tune, candidates),
          sizeof(llg_request), offsetof(llg_request, seed), sizeof(llg_result), sizeof(llg_serve_stats));
   return 0;
@@ -162,7 +164,7 @@ int main(void) {
         open(c, "w").write(src)
         exe = os.path.join(d, "l")
         subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
-        a, b, c3, c4 = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")[:4]
+        a, b, c3, c5, c4 = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")[:5]
     sizes = [int(x) for x in a.split()]
     offs = [int(x) for x in b.split()]
     assert sizes == [C.sizeof(t) for t in (sllg.Desc, sllg.Species, sllg.Anisotropy, sllg.Exchange, sllg.Dipolar,
@@ -174,6 +176,8 @@ int main(void) {
     assert [int(x) for x in c3.split()] == [C.sizeof(refinput.UcfAtom), refinput.UcfAtom.element.offset,
                                             C.sizeof(sllg.TuneResult), sllg.TuneResult.candidates.offset,
                                             C.sizeof(sllg.FourSpin), sllg.Desc.four.offset]
+    assert [int(x) for x in c5.split()] == [C.sizeof(sllg.FftInteraction), sllg.FftInteraction.J.offset,
+                                            C.sizeof(sllg.FftExch), sllg.Desc.fftx.offset]
     from sllg import batch as SB
     assert [int(x) for x in c4.split()] == [C.sizeof(SB.BatchTune), SB.BatchTune.candidates.offset,
                                             C.sizeof(SB.Request), SB.Request.seed.offset, C.sizeof(SB.Result),
