@@ -168,11 +168,20 @@ uint64_t llg_estimate_bytes(const llg_desc* d) {
     } else if (d->exch.format == LLG_EXCH_STENCIL) {
         b += d->exch.stencil_vals_per_row ? (uint64_t)d->exch.stencil_k * nc * n * (P + 8) : 4096;
     }
-    if (d->dip.enabled) {
+    const bool fftx = d->fftx.n_int > 0;
+    if (d->dip.enabled || fftx) {
         uint64_t G = 1, Gh = 1;
         for (int a = 0; a < 3; ++a) G *= 2ull * d->dip.dim[a] * d->dip.nk[a];
         Gh = G / (2ull * d->dip.dim[2] * d->dip.nk[2]) * (d->dip.dim[2] * d->dip.nk[2] + 1);
-        b += 3 * n * P + 2 * 3 * G * P + 3 * Gh * 2 * P + 6 * Gh * P + 2 * n * 4;
+        if (fftx) {
+            // one grid and spectrum pair per species, the complex 3x3 kernel per species pair,
+            // and the FP64 real/complex kernel buffers that live only while it is transformed
+            const uint64_t nms = d->n_species ? d->n_species : 1;
+            b += 3 * n * P + 2 * 3 * nms * G * P + 2 * 3 * nms * Gh * 2 * P + nms * nms * 9 * Gh * 2 * P + 2 * n * 4;
+            b += nms * nms * 9 * (G * 8 + Gh * 16);
+        } else {
+            b += 3 * n * P + 2 * 3 * G * P + 3 * Gh * 2 * P + 6 * Gh * P + 2 * n * 4;
+        }
     }
     return b;
 }

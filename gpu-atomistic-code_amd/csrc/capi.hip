@@ -740,7 +740,7 @@ int llg_get_info(llg_ctx* c, llg_info* info) {
                      &c->sp32, &c->sp64d, &c->p_sigma, &c->p_llgpf, &c->p_lambda, &c->p_k1u, &c->p_k1udir,
                      &c->p_kc, &c->red_part, &c->red_out};
     for (DevBuf* b : all) dev += b->bytes;
-    info->device_bytes = dev + (c->xs ? c->xs->bytes() : 0);
+    info->device_bytes = dev + dipolar_bytes(c) + (c->xs ? c->xs->bytes() : 0);
     std::snprintf(info->exch_encoding, sizeof info->exch_encoding, "%s", c->xs ? c->xs->encoding.c_str() : "none");
     const double n = (double)c->n;
     static const HostExch empty;

@@ -457,6 +457,17 @@ void dipolar_destroy(llg_ctx* c) {
     c->dplan = nullptr;
 }
 
+uint64_t dipolar_bytes(const llg_ctx* c) {
+    const DipolarPlan* p = c->dplan;
+    if (!p) return 0;
+    uint64_t b = p->lin.bytes + p->mu.bytes + p->grid.bytes + p->spec.bytes + p->hgrid.bytes + p->sp.bytes +
+                 p->Nkc.bytes + p->hspec.bytes + (p->Nk ? p->Nk->bytes : 0);
+    size_t w = 0;
+    if (p->have_fwd && hipfftGetSize(p->fwd, &w) == HIPFFT_SUCCESS) b += w;
+    if (p->have_inv && hipfftGetSize(p->inv, &w) == HIPFFT_SUCCESS) b += w;
+    return b;
+}
+
 int dipolar_refresh(llg_ctx* c, const void* S, void* out) {
     DipolarPlan* p = c->dplan;
     if (!p) SLLG_FAIL(LLG_E_STATE, "dipolar plan missing");

@@ -52,6 +52,7 @@ int sample_sq(llg_ctx* c, double* dev_out /* ngroups*nk*2 */);
 int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_species, const llg_fftexch* fx,
                    bool dip_terms);
 void dipolar_destroy(llg_ctx* c);
+uint64_t dipolar_bytes(const llg_ctx* c);  // plan buffers and FFT work areas (a shared kernel counted in full)
 bool dipolar_is_exchange(const llg_ctx* c);  // the plan is an FFT-exchange plan
 int dipolar_refresh(llg_ctx* c, const void* S, void* out);  // spins of the path's precision -> out (3n)
 

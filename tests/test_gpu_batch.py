@@ -120,6 +120,21 @@ def test_batch_budget_admission(gpu):
     assert B2.info()["members"] == 2
 
 
+@pytest.mark.parametrize("kind", ["dipolar", "fftx", "fftx_dip"])
+def test_estimate_covers_convolution_plans(gpu, kind):
+    """The admission estimate bounds what a handle with a dipolar or FFT-exchange plan holds
+    (plan grids, spectra, k-space kernels and FFT work areas are counted in device_bytes)."""
+    if kind == "dipolar":
+        s, s0 = W.sc_film_dip(Lx=16, Ly=12, Lz=4, precision="fp32", refresh=1)
+    else:
+        s, s0 = W.sc_film_fftx(Lx=16, Ly=12, Lz=4, precision="fp32", dipolar=kind == "fftx_dip")
+    g = sllg.Context(s)
+    held = g.info()["device_bytes"]
+    G = 32 * 24 * 8  # padded grid points
+    assert held >= 3 * G * 4  # at least one real grid of the plan is counted
+    assert held <= SB.estimate_bytes(s)
+
+
 def test_batch_dipolar_member(gpu):
     s, s0 = W.sc_film_dip(Lx=8, Ly=8, Lz=2, precision="fp32", refresh=5)
     B = SB.Batch()
