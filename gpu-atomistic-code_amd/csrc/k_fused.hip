@@ -128,7 +128,9 @@ template <int NC, int UNR, bool G3>
 __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_t i, const float* __restrict__ S,
                                                    const float* sdict, float dot[3]) {
     dot[0] = dot[1] = dot[2] = 0.0f;
-    const uint32_t sl = i >> 6, lane = i & 63u;
+    // rows come in whole waves (row = 64 k + lane in every caller), so the slice, its
+    // descriptor and the loop bounds are wave-uniform: scalar loads and uniform branches
+    const uint32_t sl = __builtin_amdgcn_readfirstlane(i >> 6), lane = i & 63u;
     const uint2 sd = e.slice[sl];
     const uint32_t q0 = sd.x, nch = sd.y >> 2;
     const uint4* __restrict__ p = reinterpret_cast<const uint4*>(e.ent + (size_t)q0 * kSlice) + lane;
@@ -414,8 +416,10 @@ __device__ __forceinline__ bool row_live(const FusedArgs& a, uint32_t i) {
 
 // t: the step the thermal field is drawn for.  src: the spins (SoA) the field is evaluated at; gsrc: where neighbours are gathered from
 // (src itself, or with G3 its AoS3 copy, which then also gives the row's own spin; src is
-// then read only by the four-spin term).
-template <int FMT, int NC, int UNR = 2, bool G3 = false, bool LFB = true>
+// then read only by the four-spin term).  EDEM: the dipolar field is loaded with the row's
+// own spin, so its latency overlaps the exchange gathers instead of following them (the
+// sums keep their order; the batch kernels, whose members are mostly dipolar films, use it).
+template <int FMT, int NC, int UNR = 2, bool G3 = false, bool LFB = true, bool EDEM = false>
 __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32_t iref, const float* __restrict__ src,
                                           const float* __restrict__ gsrc, const float* sdict, float s[3], float f[3],
                                           uint64_t t) {
@@ -429,6 +433,12 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
         s[0] = src[i];
         s[1] = src[n + i];
         s[2] = src[2 * (size_t)n + i];
+    }
+    float hd[3] = {0.f, 0.f, 0.f};
+    if (EDEM && a.dem) {
+        hd[0] = a.dem[i];
+        hd[1] = a.dem[n + i];
+        hd[2] = a.dem[2 * (size_t)n + i];
     }
     float h[3];
     if constexpr (FMT == FMT_SELL_PACK)
@@ -445,9 +455,14 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
         h[2] += h4[2];
     }
     if (a.dem) {
-        h[0] += a.dem[i];
-        h[1] += a.dem[n + i];
-        h[2] += a.dem[2 * (size_t)n + i];
+        if constexpr (!EDEM) {
+            hd[0] = a.dem[i];
+            hd[1] = a.dem[n + i];
+            hd[2] = a.dem[2 * (size_t)n + i];
+        }
+        h[0] += hd[0];
+        h[1] += hd[1];
+        h[2] += hd[2];
     }
     if (a.stg) {
         h[0] += a.stg[i];
@@ -611,7 +626,7 @@ __global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restric
     const uint32_t n = a.n;
     float s[3], f[3];
     if constexpr (PHASE == 0) {
-        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.S, bm.S, sdict, s, f, t);
+        fused_rhs<FMT, NC, 2, false, true, true>(a, i, a.perm.to_ref(i), bm.S, bm.S, sdict, s, f, t);
         float y[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -622,10 +637,13 @@ __global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restric
             bm.A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
         }
     } else {
-        fused_rhs<FMT, NC>(a, i, a.perm.to_ref(i), bm.Y, bm.Y, sdict, s, f, t);
+        float av[3];  // issued ahead of the field evaluation
+#pragma unroll
+        for (int c = 0; c < 3; ++c) av[c] = bm.A[(size_t)c * n + i];
+        fused_rhs<FMT, NC, 2, false, true, true>(a, i, a.perm.to_ref(i), bm.Y, bm.Y, sdict, s, f, t);
         float p[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) p[c] = bm.A[(size_t)c * n + i] + a.half_rdt * f[c];
+        for (int c = 0; c < 3; ++c) p[c] = av[c] + a.half_rdt * f[c];
         const float r = rsqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) bm.S[(size_t)c * n + i] = p[c] * r;
