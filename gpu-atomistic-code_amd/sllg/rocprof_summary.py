@@ -35,8 +35,9 @@ def main():
         for name, cnt, avg, mn, mx in trace(db):
             print(f"{name[:60]:60s} {cnt:6d} {avg:10.3f} {mn:10.3f} {mx:10.3f}")
     else:
+        # full names: template arguments tell a kernel's instances apart
         for (name, counter), (mean, cnt) in sorted(pmc(db, sys.argv[3:]).items()):
-            print(f"{name[:60]:60s} {counter:12s} dispatches={cnt:5d} mean={mean:.1f} KiB")
+            print(f"{name} {counter} dispatches={cnt} mean={mean:.1f}")
 
 
 if __name__ == "__main__":
