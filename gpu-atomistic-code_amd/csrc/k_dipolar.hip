@@ -55,6 +55,14 @@ struct DipolarPlan {
     DevBuf sp;     // u32 species of each spin (device order)
     DevBuf Nkc;    // [nms][nms][9][Ph] complex, path precision
     DevBuf hspec;  // [nms][3][Ph] complex
+    // FP32 dipolar with a power-of-two Gx ("x half"): the input is zero for x >= Gx/2 and
+    // only x < Gx/2 is read back, so hipFFT transforms the (y, z) planes of that half only
+    // ([3][Gx/2][Gy][Gz] real <-> [3][Gx/2][Mc] complex, Mc = Gy (Gz/2 + 1) columns) and
+    // k_dip_xfft does the x transforms and the k-space product in one pass.
+    bool xh = false;
+    size_t Pc = 0, Mc = 0;  // real points per component of the half grid; columns
+    DevBuf tw;              // exp(-2 pi i m / Gx), m < Gx / 2 (FP32 pairs from FP64)
+    std::shared_ptr<DevBuf> NkX;  // N(k) in the x kernel's block order [group][kx][6][COLS]
     ~DipolarPlan() {
         if (have_fwd) hipfftDestroy(fwd);
         if (have_inv) hipfftDestroy(inv);
@@ -74,6 +82,7 @@ struct KernelKey {
 };
 static std::mutex g_nk_mu;
 static std::map<KernelKey, std::weak_ptr<DevBuf>> g_nk_cache;
+static std::map<KernelKey, std::weak_ptr<DevBuf>> g_nkx_cache;  // the same kernels in k_dip_xfft's order
 
 #define SLLG_FFT(call)                                                                       \
     do {                                                                                     \
@@ -112,6 +121,122 @@ __global__ __launch_bounds__(kTPB) void k_dip_mult(size_t Ph, const R* __restric
     spec[k] = hx;
     spec[Ph + k] = hy;
     spec[2 * Ph + k] = hz;
+}
+
+constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
+
+// x transforms and k-space product on the half grid (FP32): a block takes COLS (y, kz)
+// columns of all three components, zero-extends x from Gx/2 to Gx, runs a radix-2 DIT
+// forward FFT along x (bit-reversed load, natural-order result), multiplies by the real
+// symmetric kernel N(k) (6 comps, stored per block: [group][kx][comp][COLS]), runs a radix-2 DIF inverse FFT
+// (natural-order input, bit-reversed result) and writes back x < Gx/2 only.  Unnormalised
+// both ways (the gather scales by 1/P).  LDS: 3 COLS GX complex = 48 KB for every GX.
+constexpr int kXT = 512;  // threads of k_dip_xfft
+
+template <int GX>
+__global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __restrict__ tw,
+                                                  const float* __restrict__ NkX, float2* __restrict__ spec) {
+    constexpr int COLS = 2048 / GX, LX = GX / 2, LG = ilog2c(GX), NB = 3 * COLS * (GX / 2);
+    static_assert(COLS >= 1 && (1 << LG) == GX, "power-of-two x extent of at most 2048");
+    static_assert((COLS * GX) % kXT == 0, "whole (column, kx) pairs per thread");
+    __shared__ float2 buf[3 * COLS * GX];
+    __shared__ float2 w[GX / 2];
+    // neighbouring column groups share 128-byte lines of the spectrum: give each XCD a
+    // contiguous range of groups (blocks are dealt to the 8 XCDs round-robin)
+    const uint32_t nb = gridDim.x, per = nb >> 3, rem = nb & 7u, xb = blockIdx.x & 7u;
+    const uint32_t grp = xb * per + (xb < rem ? xb : rem) + (blockIdx.x >> 3);
+    const uint32_t j0 = grp * COLS;
+    // this thread's kernel values (COLS GX / kXT (column, kx) pairs), issued first so their
+    // latency overlaps the spectrum loads and the forward stages; padding groups read zeros
+    const float* const ng = NkX + (size_t)grp * GX * 6 * COLS;
+    constexpr int NPT = COLS * GX / kXT;
+    float nreg[NPT][6];
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+        const uint32_t e = threadIdx.x + r * kXT, col = e % COLS, kx = e / COLS;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) nreg[r][q] = ng[(size_t)kx * 6 * COLS + q * COLS + col];
+    }
+    for (uint32_t t = threadIdx.x; t < (uint32_t)(GX / 2); t += kXT) w[t] = tw[t];
+    for (uint32_t e = threadIdx.x; e < 3u * COLS * GX; e += kXT) {
+        const uint32_t col = e % COLS, x = (e / COLS) % GX, c = e / (COLS * GX), j = j0 + col;
+        float2 v = make_float2(0.f, 0.f);
+        if (x < (uint32_t)LX && j < M) v = spec[((size_t)c * LX + x) * M + j];
+        buf[(c * COLS + col) * GX + (__brev(x) >> (32 - LG))] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int len = 2; len <= GX; len <<= 1) {
+        const int half = len >> 1, step = GX / len;
+        for (uint32_t q = threadIdx.x; q < (uint32_t)NB; q += kXT) {
+            const uint32_t sq = q / (GX / 2), k = q % (GX / 2), pos = k % half;
+            const uint32_t i0 = sq * GX + (k / half) * len + pos, i1 = i0 + half;
+            const float2 wv = w[pos * step], u = buf[i0], v = buf[i1];
+            const float2 t = make_float2(wv.x * v.x - wv.y * v.y, wv.x * v.y + wv.y * v.x);
+            buf[i0] = make_float2(u.x + t.x, u.y + t.y);
+            buf[i1] = make_float2(u.x - t.x, u.y - t.y);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+        const uint32_t e = threadIdx.x + r * kXT, col = e % COLS, kx = e / COLS, j = j0 + col;
+        if (j >= M) continue;
+        const float nxx = nreg[r][0], nxy = nreg[r][1], nxz = nreg[r][2];
+        const float nyy = nreg[r][3], nyz = nreg[r][4], nzz = nreg[r][5];
+        float2* const bx = buf + (0 * COLS + col) * GX + kx;
+        float2* const by = buf + (1 * COLS + col) * GX + kx;
+        float2* const bz = buf + (2 * COLS + col) * GX + kx;
+        const float2 sx = *bx, sy = *by, sz = *bz;
+        *bx = make_float2(nxx * sx.x + nxy * sy.x + nxz * sz.x, nxx * sx.y + nxy * sy.y + nxz * sz.y);
+        *by = make_float2(nxy * sx.x + nyy * sy.x + nyz * sz.x, nxy * sx.y + nyy * sy.y + nyz * sz.y);
+        *bz = make_float2(nxz * sx.x + nyz * sy.x + nzz * sz.x, nxz * sx.y + nyz * sy.y + nzz * sz.y);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int len = GX; len >= 2; len >>= 1) {
+        const int half = len >> 1, step = GX / len;
+        for (uint32_t q = threadIdx.x; q < (uint32_t)NB; q += kXT) {
+            const uint32_t sq = q / (GX / 2), k = q % (GX / 2), pos = k % half;
+            const uint32_t i0 = sq * GX + (k / half) * len + pos, i1 = i0 + half;
+            const float2 wv = w[pos * step], u = buf[i0], v = buf[i1];  // conj(wv) = exp(+2 pi i pos / len)
+            const float2 d = make_float2(u.x - v.x, u.y - v.y);
+            buf[i0] = make_float2(u.x + v.x, u.y + v.y);
+            buf[i1] = make_float2(wv.x * d.x + wv.y * d.y, wv.x * d.y - wv.y * d.x);
+        }
+        __syncthreads();
+    }
+    for (uint32_t e = threadIdx.x; e < 3u * COLS * LX; e += kXT) {
+        const uint32_t col = e % COLS, x = (e / COLS) % LX, c = e / (COLS * LX), j = j0 + col;
+        if (j < M) spec[((size_t)c * LX + x) * M + j] = buf[(c * COLS + col) * GX + (__brev(x) >> (32 - LG))];
+    }
+}
+
+// N(k) from [6][kx M + j] to [j / COLS][kx][6][j % COLS] (zero past the last column).
+__global__ __launch_bounds__(kTPB) void k_nk_to_x(uint32_t M, uint32_t gx, uint32_t cols, size_t total,
+                                                  const float* __restrict__ Nk, float* __restrict__ NkX) {
+    const size_t e = (size_t)blockIdx.x * kTPB + threadIdx.x;
+    if (e >= total) return;
+    const uint32_t col = (uint32_t)(e % cols);
+    const uint32_t p = (uint32_t)((e / cols) % 6);
+    const uint32_t kx = (uint32_t)((e / ((size_t)cols * 6)) % gx);
+    const size_t g = e / ((size_t)cols * 6 * gx);
+    const size_t j = g * cols + col;
+    NkX[e] = j < M ? Nk[(size_t)p * gx * M + (size_t)kx * M + j] : 0.f;
+}
+
+static int launch_dip_xfft(int gx, uint32_t M, const float2* tw, const float* Nk, float2* spec, hipStream_t s) {
+#define SLLG_XF(G_)                                                                          \
+    case G_:                                                                                 \
+        k_dip_xfft<G_><<<(unsigned)((M + 2048 / G_ - 1) / (2048 / G_)), kXT, 0, s>>>(M, tw, Nk, spec); \
+        break;
+    switch (gx) {
+        SLLG_XF(16) SLLG_XF(32) SLLG_XF(64) SLLG_XF(128) SLLG_XF(256) SLLG_XF(512) SLLG_XF(1024) SLLG_XF(2048)
+        default: SLLG_FAIL(LLG_E_UNSUPPORTED, "x extent %d has no half-grid transform", gx);
+    }
+#undef SLLG_XF
+    SLLG_HIP(hipGetLastError());
+    return 0;
 }
 
 template <class R>
@@ -397,12 +522,20 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
         key.nk[a] = d.nk[a];
         key.abc[a] = d.abc[a];
     }
+    const int gx = p->G[0];
+    p->xh = !p->f64 && gx >= 16 && gx <= 2048 && (gx & (gx - 1)) == 0 && !(c->flags & LLG_FLAG_DIPOLAR_FULL_FFT);
     {
         std::lock_guard<std::mutex> lk(g_nk_mu);
-        auto it = g_nk_cache.find(key);
-        if (it != g_nk_cache.end()) p->Nk = it->second.lock();
+        if (p->xh) {
+            auto it = g_nkx_cache.find(key);
+            if (it != g_nkx_cache.end()) p->NkX = it->second.lock();
+        }
+        if (!p->NkX) {
+            auto it = g_nk_cache.find(key);
+            if (it != g_nk_cache.end()) p->Nk = it->second.lock();
+        }
     }
-    if (!p->Nk) {
+    if (!p->Nk && !p->NkX) {
         p->Nk = std::make_shared<DevBuf>();
         std::vector<double> N;
         fill_kernel(p->G, d.dim, d.nk, d.abc, N);
@@ -426,6 +559,46 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
         SLLG_HIP(hipDeviceSynchronize());
         std::lock_guard<std::mutex> lk(g_nk_mu);
         g_nk_cache[key] = p->Nk;
+    }
+    if (p->xh) {
+        // half grid: [3][Gx/2][Gy][Gz] real, [3][Gx/2][Mc] complex; lin is unchanged (x slowest)
+        p->Pc = p->P / 2;
+        p->Mc = (size_t)p->G[1] * (p->G[2] / 2 + 1);
+        if (!p->NkX) {  // N(k) in the x kernel's block order, shared like N(k) itself
+            const uint32_t cols = 2048u / (uint32_t)gx;
+            const size_t groups = (p->Mc + cols - 1) / cols, total = groups * (size_t)gx * 6 * cols;
+            p->NkX = std::make_shared<DevBuf>();
+            SLLG_TRY(p->NkX->alloc(total * sizeof(float)));
+            k_nk_to_x<<<blocks_for(total, kTPB), kTPB>>>((uint32_t)p->Mc, (uint32_t)gx, cols, total,
+                                                         p->Nk->as<float>(), p->NkX->as<float>());
+            SLLG_HIP(hipGetLastError());
+            SLLG_HIP(hipDeviceSynchronize());
+            std::lock_guard<std::mutex> lk(g_nk_mu);
+            g_nkx_cache[key] = p->NkX;
+        }
+        p->Nk.reset();  // the [6][Ph] layout is not read on this path
+        std::vector<float2> tw(gx / 2);
+        for (int m = 0; m < gx / 2; ++m) {
+            const double ang = -2.0 * M_PI * (double)m / (double)gx;
+            tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+        SLLG_TRY(p->tw.alloc(tw.size() * sizeof(float2)));
+        SLLG_HIP(hipMemcpy(p->tw.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+        SLLG_TRY(p->grid.alloc(3 * p->Pc * rs));
+        SLLG_TRY(p->hgrid.alloc(3 * p->Pc * rs));
+        SLLG_TRY(p->spec.alloc(3 * (size_t)(gx / 2) * p->Mc * 2 * rs));
+        SLLG_HIP(hipMemset(p->grid.p, 0, 3 * p->Pc * rs));
+        int d2[2] = {p->G[1], p->G[2]};
+        const int planes = 3 * (gx / 2);
+        SLLG_FFT(hipfftPlanMany(&p->fwd, 2, d2, nullptr, 1, p->G[1] * p->G[2], nullptr, 1, (int)p->Mc, HIPFFT_R2C,
+                                planes));
+        p->have_fwd = true;
+        SLLG_FFT(hipfftPlanMany(&p->inv, 2, d2, nullptr, 1, (int)p->Mc, nullptr, 1, p->G[1] * p->G[2], HIPFFT_C2R,
+                                planes));
+        p->have_inv = true;
+        SLLG_FFT(hipfftSetStream(p->fwd, c->stream));
+        SLLG_FFT(hipfftSetStream(p->inv, c->stream));
+        return 0;
     }
     SLLG_TRY(p->grid.alloc(3 * p->P * rs));
     SLLG_TRY(p->hgrid.alloc(3 * p->P * rs));
@@ -461,7 +634,8 @@ uint64_t dipolar_bytes(const llg_ctx* c) {
     const DipolarPlan* p = c->dplan;
     if (!p) return 0;
     uint64_t b = p->lin.bytes + p->mu.bytes + p->grid.bytes + p->spec.bytes + p->hgrid.bytes + p->sp.bytes +
-                 p->Nkc.bytes + p->hspec.bytes + (p->Nk ? p->Nk->bytes : 0);
+                 p->Nkc.bytes + p->hspec.bytes + p->tw.bytes + (p->Nk ? p->Nk->bytes : 0) +
+                 (p->NkX ? p->NkX->bytes : 0);
     size_t w = 0;
     if (p->have_fwd && hipfftGetSize(p->fwd, &w) == HIPFFT_SUCCESS) b += w;
     if (p->have_inv && hipfftGetSize(p->inv, &w) == HIPFFT_SUCCESS) b += w;
@@ -513,6 +687,16 @@ int dipolar_refresh(llg_ctx* c, const void* S, void* out) {
         SLLG_FFT(hipfftExecZ2D(p->inv, p->spec.as<hipfftDoubleComplex>(), p->hgrid.as<double>()));
         k_dip_gather<double><<<gn, b, 0, c->stream>>>(n, p->P, 1.0 / (double)p->P, p->lin.as<uint32_t>(),
                                                       p->hgrid.as<double>(), (double*)out);
+    } else if (p->xh) {
+        k_dip_scatter<float><<<gn, b, 0, c->stream>>>(n, p->Pc, p->lin.as<uint32_t>(), p->mu.as<float>(),
+                                                      (const float*)S, p->grid.as<float>());
+        SLLG_HIP(hipGetLastError());
+        SLLG_FFT(hipfftExecR2C(p->fwd, p->grid.as<float>(), p->spec.as<hipfftComplex>()));
+        SLLG_TRY(launch_dip_xfft(p->G[0], (uint32_t)p->Mc, p->tw.as<float2>(), p->NkX->as<float>(),
+                                 p->spec.as<float2>(), c->stream));
+        SLLG_FFT(hipfftExecC2R(p->inv, p->spec.as<hipfftComplex>(), p->hgrid.as<float>()));
+        k_dip_gather<float><<<gn, b, 0, c->stream>>>(n, p->Pc, 1.0f / (float)p->P, p->lin.as<uint32_t>(),
+                                                     p->hgrid.as<float>(), (float*)out);
     } else {
         k_dip_scatter<float><<<gn, b, 0, c->stream>>>(n, p->P, p->lin.as<uint32_t>(), p->mu.as<float>(),
                                                       (const float*)S, p->grid.as<float>());

@@ -31,6 +31,7 @@ FLAG_NO_XCD_REMAP = 0x100
 FLAG_NO_SUBLATTICE_ORDER = 0x200
 FLAG_NO_SLICE_SHARING = 0x400
 FLAG_NO_LINE_MODE = 0x800
+FLAG_DIPOLAR_FULL_FFT = 0x1000
 
 STATUS = {0: "OK", -1: "INVALID", -2: "NOMEM", -3: "DEVICE", -4: "UNSUPPORTED", -5: "NODEVICE",
           -6: "STATE", -7: "NONFINITE"}

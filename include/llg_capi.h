@@ -88,6 +88,10 @@ typedef enum llg_noise {
 #define LLG_FLAG_NO_SUBLATTICE_ORDER 0x200u /* keep the reference spin order on the device */
 #define LLG_FLAG_NO_SLICE_SHARING 0x400u    /* store every 64-row slice's entries, even repeats (no line mode) */
 #define LLG_FLAG_NO_LINE_MODE 0x800u        /* no line encoding: per-lane gathers only */
+/* FP32 dipolar convolution by hipFFT 3D transforms of the whole padded grid instead of 2D
+ * transforms of its x < Gx/2 half plus the fused x-transform / k-space product kernel
+ * (same result to FP32 rounding). */
+#define LLG_FLAG_DIPOLAR_FULL_FFT 0x1000u
 
 /* Per-species (unit-cell atom) constants; reference geom::ucm (inc/unitcell.h) and
  * the per-spin arrays filled in src/geom.cpp:239-247 and src/llg.cpp:591-592,626-631. */
