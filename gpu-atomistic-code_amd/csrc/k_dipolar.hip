@@ -125,21 +125,36 @@ __global__ __launch_bounds__(kTPB) void k_dip_mult(size_t Ph, const R* __restric
 
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 
-// x transforms and k-space product on the half grid (FP32): a block takes COLS (y, kz)
-// columns of all three components, zero-extends x from Gx/2 to Gx, runs a radix-2 DIT
-// forward FFT along x (bit-reversed load, natural-order result), multiplies by the real
-// symmetric kernel N(k) (6 comps, stored per block: [group][kx][comp][COLS]), runs a radix-2 DIF inverse FFT
-// (natural-order input, bit-reversed result) and writes back x < Gx/2 only.  Unnormalised
-// both ways (the gather scales by 1/P).  LDS: 3 COLS GX complex = 48 KB for every GX.
 constexpr int kXT = 512;  // threads of k_dip_xfft
 
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 w, float2 v) {
+    return make_float2(w.x * v.x - w.y * v.y, w.x * v.y + w.y * v.x);
+}
+// v conj(w)
+__device__ __forceinline__ float2 cmulc(float2 w, float2 v) {
+    return make_float2(w.x * v.x + w.y * v.y, w.x * v.y - w.y * v.x);
+}
+
+// x transforms and k-space product on the half grid (FP32): a block takes COLS (y, kz)
+// columns of all three components, zero-extends x from Gx/2 to Gx, runs a radix-2 DIT
+// forward FFT along x (bit-reversed load, natural-order result; two stages per LDS pass), multiplies by the real
+// symmetric kernel N(k) (6 comps, stored per block: [group][kx][comp][COLS]), runs a radix-2 DIF inverse FFT
+// (natural-order input, bit-reversed result) and writes back x < Gx/2 only.  Unnormalised
+// both ways (the gather scales by 1/P).  LDS: 3 COLS GX complex (+1/32 padding), about
+// 50 KB for every GX.
 template <int GX>
 __global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __restrict__ tw,
                                                   const float* __restrict__ NkX, float2* __restrict__ spec) {
-    constexpr int COLS = 2048 / GX, LX = GX / 2, LG = ilog2c(GX), NB = 3 * COLS * (GX / 2);
+    constexpr int COLS = 2048 / GX, LX = GX / 2, LG = ilog2c(GX), NB = 3 * COLS * (GX / 2), NQ = NB / 2;
     static_assert(COLS >= 1 && (1 << LG) == GX, "power-of-two x extent of at most 2048");
     static_assert((COLS * GX) % kXT == 0, "whole (column, kx) pairs per thread");
-    __shared__ float2 buf[3 * COLS * GX];
+    // a sequence's element i sits at i + i / 32 (one pad slot per 32): the bit-reversed
+    // load and store and the column-major product would otherwise hit one LDS bank pair
+    constexpr int GXP = GX + GX / 32;
+    __shared__ float2 buf[3 * COLS * GXP];
+    auto at = [](uint32_t sq, uint32_t i) -> uint32_t { return sq * GXP + i + (i >> 5); };
     __shared__ float2 w[GX / 2];
     // neighbouring column groups share 128-byte lines of the spectrum: give each XCD a
     // contiguous range of groups (blocks are dealt to the 8 XCDs round-robin)
@@ -162,19 +177,40 @@ __global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __re
         const uint32_t col = e % COLS, x = (e / COLS) % GX, c = e / (COLS * GX), j = j0 + col;
         float2 v = make_float2(0.f, 0.f);
         if (x < (uint32_t)LX && j < M) v = spec[((size_t)c * LX + x) * M + j];
-        buf[(c * COLS + col) * GX + (__brev(x) >> (32 - LG))] = v;
+        buf[at(c * COLS + col, __brev(x) >> (32 - LG))] = v;
     }
     __syncthreads();
-#pragma unroll
-    for (int len = 2; len <= GX; len <<= 1) {
-        const int half = len >> 1, step = GX / len;
+    // forward DIT: an odd stage count starts with one length-2 stage; then stages L and 2L
+    // run as one pass, a thread taking the quartet pos, pos + L/2, pos + L, pos + 3L/2 of a
+    // group of 2L (both butterflies of both stages in registers)
+    if constexpr (LG & 1) {
         for (uint32_t q = threadIdx.x; q < (uint32_t)NB; q += kXT) {
-            const uint32_t sq = q / (GX / 2), k = q % (GX / 2), pos = k % half;
-            const uint32_t i0 = sq * GX + (k / half) * len + pos, i1 = i0 + half;
-            const float2 wv = w[pos * step], u = buf[i0], v = buf[i1];
-            const float2 t = make_float2(wv.x * v.x - wv.y * v.y, wv.x * v.y + wv.y * v.x);
-            buf[i0] = make_float2(u.x + t.x, u.y + t.y);
-            buf[i1] = make_float2(u.x - t.x, u.y - t.y);
+            const uint32_t sq = q / (GX / 2), i0 = at(sq, 2 * (q % (GX / 2))), i1 = i0 + 1;  // w[0] = 1
+            const float2 u = buf[i0], v = buf[i1];
+            buf[i0] = cadd(u, v);
+            buf[i1] = csub(u, v);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int L = (LG & 1) ? 4 : 2; 2 * L <= GX; L <<= 2) {
+        const int qh = L >> 1;
+        for (uint32_t q = threadIdx.x; q < (uint32_t)NQ; q += kXT) {
+            const uint32_t sq = q / (GX / 4), k = q % (GX / 4), pos = k % qh;
+            const uint32_t b = (k / qh) * 2 * L + pos;
+            const uint32_t i0 = at(sq, b), i1 = at(sq, b + qh), i2 = at(sq, b + L), i3 = at(sq, b + L + qh);
+            const float2 w1 = w[pos * (GX / L)], wa = w[pos * (GX / (2 * L))], wb = w[(pos + qh) * (GX / (2 * L))];
+            const float2 a0 = buf[i0], a1 = buf[i1], a2 = buf[i2], a3 = buf[i3];
+            float2 t = cmul(w1, a1);
+            const float2 b0 = cadd(a0, t), b1 = csub(a0, t);
+            t = cmul(w1, a3);
+            const float2 b2 = cadd(a2, t), b3 = csub(a2, t);
+            t = cmul(wa, b2);
+            buf[i0] = cadd(b0, t);
+            buf[i2] = csub(b0, t);
+            t = cmul(wb, b3);
+            buf[i1] = cadd(b1, t);
+            buf[i3] = csub(b1, t);
         }
         __syncthreads();
     }
@@ -184,31 +220,47 @@ __global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __re
         if (j >= M) continue;
         const float nxx = nreg[r][0], nxy = nreg[r][1], nxz = nreg[r][2];
         const float nyy = nreg[r][3], nyz = nreg[r][4], nzz = nreg[r][5];
-        float2* const bx = buf + (0 * COLS + col) * GX + kx;
-        float2* const by = buf + (1 * COLS + col) * GX + kx;
-        float2* const bz = buf + (2 * COLS + col) * GX + kx;
+        float2* const bx = buf + at(0 * COLS + col, kx);
+        float2* const by = buf + at(1 * COLS + col, kx);
+        float2* const bz = buf + at(2 * COLS + col, kx);
         const float2 sx = *bx, sy = *by, sz = *bz;
         *bx = make_float2(nxx * sx.x + nxy * sy.x + nxz * sz.x, nxx * sx.y + nxy * sy.y + nxz * sz.y);
         *by = make_float2(nxy * sx.x + nyy * sy.x + nyz * sz.x, nxy * sx.y + nyy * sy.y + nyz * sz.y);
         *bz = make_float2(nxz * sx.x + nyz * sy.x + nzz * sz.x, nxz * sx.y + nyz * sy.y + nzz * sz.y);
     }
     __syncthreads();
+    // inverse DIF (twiddles conjugated): stages 2L and L as one pass from the top; an odd
+    // stage count ends with one length-2 stage
 #pragma unroll
-    for (int len = GX; len >= 2; len >>= 1) {
-        const int half = len >> 1, step = GX / len;
+    for (int L = GX / 2; L >= ((LG & 1) ? 4 : 2); L >>= 2) {
+        const int qh = L >> 1;
+        for (uint32_t q = threadIdx.x; q < (uint32_t)NQ; q += kXT) {
+            const uint32_t sq = q / (GX / 4), k = q % (GX / 4), pos = k % qh;
+            const uint32_t b = (k / qh) * 2 * L + pos;
+            const uint32_t i0 = at(sq, b), i1 = at(sq, b + qh), i2 = at(sq, b + L), i3 = at(sq, b + L + qh);
+            const float2 w1 = w[pos * (GX / L)], wa = w[pos * (GX / (2 * L))], wb = w[(pos + qh) * (GX / (2 * L))];
+            const float2 a0 = buf[i0], a1 = buf[i1], a2 = buf[i2], a3 = buf[i3];
+            const float2 b0 = cadd(a0, a2), b2 = cmulc(wa, csub(a0, a2));
+            const float2 b1 = cadd(a1, a3), b3 = cmulc(wb, csub(a1, a3));
+            buf[i0] = cadd(b0, b1);
+            buf[i1] = cmulc(w1, csub(b0, b1));
+            buf[i2] = cadd(b2, b3);
+            buf[i3] = cmulc(w1, csub(b2, b3));
+        }
+        __syncthreads();
+    }
+    if constexpr (LG & 1) {
         for (uint32_t q = threadIdx.x; q < (uint32_t)NB; q += kXT) {
-            const uint32_t sq = q / (GX / 2), k = q % (GX / 2), pos = k % half;
-            const uint32_t i0 = sq * GX + (k / half) * len + pos, i1 = i0 + half;
-            const float2 wv = w[pos * step], u = buf[i0], v = buf[i1];  // conj(wv) = exp(+2 pi i pos / len)
-            const float2 d = make_float2(u.x - v.x, u.y - v.y);
-            buf[i0] = make_float2(u.x + v.x, u.y + v.y);
-            buf[i1] = make_float2(wv.x * d.x + wv.y * d.y, wv.x * d.y - wv.y * d.x);
+            const uint32_t sq = q / (GX / 2), i0 = at(sq, 2 * (q % (GX / 2))), i1 = i0 + 1;
+            const float2 u = buf[i0], v = buf[i1];
+            buf[i0] = cadd(u, v);
+            buf[i1] = csub(u, v);
         }
         __syncthreads();
     }
     for (uint32_t e = threadIdx.x; e < 3u * COLS * LX; e += kXT) {
         const uint32_t col = e % COLS, x = (e / COLS) % LX, c = e / (COLS * LX), j = j0 + col;
-        if (j < M) spec[((size_t)c * LX + x) * M + j] = buf[(c * COLS + col) * GX + (__brev(x) >> (32 - LG))];
+        if (j < M) spec[((size_t)c * LX + x) * M + j] = buf[at(c * COLS + col, __brev(x) >> (32 - LG))];
     }
 }
 
