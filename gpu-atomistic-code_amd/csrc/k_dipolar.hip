@@ -63,6 +63,7 @@ struct DipolarPlan {
     size_t Pc = 0, Mc = 0;  // real points per component of the half grid; columns
     DevBuf tw;              // exp(-2 pi i m / Gx), m < Gx / 2 (FP32 pairs from FP64)
     std::shared_ptr<DevBuf> NkX;  // N(k) in the x kernel's block order [group][kx][6][COLS]
+    DevBuf NkXc;                  // FFT exchange, half grid: [group][kx][nms^2 9][COLS] complex
     ~DipolarPlan() {
         if (have_fwd) hipfftDestroy(fwd);
         if (have_inv) hipfftDestroy(inv);
@@ -137,68 +138,68 @@ __device__ __forceinline__ float2 cmulc(float2 w, float2 v) {
     return make_float2(w.x * v.x + w.y * v.y, w.x * v.y - w.y * v.x);
 }
 
-// x transforms and k-space product on the half grid (FP32): a block takes COLS (y, kz)
-// columns of all three components, zero-extends x from Gx/2 to Gx, runs a radix-2 DIT
-// forward FFT along x (bit-reversed load, natural-order result; two stages per LDS pass), multiplies by the real
-// symmetric kernel N(k) (6 comps, stored per block: [group][kx][comp][COLS]), runs a radix-2 DIF inverse FFT
-// (natural-order input, bit-reversed result) and writes back x < Gx/2 only.  Unnormalised
-// both ways (the gather scales by 1/P).  LDS: 3 COLS GX complex (+1/32 padding), about
-// 50 KB for every GX.
+// Half-grid x transforms (FP32).  A block takes COLS (y, kz) columns of NS spectrum
+// sequences each (3 components, times the species for FFT exchange), zero-extends x from
+// Gx/2 to Gx, runs a radix-2 DIT forward FFT along x (bit-reversed load, natural-order
+// result, two stages per LDS pass), forms the k-space product, runs a radix-2 DIF inverse
+// FFT (natural-order input, bit-reversed result) and writes back x < Gx/2 only.
+// Unnormalised both ways (the gather scales by 1/P).  NS COLS = 3 * 2048 / Gx sequences:
+// about 50 KB of LDS for every Gx.  A sequence's element i sits at i + i / 32 (one pad slot
+// per 32): the bit-reversed load and store and the column-major product would otherwise
+// hit one LDS bank pair.
 template <int GX>
-__global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __restrict__ tw,
-                                                  const float* __restrict__ NkX, float2* __restrict__ spec) {
-    constexpr int COLS = 2048 / GX, LX = GX / 2, LG = ilog2c(GX), NB = 3 * COLS * (GX / 2), NQ = NB / 2;
-    static_assert(COLS >= 1 && (1 << LG) == GX, "power-of-two x extent of at most 2048");
-    static_assert((COLS * GX) % kXT == 0, "whole (column, kx) pairs per thread");
-    // a sequence's element i sits at i + i / 32 (one pad slot per 32): the bit-reversed
-    // load and store and the column-major product would otherwise hit one LDS bank pair
-    constexpr int GXP = GX + GX / 32;
-    __shared__ float2 buf[3 * COLS * GXP];
-    auto at = [](uint32_t sq, uint32_t i) -> uint32_t { return sq * GXP + i + (i >> 5); };
-    __shared__ float2 w[GX / 2];
-    // neighbouring column groups share 128-byte lines of the spectrum: give each XCD a
-    // contiguous range of groups (blocks are dealt to the 8 XCDs round-robin)
+__device__ __forceinline__ uint32_t xf_at(uint32_t sq, uint32_t i) {
+    return sq * (GX + GX / 32) + i + (i >> 5);
+}
+
+// neighbouring column groups share 128-byte lines of the spectrum: each XCD takes a
+// contiguous range of groups (blocks are dealt to the 8 XCDs round-robin)
+__device__ __forceinline__ uint32_t xf_group() {
     const uint32_t nb = gridDim.x, per = nb >> 3, rem = nb & 7u, xb = blockIdx.x & 7u;
-    const uint32_t grp = xb * per + (xb < rem ? xb : rem) + (blockIdx.x >> 3);
-    const uint32_t j0 = grp * COLS;
-    // this thread's kernel values (COLS GX / kXT (column, kx) pairs), issued first so their
-    // latency overlaps the spectrum loads and the forward stages; padding groups read zeros
-    const float* const ng = NkX + (size_t)grp * GX * 6 * COLS;
-    constexpr int NPT = COLS * GX / kXT;
-    float nreg[NPT][6];
-#pragma unroll
-    for (int r = 0; r < NPT; ++r) {
-        const uint32_t e = threadIdx.x + r * kXT, col = e % COLS, kx = e / COLS;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) nreg[r][q] = ng[(size_t)kx * 6 * COLS + q * COLS + col];
-    }
+    return xb * per + (xb < rem ? xb : rem) + (blockIdx.x >> 3);
+}
+
+// spectrum [sequence][x < Gx/2][M] -> buf (bit-reversed, zero for x >= Gx/2), twiddles -> w
+template <int GX, int NS, int COLS>
+__device__ __forceinline__ void xf_load(float2* buf, float2* w, const float2* __restrict__ tw,
+                                        const float2* __restrict__ spec, uint32_t M, uint32_t j0) {
+    constexpr int LX = GX / 2, LG = ilog2c(GX);
     for (uint32_t t = threadIdx.x; t < (uint32_t)(GX / 2); t += kXT) w[t] = tw[t];
-    for (uint32_t e = threadIdx.x; e < 3u * COLS * GX; e += kXT) {
-        const uint32_t col = e % COLS, x = (e / COLS) % GX, c = e / (COLS * GX), j = j0 + col;
+    for (uint32_t e = threadIdx.x; e < (uint32_t)(NS * COLS * GX); e += kXT) {
+        const uint32_t col = e % COLS, x = (e / COLS) % GX, sc = e / (COLS * GX), j = j0 + col;
         float2 v = make_float2(0.f, 0.f);
-        if (x < (uint32_t)LX && j < M) v = spec[((size_t)c * LX + x) * M + j];
-        buf[at(c * COLS + col, __brev(x) >> (32 - LG))] = v;
+        if (x < (uint32_t)LX && j < M) v = spec[((size_t)sc * LX + x) * M + j];
+        buf[xf_at<GX>(sc * COLS + col, __brev(x) >> (32 - LG))] = v;
     }
     __syncthreads();
-    // forward DIT: an odd stage count starts with one length-2 stage; then stages L and 2L
-    // run as one pass, a thread taking the quartet pos, pos + L/2, pos + L, pos + 3L/2 of a
-    // group of 2L (both butterflies of both stages in registers)
-    if constexpr (LG & 1) {
-        for (uint32_t q = threadIdx.x; q < (uint32_t)NB; q += kXT) {
-            const uint32_t sq = q / (GX / 2), i0 = at(sq, 2 * (q % (GX / 2))), i1 = i0 + 1;  // w[0] = 1
-            const float2 u = buf[i0], v = buf[i1];
-            buf[i0] = cadd(u, v);
-            buf[i1] = csub(u, v);
-        }
-        __syncthreads();
+}
+
+template <int GX, int NSEQ>
+__device__ __forceinline__ void xf_single_stage(float2* buf) {  // length 2: w[0] = 1
+    for (uint32_t q = threadIdx.x; q < (uint32_t)(NSEQ * GX / 2); q += kXT) {
+        const uint32_t sq = q / (GX / 2), i0 = xf_at<GX>(sq, 2 * (q % (GX / 2))), i1 = i0 + 1;
+        const float2 u = buf[i0], v = buf[i1];
+        buf[i0] = cadd(u, v);
+        buf[i1] = csub(u, v);
     }
+    __syncthreads();
+}
+
+// forward DIT: an odd stage count starts with one length-2 stage; then stages L and 2L run
+// as one pass, a thread taking the quartet pos, pos + L/2, pos + L, pos + 3L/2 of a group
+// of 2L (both butterflies of both stages in registers)
+template <int GX, int NSEQ>
+__device__ __forceinline__ void xf_forward(float2* buf, const float2* w) {
+    constexpr int LG = ilog2c(GX);
+    if constexpr (LG & 1) xf_single_stage<GX, NSEQ>(buf);
 #pragma unroll
     for (int L = (LG & 1) ? 4 : 2; 2 * L <= GX; L <<= 2) {
         const int qh = L >> 1;
-        for (uint32_t q = threadIdx.x; q < (uint32_t)NQ; q += kXT) {
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(NSEQ * GX / 4); q += kXT) {
             const uint32_t sq = q / (GX / 4), k = q % (GX / 4), pos = k % qh;
             const uint32_t b = (k / qh) * 2 * L + pos;
-            const uint32_t i0 = at(sq, b), i1 = at(sq, b + qh), i2 = at(sq, b + L), i3 = at(sq, b + L + qh);
+            const uint32_t i0 = xf_at<GX>(sq, b), i1 = xf_at<GX>(sq, b + qh), i2 = xf_at<GX>(sq, b + L),
+                           i3 = xf_at<GX>(sq, b + L + qh);
             const float2 w1 = w[pos * (GX / L)], wa = w[pos * (GX / (2 * L))], wb = w[(pos + qh) * (GX / (2 * L))];
             const float2 a0 = buf[i0], a1 = buf[i1], a2 = buf[i2], a3 = buf[i3];
             float2 t = cmul(w1, a1);
@@ -214,30 +215,21 @@ __global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __re
         }
         __syncthreads();
     }
-#pragma unroll
-    for (int r = 0; r < NPT; ++r) {
-        const uint32_t e = threadIdx.x + r * kXT, col = e % COLS, kx = e / COLS, j = j0 + col;
-        if (j >= M) continue;
-        const float nxx = nreg[r][0], nxy = nreg[r][1], nxz = nreg[r][2];
-        const float nyy = nreg[r][3], nyz = nreg[r][4], nzz = nreg[r][5];
-        float2* const bx = buf + at(0 * COLS + col, kx);
-        float2* const by = buf + at(1 * COLS + col, kx);
-        float2* const bz = buf + at(2 * COLS + col, kx);
-        const float2 sx = *bx, sy = *by, sz = *bz;
-        *bx = make_float2(nxx * sx.x + nxy * sy.x + nxz * sz.x, nxx * sx.y + nxy * sy.y + nxz * sz.y);
-        *by = make_float2(nxy * sx.x + nyy * sy.x + nyz * sz.x, nxy * sx.y + nyy * sy.y + nyz * sz.y);
-        *bz = make_float2(nxz * sx.x + nyz * sy.x + nzz * sz.x, nxz * sx.y + nyz * sy.y + nzz * sz.y);
-    }
-    __syncthreads();
-    // inverse DIF (twiddles conjugated): stages 2L and L as one pass from the top; an odd
-    // stage count ends with one length-2 stage
+}
+
+// inverse DIF (twiddles conjugated): stages 2L and L as one pass from the top; an odd stage
+// count ends with one length-2 stage
+template <int GX, int NSEQ>
+__device__ __forceinline__ void xf_inverse(float2* buf, const float2* w) {
+    constexpr int LG = ilog2c(GX);
 #pragma unroll
     for (int L = GX / 2; L >= ((LG & 1) ? 4 : 2); L >>= 2) {
         const int qh = L >> 1;
-        for (uint32_t q = threadIdx.x; q < (uint32_t)NQ; q += kXT) {
+        for (uint32_t q = threadIdx.x; q < (uint32_t)(NSEQ * GX / 4); q += kXT) {
             const uint32_t sq = q / (GX / 4), k = q % (GX / 4), pos = k % qh;
             const uint32_t b = (k / qh) * 2 * L + pos;
-            const uint32_t i0 = at(sq, b), i1 = at(sq, b + qh), i2 = at(sq, b + L), i3 = at(sq, b + L + qh);
+            const uint32_t i0 = xf_at<GX>(sq, b), i1 = xf_at<GX>(sq, b + qh), i2 = xf_at<GX>(sq, b + L),
+                           i3 = xf_at<GX>(sq, b + L + qh);
             const float2 w1 = w[pos * (GX / L)], wa = w[pos * (GX / (2 * L))], wb = w[(pos + qh) * (GX / (2 * L))];
             const float2 a0 = buf[i0], a1 = buf[i1], a2 = buf[i2], a3 = buf[i3];
             const float2 b0 = cadd(a0, a2), b2 = cmulc(wa, csub(a0, a2));
@@ -249,19 +241,96 @@ __global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __re
         }
         __syncthreads();
     }
-    if constexpr (LG & 1) {
-        for (uint32_t q = threadIdx.x; q < (uint32_t)NB; q += kXT) {
-            const uint32_t sq = q / (GX / 2), i0 = at(sq, 2 * (q % (GX / 2))), i1 = i0 + 1;
-            const float2 u = buf[i0], v = buf[i1];
-            buf[i0] = cadd(u, v);
-            buf[i1] = csub(u, v);
-        }
-        __syncthreads();
+    if constexpr (LG & 1) xf_single_stage<GX, NSEQ>(buf);
+}
+
+template <int GX, int NS, int COLS>
+__device__ __forceinline__ void xf_store(const float2* buf, float2* __restrict__ spec, uint32_t M, uint32_t j0) {
+    constexpr int LX = GX / 2, LG = ilog2c(GX);
+    for (uint32_t e = threadIdx.x; e < (uint32_t)(NS * COLS * LX); e += kXT) {
+        const uint32_t col = e % COLS, x = (e / COLS) % LX, sc = e / (COLS * LX), j = j0 + col;
+        if (j < M) spec[((size_t)sc * LX + x) * M + j] = buf[xf_at<GX>(sc * COLS + col, __brev(x) >> (32 - LG))];
     }
-    for (uint32_t e = threadIdx.x; e < 3u * COLS * LX; e += kXT) {
-        const uint32_t col = e % COLS, x = (e / COLS) % LX, c = e / (COLS * LX), j = j0 + col;
-        if (j < M) spec[((size_t)c * LX + x) * M + j] = buf[at(c * COLS + col, __brev(x) >> (32 - LG))];
+}
+
+// Dipolar: 3 sequences per column, product with the real symmetric N(k) (6 comps, stored
+// per block: [group][kx][comp][COLS]); each thread's kernel values are loaded first so
+// their latency overlaps the spectrum loads and the forward stages.
+template <int GX>
+__global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __restrict__ tw,
+                                                  const float* __restrict__ NkX, float2* __restrict__ spec) {
+    constexpr int COLS = 2048 / GX, LG = ilog2c(GX), NPT = COLS * GX / kXT;
+    static_assert(COLS >= 1 && (1 << LG) == GX, "power-of-two x extent of at most 2048");
+    static_assert((COLS * GX) % kXT == 0, "whole (column, kx) pairs per thread");
+    __shared__ float2 buf[3 * COLS * (GX + GX / 32)];
+    __shared__ float2 w[GX / 2];
+    const uint32_t grp = xf_group(), j0 = grp * COLS;
+    const float* const ng = NkX + (size_t)grp * GX * 6 * COLS;  // padding columns hold zeros
+    float nreg[NPT][6];
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+        const uint32_t e = threadIdx.x + r * kXT, col = e % COLS, kx = e / COLS;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) nreg[r][q] = ng[(size_t)kx * 6 * COLS + q * COLS + col];
     }
+    xf_load<GX, 3, COLS>(buf, w, tw, spec, M, j0);
+    xf_forward<GX, 3 * COLS>(buf, w);
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+        const uint32_t e = threadIdx.x + r * kXT, col = e % COLS, kx = e / COLS, j = j0 + col;
+        if (j >= M) continue;
+        const float nxx = nreg[r][0], nxy = nreg[r][1], nxz = nreg[r][2];
+        const float nyy = nreg[r][3], nyz = nreg[r][4], nzz = nreg[r][5];
+        float2* const bx = buf + xf_at<GX>(0 * COLS + col, kx);
+        float2* const by = buf + xf_at<GX>(1 * COLS + col, kx);
+        float2* const bz = buf + xf_at<GX>(2 * COLS + col, kx);
+        const float2 sx = *bx, sy = *by, sz = *bz;
+        *bx = make_float2(nxx * sx.x + nxy * sy.x + nxz * sz.x, nxx * sx.y + nxy * sy.y + nxz * sz.y);
+        *by = make_float2(nxy * sx.x + nyy * sy.x + nyz * sz.x, nxy * sx.y + nyy * sy.y + nyz * sz.y);
+        *bz = make_float2(nxz * sx.x + nyz * sy.x + nzz * sz.x, nxz * sx.y + nyz * sy.y + nzz * sz.y);
+    }
+    __syncthreads();
+    xf_inverse<GX, 3 * COLS>(buf, w);
+    xf_store<GX, 3, COLS>(buf, spec, M, j0);
+}
+
+// FFT exchange: 3 NMS sequences per column ([species][component]), product with the
+// complex kernel, H(s1, a) = sum_{s2, b} N(s1, s2, a, b) S(s2, b), N stored per block:
+// [group][kx][(s1 NMS + s2) 9 + 3 a + b][COLS].
+template <int GX, int NMS>
+__global__ __launch_bounds__(kXT) void k_fx_xfft(uint32_t M, const float2* __restrict__ tw,
+                                                 const float2* __restrict__ NkX, float2* __restrict__ spec) {
+    constexpr int COLS = 2048 / (GX * NMS), LG = ilog2c(GX), NN = NMS * NMS * 9;
+    static_assert(COLS >= 1 && (1 << LG) == GX, "power-of-two x extent, GX NMS at most 2048");
+    __shared__ float2 buf[3 * NMS * COLS * (GX + GX / 32)];
+    __shared__ float2 w[GX / 2];
+    const uint32_t grp = xf_group(), j0 = grp * COLS;
+    const float2* const ng = NkX + (size_t)grp * GX * NN * COLS;
+    xf_load<GX, 3 * NMS, COLS>(buf, w, tw, spec, M, j0);
+    xf_forward<GX, 3 * NMS * COLS>(buf, w);
+    for (uint32_t e = threadIdx.x; e < (uint32_t)(COLS * GX); e += kXT) {
+        const uint32_t col = e % COLS, kx = e / COLS, j = j0 + col;
+        if (j >= M) continue;
+        float2 sv[3 * NMS];
+#pragma unroll
+        for (int q = 0; q < 3 * NMS; ++q) sv[q] = buf[xf_at<GX>(q * COLS + col, kx)];
+        const float2* const nv = ng + (size_t)kx * NN * COLS + col;
+#pragma unroll
+        for (int s1 = 0; s1 < NMS; ++s1)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float2 h = make_float2(0.f, 0.f);
+#pragma unroll
+                for (int s2 = 0; s2 < NMS; ++s2)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b)
+                        h = cadd(h, cmul(nv[(size_t)((s1 * NMS + s2) * 9 + a * 3 + b) * COLS], sv[s2 * 3 + b]));
+                buf[xf_at<GX>((s1 * 3 + a) * COLS + col, kx)] = h;
+            }
+    }
+    __syncthreads();
+    xf_inverse<GX, 3 * NMS * COLS>(buf, w);
+    xf_store<GX, 3 * NMS, COLS>(buf, spec, M, j0);
 }
 
 // N(k) from [6][kx M + j] to [j / COLS][kx][6][j % COLS] (zero past the last column).
@@ -277,6 +346,20 @@ __global__ __launch_bounds__(kTPB) void k_nk_to_x(uint32_t M, uint32_t gx, uint3
     NkX[e] = j < M ? Nk[(size_t)p * gx * M + (size_t)kx * M + j] : 0.f;
 }
 
+
+// FFT-exchange kernel from [nn][kx M + j] (complex) to [j / COLS][kx][nn][j % COLS].
+__global__ __launch_bounds__(kTPB) void k_nkc_to_x(uint32_t M, uint32_t gx, uint32_t cols, uint32_t nn, size_t total,
+                                                   const float2* __restrict__ Nk, float2* __restrict__ NkX) {
+    const size_t e = (size_t)blockIdx.x * kTPB + threadIdx.x;
+    if (e >= total) return;
+    const uint32_t col = (uint32_t)(e % cols);
+    const uint32_t p = (uint32_t)((e / cols) % nn);
+    const uint32_t kx = (uint32_t)((e / ((size_t)cols * nn)) % gx);
+    const size_t g = e / ((size_t)cols * nn * gx);
+    const size_t j = g * cols + col;
+    NkX[e] = j < M ? Nk[(size_t)p * gx * M + (size_t)kx * M + j] : make_float2(0.f, 0.f);
+}
+
 static int launch_dip_xfft(int gx, uint32_t M, const float2* tw, const float* Nk, float2* spec, hipStream_t s) {
 #define SLLG_XF(G_)                                                                          \
     case G_:                                                                                 \
@@ -288,6 +371,54 @@ static int launch_dip_xfft(int gx, uint32_t M, const float2* tw, const float* Nk
     }
 #undef SLLG_XF
     SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+static int launch_fx_xfft(int gx, uint32_t nms, uint32_t M, const float2* tw, const float2* Nk, float2* spec,
+                          hipStream_t s) {
+#define SLLG_XF(G_, S_)                                                                              \
+    case G_:                                                                                         \
+        k_fx_xfft<G_, S_><<<(unsigned)((M + 2048 / (G_ * S_) - 1) / (2048 / (G_ * S_))), kXT, 0, s>>>(M, tw, Nk, \
+                                                                                                   spec); \
+        break;
+    if (nms == 1) {
+        switch (gx) {
+            SLLG_XF(16, 1) SLLG_XF(32, 1) SLLG_XF(64, 1) SLLG_XF(128, 1) SLLG_XF(256, 1) SLLG_XF(512, 1)
+            SLLG_XF(1024, 1) SLLG_XF(2048, 1)
+            default: SLLG_FAIL(LLG_E_UNSUPPORTED, "x extent %d has no half-grid transform", gx);
+        }
+    } else if (nms == 2) {
+        switch (gx) {
+            SLLG_XF(16, 2) SLLG_XF(32, 2) SLLG_XF(64, 2) SLLG_XF(128, 2) SLLG_XF(256, 2) SLLG_XF(512, 2)
+            SLLG_XF(1024, 2)
+            default: SLLG_FAIL(LLG_E_UNSUPPORTED, "x extent %d has no half-grid transform", gx);
+        }
+    } else {
+        SLLG_FAIL(LLG_E_UNSUPPORTED, "half-grid FFT exchange with %u species", nms);
+    }
+#undef SLLG_XF
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+// The half-grid path applies to FP32 plans with a power-of-two x extent (and, for FFT
+// exchange, one or two species) unless the caller asks for the whole-grid transforms.
+static bool half_grid_ok(const llg_ctx* c, const DipolarPlan* p, uint32_t nms) {
+    const int gx = p->G[0];
+    return !p->f64 && gx >= 16 && (gx & (gx - 1)) == 0 && nms >= 1 && nms <= 2 && gx * (int)nms <= 2048 &&
+           !(c->flags & LLG_FLAG_DIPOLAR_FULL_FFT);
+}
+
+// exp(-2 pi i m / Gx), m < Gx / 2, FP32 pairs from FP64
+static int upload_twiddles(DipolarPlan* p) {
+    const int gx = p->G[0];
+    std::vector<float2> tw(gx / 2);
+    for (int m = 0; m < gx / 2; ++m) {
+        const double ang = -2.0 * M_PI * (double)m / (double)gx;
+        tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    SLLG_TRY(p->tw.alloc(tw.size() * sizeof(float2)));
+    SLLG_HIP(hipMemcpy(p->tw.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -508,6 +639,38 @@ static int fftexch_create(llg_ctx* c, DipolarPlan* p, const llg_dipolar& d, cons
         SLLG_HIP(hipGetLastError());
         SLLG_HIP(hipDeviceSynchronize());
     }
+    p->xh = half_grid_ok(c, p, nms);
+    if (p->xh) {
+        // half grid: [nms][3][Gx/2][Gy][Gz] real, [nms][3][Gx/2][Mc] complex (x slowest, so
+        // lin is unchanged); the kernel in k_fx_xfft's block order replaces Nkc
+        const int gx = p->G[0];
+        const uint32_t cols = 2048u / ((uint32_t)gx * nms), nn = nms * nms * 9;
+        p->Pc = p->P / 2;
+        p->Mc = (size_t)p->G[1] * (p->G[2] / 2 + 1);
+        const size_t groups = (p->Mc + cols - 1) / cols, total = groups * (size_t)gx * nn * cols;
+        SLLG_TRY(p->NkXc.alloc(total * sizeof(float2)));
+        k_nkc_to_x<<<blocks_for(total, kTPB), kTPB>>>((uint32_t)p->Mc, (uint32_t)gx, cols, nn, total,
+                                                      p->Nkc.as<float2>(), p->NkXc.as<float2>());
+        SLLG_HIP(hipGetLastError());
+        SLLG_HIP(hipDeviceSynchronize());
+        p->Nkc.release();
+        SLLG_TRY(upload_twiddles(p));
+        SLLG_TRY(p->grid.alloc((size_t)3 * nms * p->Pc * rs));
+        SLLG_TRY(p->hgrid.alloc((size_t)3 * nms * p->Pc * rs));
+        SLLG_TRY(p->spec.alloc((size_t)3 * nms * (gx / 2) * p->Mc * 2 * rs));
+        SLLG_HIP(hipMemset(p->grid.p, 0, (size_t)3 * nms * p->Pc * rs));
+        int d2[2] = {p->G[1], p->G[2]};
+        const int planes = (int)(3 * nms) * (gx / 2);
+        SLLG_FFT(hipfftPlanMany(&p->fwd, 2, d2, nullptr, 1, p->G[1] * p->G[2], nullptr, 1, (int)p->Mc, HIPFFT_R2C,
+                                planes));
+        p->have_fwd = true;
+        SLLG_FFT(hipfftPlanMany(&p->inv, 2, d2, nullptr, 1, (int)p->Mc, nullptr, 1, p->G[1] * p->G[2], HIPFFT_C2R,
+                                planes));
+        p->have_inv = true;
+        SLLG_FFT(hipfftSetStream(p->fwd, c->stream));
+        SLLG_FFT(hipfftSetStream(p->inv, c->stream));
+        return 0;
+    }
     SLLG_TRY(p->grid.alloc((size_t)3 * nms * p->P * rs));
     SLLG_TRY(p->hgrid.alloc((size_t)3 * nms * p->P * rs));
     SLLG_TRY(p->spec.alloc((size_t)3 * nms * p->Ph * 2 * rs));
@@ -575,7 +738,7 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
         key.abc[a] = d.abc[a];
     }
     const int gx = p->G[0];
-    p->xh = !p->f64 && gx >= 16 && gx <= 2048 && (gx & (gx - 1)) == 0 && !(c->flags & LLG_FLAG_DIPOLAR_FULL_FFT);
+    p->xh = half_grid_ok(c, p, 1);
     {
         std::lock_guard<std::mutex> lk(g_nk_mu);
         if (p->xh) {
@@ -629,13 +792,7 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
             g_nkx_cache[key] = p->NkX;
         }
         p->Nk.reset();  // the [6][Ph] layout is not read on this path
-        std::vector<float2> tw(gx / 2);
-        for (int m = 0; m < gx / 2; ++m) {
-            const double ang = -2.0 * M_PI * (double)m / (double)gx;
-            tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
-        }
-        SLLG_TRY(p->tw.alloc(tw.size() * sizeof(float2)));
-        SLLG_HIP(hipMemcpy(p->tw.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+        SLLG_TRY(upload_twiddles(p));
         SLLG_TRY(p->grid.alloc(3 * p->Pc * rs));
         SLLG_TRY(p->hgrid.alloc(3 * p->Pc * rs));
         SLLG_TRY(p->spec.alloc(3 * (size_t)(gx / 2) * p->Mc * 2 * rs));
@@ -687,7 +844,7 @@ uint64_t dipolar_bytes(const llg_ctx* c) {
     if (!p) return 0;
     uint64_t b = p->lin.bytes + p->mu.bytes + p->grid.bytes + p->spec.bytes + p->hgrid.bytes + p->sp.bytes +
                  p->Nkc.bytes + p->hspec.bytes + p->tw.bytes + (p->Nk ? p->Nk->bytes : 0) +
-                 (p->NkX ? p->NkX->bytes : 0);
+                 (p->NkX ? p->NkX->bytes : 0) + p->NkXc.bytes;
     size_t w = 0;
     if (p->have_fwd && hipfftGetSize(p->fwd, &w) == HIPFFT_SUCCESS) b += w;
     if (p->have_inv && hipfftGetSize(p->inv, &w) == HIPFFT_SUCCESS) b += w;
@@ -713,6 +870,15 @@ int dipolar_refresh(llg_ctx* c, const void* S, void* out) {
             SLLG_FFT(hipfftExecZ2D(p->inv, p->hspec.as<hipfftDoubleComplex>(), p->hgrid.as<double>()));
             k_fx_gather<double><<<gn, b, 0, c->stream>>>(n, p->P, 1.0 / (double)p->P, lin, sp, p->hgrid.as<double>(),
                                                          (double*)out);
+        } else if (p->xh) {
+            k_fx_scatter<float><<<gn, b, 0, c->stream>>>(n, p->Pc, lin, sp, (const float*)S, p->grid.as<float>());
+            SLLG_HIP(hipGetLastError());
+            SLLG_FFT(hipfftExecR2C(p->fwd, p->grid.as<float>(), p->spec.as<hipfftComplex>()));
+            SLLG_TRY(launch_fx_xfft(p->G[0], p->nms, (uint32_t)p->Mc, p->tw.as<float2>(), p->NkXc.as<float2>(),
+                                    p->spec.as<float2>(), c->stream));
+            SLLG_FFT(hipfftExecC2R(p->inv, p->spec.as<hipfftComplex>(), p->hgrid.as<float>()));
+            k_fx_gather<float><<<gn, b, 0, c->stream>>>(n, p->Pc, 1.0f / (float)p->P, lin, sp, p->hgrid.as<float>(),
+                                                        (float*)out);
         } else {
             k_fx_scatter<float><<<gn, b, 0, c->stream>>>(n, p->P, lin, sp, (const float*)S, p->grid.as<float>());
             SLLG_HIP(hipGetLastError());

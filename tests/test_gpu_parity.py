@@ -541,16 +541,19 @@ def _fftx_system(prec, dip, seed=7, dims=(6, 5, 4)):
     return fft, csr, sllg.random_spins(n, seed, seed + 100)
 
 
-@pytest.mark.parametrize("prec,dip", [("fp64", False), ("fp64", True), ("fp32", False), ("fp32", True)])
-def test_fft_exchange_matches_direct_sum(gpu, prec, dip):
+@pytest.mark.parametrize("prec,dip,dims", [("fp64", False, (6, 5, 4)), ("fp64", True, (6, 5, 4)),
+                                            ("fp32", False, (6, 5, 4)), ("fp32", True, (6, 5, 4)),
+                                            ("fp32", False, (8, 5, 4)), ("fp32", True, (8, 6, 2))])
+def test_fft_exchange_matches_direct_sum(gpu, prec, dip, dims):
     """FFT exchange (species-resolved 3x3 convolution on the zero-padded mesh, with the
     dipolar terms in the same kernel when enabled) equals the direct sum of the same
     couplings: fields at the same spins (exchange only), energies, and 30 Heun steps at
     T = 0 against the oracle running the CSR (plus its brute-force dipolar sum, refreshed
-    every stage)."""
+    every stage).  Gx = 12 transforms the whole padded grid; Gx = 16 (FP32) runs the
+    half-grid path with two species."""
     import pyoracle
 
-    fft, csr, s0 = _fftx_system(prec, dip)
+    fft, csr, s0 = _fftx_system(prec, dip, dims=dims)
     g = sllg.Context(fft)
     h = sllg.Context(csr)
     o = pyoracle.OracleContext(csr)
@@ -570,6 +573,27 @@ def test_fft_exchange_matches_direct_sum(gpu, prec, dip):
     d = _maxdiff(g.get_spins(), o.get_spins())
     print(prec, dip, "30 steps max|dS|", d)
     assert d <= (1e-10 if prec == "fp64" else 1e-5)
+
+
+def test_fft_exchange_half_grid_equals_full_fft(gpu):
+    """One-species FFT exchange on an open film: the half-grid path (2D plane transforms and
+    the x kernel with the complex product) equals the whole-grid 3D transforms to FP32
+    rounding over 20 thermal steps, with and without the dipolar terms."""
+    for dip in (False, True):
+        res = []
+        for flags in (0, sllg.FLAG_DIPOLAR_FULL_FFT):
+            sysd, s0 = W.sc_film_fftx(Lx=32, Ly=12, Lz=4, precision="fp32", dipolar=dip)
+            sysd.flags = flags
+            g = sllg.Context(sysd)
+            g.set_spins(s0)
+            g.set_temperature(50.0)
+            g.step(0, 20)
+            res.append((g.get_spins(), g.field()))
+            g.close()
+        ds = _maxdiff(res[0][0], res[1][0])
+        rel = _maxdiff(res[0][1], res[1][1]) / np.abs(res[1][1]).max()
+        print("dip", dip, "half grid vs full FFT: max|dS|", ds, "field rel", rel)
+        assert ds <= 1e-5 and rel <= 1e-5
 
 
 def test_fft_exchange_rejects_bad_descriptors(gpu):
