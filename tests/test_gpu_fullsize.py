@@ -1,0 +1,89 @@
+"""GPU checks at the benchmark sizes, through properties that do not depend on size.
+
+The oracle takes minutes per step at these sizes, so the reference here is the FP64 plain
+path, which the smaller parity tests pin bit for bit to the oracle (test_gpu_parity.py).
+Tolerances: FP32 vs FP64 trajectories at T = 0 within 1e-4 per spin component and 1e-5 in
+<M>; unit spin norms to FP32 rounding; the damped energy at T = 0 never rises."""
+import numpy as np
+import pytest
+
+import sllg
+from sllg import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _fcc(prec, **kw):
+    return W.fcc_3shell(L=64, precision=prec, **kw)  # 64^3 x 4 = 1,048,576 spins, 42 nnz/row
+
+
+def test_fcc_1m_fp32_tracks_fp64_plain(gpu):
+    """The 1M-spin fcc system of the headline benchmark (line-mode encoding, tuned launch):
+    200 Heun steps at T = 0 from the same random state, FP32 fused path vs FP64 plain path."""
+    sf, s0 = _fcc("fp32")
+    sd, _ = _fcc("fp64")
+    out = []
+    for sysd in (sf, sd):
+        g = sllg.Context(sysd)
+        if sysd is sf:
+            assert "line64" in g.info()["exch_encoding"]
+            g.autotune(5)
+        g.set_spins(s0)
+        g.set_temperature(0.0)
+        g.step(0, 200)
+        out.append((g.get_spins(), g.reduce(energy=False)[0].mean(axis=0)))
+        g.close()
+    d = float(np.abs(out[0][0] - out[1][0]).max())
+    dm = float(np.abs(out[0][1] - out[1][1]).max())
+    print("1M fcc, 200 steps T=0: max|dS| FP32 vs FP64", d, "|d<M>|", dm)
+    assert d <= 1e-4 and dm <= 1e-5
+
+
+def test_fcc_1m_norms_energy_and_reproducibility(gpu):
+    """At full size: spin norms stay 1 to FP32 rounding, the energy at T = 0 (damped, no
+    field) does not rise over 4 blocks of 50 steps, and a thermal run repeated from the same
+    state and seed is bit-identical; the line reader and the packed entries agree to FP32
+    rounding after 40 thermal steps."""
+    sysd, s0 = _fcc("fp32")
+    g = sllg.Context(sysd)
+    g.set_spins(s0)
+    g.set_temperature(0.0)
+    e_prev = g.reduce()[1]
+    for b in range(4):
+        g.step(50 * b, 50)
+        e = g.reduce()[1]
+        assert e <= e_prev + 1e-7 * abs(e_prev), (b, e, e_prev)
+        e_prev = e
+    s = g.get_spins()
+    nrm = np.linalg.norm(s, axis=0)
+    assert float(np.abs(nrm - 1.0).max()) <= 2e-6
+    runs = []
+    for line in (1, 1, 0):
+        g.set_launch(512, 0, 2, 0, 1, line)
+        g.set_spins(s0)
+        g.set_temperature(300.0)
+        g.step(0, 40)
+        runs.append(g.get_spins())
+    g.close()
+    assert np.array_equal(runs[0], runs[1])
+    d = float(np.abs(runs[0] - runs[2]).max())
+    print("1M fcc, 40 thermal steps: line reader vs packed entries max|dS|", d)
+    assert d <= 2e-5
+
+
+def test_film_full_size_dipolar_fp32_tracks_fp64(gpu):
+    """One member of the batched benchmark at its real shape (128 x 128 x 16 sc film, NN
+    exchange, dipolar on the 256 x 256 x 32 padded grid, refreshed every step): the FP32
+    half-grid convolution against the FP64 whole-grid path over 50 steps at T = 0."""
+    out = []
+    for prec in ("fp32", "fp64"):
+        sysd, s0 = W.sc_film_dip(Lx=128, Ly=128, Lz=16, precision=prec, refresh=1)
+        g = sllg.Context(sysd)
+        g.set_spins(s0)
+        g.set_temperature(0.0)
+        g.step(0, 50)
+        out.append(g.get_spins())
+        g.close()
+    d = float(np.abs(out[0] - out[1]).max())
+    print("film 128x128x16 with dipolar, 50 steps T=0: max|dS| FP32 vs FP64", d)
+    assert d <= 1e-5
