@@ -51,6 +51,7 @@ struct llg_batch {
     uint32_t tpb = 256;  // threads per tile (one block per tile)
     int xcd = 1;         // XCD-aware block order
     int line = 1;        // line reader for members with a line encoding (else their packed entries)
+    int g3 = 0;          // gather from AoS3 copies of S and Y (packed and line readers)
     DevBuf red;
     std::vector<std::shared_ptr<ExchStore>> pinned;  // coupling stores kept for future members
 };
@@ -113,16 +114,16 @@ int upload_tables(llg_batch* b) {
             cs.push_back(b->m[k].c);
             t0.push_back(b->m[k].t);
         }
-        SLLG_TRY(batch_build_table(cs, t0, b->tpb, g->dmem, g->dmap, g->ntiles, b->stream));
+        SLLG_TRY(batch_build_table(cs, t0, b->tpb, b->g3 != 0, g->dmem, g->dmap, g->ntiles, b->stream));
     }
     return 0;
 }
 
 int launch_chunk(llg_batch* b, uint32_t nsteps, bool dipolar) {
     for (auto& m : b->m)
-        if (m.c) {
-            m.c->g3_valid = false;    // the batch kernels write S and Y without AoS3 copies
-            m.c->y_soa_valid = true;
+        if (m.c) {  // what the batch kernels leave current: S3 and Y3 with the gather copy, SoA Y without
+            m.c->g3_valid = b->g3 != 0;
+            m.c->y_soa_valid = !b->g3 || batch_needs_y_soa(m.c);
         }
     for (uint32_t k = 0; k < nsteps; ++k) {
         if (dipolar)
@@ -130,13 +131,13 @@ int launch_chunk(llg_batch* b, uint32_t nsteps, bool dipolar) {
                 if (mm.live && mm.c->dip && (mm.t + k) % mm.c->dip_refresh == 0)
                     SLLG_TRY(dipolar_refresh(mm.c, mm.c->S.p, mm.c->DEM.p));
         for (auto& g : b->groups)
-            SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 0, b->tpb, b->xcd));
+            SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 0, b->tpb, b->xcd, b->g3 != 0));
         if (dipolar)
             for (auto& mm : b->m)
                 if (mm.live && mm.c->dip && mm.c->dip_every_stage)
                     SLLG_TRY(dipolar_refresh(mm.c, mm.c->Y.p, mm.c->DEM.p));
         for (auto& g : b->groups)
-            SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 1, b->tpb, b->xcd));
+            SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 1, b->tpb, b->xcd, b->g3 != 0));
     }
     return 0;
 }
@@ -350,10 +351,12 @@ int llg_batch_step(llg_batch* b, uint32_t nsteps) {
     return 0;
 }
 
-int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd, int line_mode) {
+int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd, int line_mode, int gather_copy) {
     if (!b) SLLG_FAIL(LLG_E_INVALID, "NULL batch");
     if (tpb != 128 && tpb != 256 && tpb != 512) SLLG_FAIL(LLG_E_INVALID, "tile of %u threads (128, 256, 512)", tpb);
-    if (tpb != b->tpb || (xcd ? 1 : 0) != b->xcd) drop_graph(b);  // the captured launches have the old shape
+    // the captured launches have the old shape
+    if (tpb != b->tpb || (xcd ? 1 : 0) != b->xcd || (gather_copy ? 1 : 0) != b->g3) drop_graph(b);
+    b->g3 = gather_copy ? 1 : 0;
     if ((line_mode ? 1 : 0) != b->line) b->dirty = true;        // members regroup by the reader they run
     b->tpb = tpb;
     b->xcd = xcd ? 1 : 0;
@@ -376,7 +379,7 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
         t_save[k] = b->m[k].t;
     }
     const uint32_t tpb0 = b->tpb;
-    const int xcd0 = b->xcd, line0 = b->line;
+    const int xcd0 = b->xcd, line0 = b->line, g30 = b->g3;
     bool any_line = false;
     for (auto& mm : b->m) any_line = any_line || (mm.live && is_line_fmt(mm.c->ex32.fmt));
     const int nline = any_line ? 2 : 1;
@@ -386,12 +389,13 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
     const uint32_t tpbs[3] = {128, 256, 512};
     double best = 1e300;
     uint32_t btpb = 256;
-    int bxcd = 1, bline = line0, rc = 0;
+    int bxcd = 1, bline = line0, bg3 = 0, rc = 0;
     for (int li = 0; li < nline && !rc; ++li)
+    for (int gi = 0; gi < 2 && !rc; ++gi)
     for (int ti = 0; ti < 3 && !rc; ++ti)
         for (int xcd = 1; xcd >= 0 && !rc; --xcd) {
             const int line = nline == 2 ? 1 - li : line0;
-            rc = llg_batch_set_launch(b, tpbs[ti], xcd, line);
+            rc = llg_batch_set_launch(b, tpbs[ti], xcd, line, gi);
             if (!rc) rc = llg_batch_step(b, 2);  // warm-up (and graph capture when enabled)
             if (!rc) rc = llg_batch_step(b, nsteps);  // a graph of this length is captured here
             if (rc) break;
@@ -403,13 +407,14 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
             if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = LLG_E_DEVICE;
             if (rc) break;
             const double per = (double)ms / nsteps;
-            if (tpbs[ti] == 256 && xcd == 1 && li == 0) out->ms_per_step_default = per;
+            if (tpbs[ti] == 256 && xcd == 1 && li == 0 && gi == 0) out->ms_per_step_default = per;
             out->candidates++;
             if (per < best) {
                 best = per;
                 btpb = tpbs[ti];
                 bxcd = xcd;
                 bline = line;
+                bg3 = gi;
             }
         }
     (void)hipEventDestroy(e0);
@@ -420,17 +425,19 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
         llg_ctx* c = b->m[k].c;
         const hipError_t e = hipMemcpyAsync(c->S.p, save[k].p, c->S.bytes, hipMemcpyDeviceToDevice, b->stream);
         if (e != hipSuccess && !rc) rc = LLG_E_DEVICE;
+        c->g3_valid = false;  // S changed under the AoS3 copy: rebuilt before the next gather-copy launch
         b->m[k].t = t_save[k];
     }
     if (hipStreamSynchronize(b->stream) != hipSuccess && !rc) rc = LLG_E_DEVICE;
     if (rc) {
-        (void)llg_batch_set_launch(b, tpb0, xcd0, line0);
+        (void)llg_batch_set_launch(b, tpb0, xcd0, line0, g30);
         return rc;
     }
-    SLLG_TRY(llg_batch_set_launch(b, btpb, bxcd, bline));
+    SLLG_TRY(llg_batch_set_launch(b, btpb, bxcd, bline, bg3));
     out->tpb = (int)btpb;
     out->xcd = bxcd;
     out->line_mode = bline;
+    out->gather_copy = bg3;
     out->ms_per_step = best;
     return 0;
 }

@@ -607,11 +607,15 @@ struct BatchMember {
     float* S;
     float* Y;
     float* A;
+    float* S3;      // AoS3 gather copies (launches with the gather copy)
+    float* Y3;
+    uint32_t ysoa;  // the predictor also writes the SoA Y (four-spin term, dipolar at Y)
+    uint32_t pad_;
     uint32_t tile0, ntiles;
     uint64_t t0;  // step of the first step of the current launch chunk
 };
 
-template <int PHASE, int FMT, int NC, int TPB>
+template <int PHASE, int FMT, int NC, int TPB, bool G3>
 __global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restrict__ mem,
                                                     const uint32_t* __restrict__ tile_member, uint32_t step_off,
                                                     int xcd) {
@@ -626,27 +630,30 @@ __global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restric
     const uint32_t n = a.n;
     float s[3], f[3];
     if constexpr (PHASE == 0) {
-        fused_rhs<FMT, NC, 2, false, true, true>(a, i, a.perm.to_ref(i), bm.S, bm.S, sdict, s, f, t);
+        fused_rhs<FMT, NC, 2, G3, true, true>(a, i, a.perm.to_ref(i), bm.S, G3 ? bm.S3 : bm.S, sdict, s, f, t);
         float y[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
         const float r = rsqrtf(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            bm.Y[(size_t)c * n + i] = y[c] * r;
-            bm.A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
+        for (int c = 0; c < 3; ++c) bm.A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
+        if (!G3 || bm.ysoa) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bm.Y[(size_t)c * n + i] = y[c] * r;
         }
+        if constexpr (G3) reinterpret_cast<Spin3*>(bm.Y3)[i] = Spin3{y[0] * r, y[1] * r, y[2] * r};
     } else {
         float av[3];  // issued ahead of the field evaluation
 #pragma unroll
         for (int c = 0; c < 3; ++c) av[c] = bm.A[(size_t)c * n + i];
-        fused_rhs<FMT, NC, 2, false, true, true>(a, i, a.perm.to_ref(i), bm.Y, bm.Y, sdict, s, f, t);
+        fused_rhs<FMT, NC, 2, G3, true, true>(a, i, a.perm.to_ref(i), bm.Y, G3 ? bm.Y3 : bm.Y, sdict, s, f, t);
         float p[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[c] = av[c] + a.half_rdt * f[c];
         const float r = rsqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) bm.S[(size_t)c * n + i] = p[c] * r;
+        if constexpr (G3) reinterpret_cast<Spin3*>(bm.S3)[i] = Spin3{p[0] * r, p[1] * r, p[2] * r};
     }
 }
 
@@ -917,6 +924,9 @@ static void launch_heun_corr(const FusedArgs& a, const HeunBufs& b, uint32_t n, 
 // (12 B written + 12 B read per spin instead of a second Philox + Box-Muller draw).
 static bool cache_thermal(const llg_ctx* c) { return c->tune_thc && c->T > 0.0 && c->noise_mode == LLG_NOISE_COUNTER; }
 
+// Readers of the SoA predicted spins: the four-spin term and a dipolar refresh at Y.
+bool batch_needs_y_soa(const llg_ctx* c) { return c->ex32.fs_xadj || (c->dip && c->dip_every_stage); }
+
 // The AoS3 gather copy is used for the packed and line encodings when the tuner chose it.
 static bool use_g3(const llg_ctx* c) {
     const int f = exec_fmt(c);
@@ -1084,7 +1094,7 @@ int fused_detfield(llg_ctx* c, const float* Sp, double* out) {
 size_t batch_member_size() { return sizeof(BatchMember); }
 
 int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<uint64_t>& t0, uint32_t tpb,
-                      DevBuf& dmem, DevBuf& dmap, uint32_t& ntiles, hipStream_t s) {
+                      bool g3, DevBuf& dmem, DevBuf& dmap, uint32_t& ntiles, hipStream_t s) {
     std::vector<BatchMember> hm(members.size());
     std::vector<uint32_t> map;
     uint32_t tile = 0;
@@ -1095,6 +1105,21 @@ int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<ui
         hm[k].S = c->S.as<float>();
         hm[k].Y = c->Y.as<float>();
         hm[k].A = c->A.as<float>();
+        hm[k].S3 = hm[k].Y3 = nullptr;
+        hm[k].ysoa = 1;
+        hm[k].pad_ = 0;
+        if (g3) {  // gather copies, brought up to date where the spins changed outside the kernels
+            HeunBufs hb;
+            SLLG_TRY(heun_bufs(c, true, hb));
+            if (!c->g3_valid) {
+                k_to_aos3<<<blocks_for(c->n, kTPB), kTPB, 0, s>>>(c->n, hb.S, hb.S3);
+                SLLG_HIP(hipGetLastError());
+                c->g3_valid = true;
+            }
+            hm[k].S3 = hb.S3;
+            hm[k].Y3 = hb.Y3;
+            hm[k].ysoa = batch_needs_y_soa(c) ? 1u : 0u;
+        }
         hm[k].tile0 = tile;
         hm[k].ntiles = blocks_for(c->n, tpb);
         hm[k].t0 = t0[k];
@@ -1111,26 +1136,39 @@ int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<ui
     return 0;
 }
 
-template <int PHASE, int F, int N>
-static void launch_batch_tpb(hipStream_t s, const BatchMember* m, const uint32_t* map, uint32_t ntiles,
-                             uint32_t step_off, uint32_t tpb, int xcd) {
+template <int PHASE, int F, int N, bool G3>
+static void launch_batch_g3(hipStream_t s, const BatchMember* m, const uint32_t* map, uint32_t ntiles,
+                            uint32_t step_off, uint32_t tpb, int xcd) {
     // members of a line format may have slices read from packed entries: reserve the dictionary
     const size_t lds = is_line_fmt(F) ? kDictLds * sizeof(float) : 0;
-    if (tpb == 128) k_heun_batch<PHASE, F, N, 128><<<ntiles, 128, lds, s>>>(m, map, step_off, xcd);
-    else if (tpb == 512) k_heun_batch<PHASE, F, N, 512><<<ntiles, 512, lds, s>>>(m, map, step_off, xcd);
-    else k_heun_batch<PHASE, F, N, 256><<<ntiles, 256, lds, s>>>(m, map, step_off, xcd);
+    if (tpb == 128) k_heun_batch<PHASE, F, N, 128, G3><<<ntiles, 128, lds, s>>>(m, map, step_off, xcd);
+    else if (tpb == 512) k_heun_batch<PHASE, F, N, 512, G3><<<ntiles, 512, lds, s>>>(m, map, step_off, xcd);
+    else k_heun_batch<PHASE, F, N, 256, G3><<<ntiles, 256, lds, s>>>(m, map, step_off, xcd);
+}
+
+// The gather copy applies to the packed and line readers (the others read SoA only).
+template <int PHASE, int F, int N>
+static void launch_batch_tpb(hipStream_t s, const BatchMember* m, const uint32_t* map, uint32_t ntiles,
+                             uint32_t step_off, uint32_t tpb, int xcd, bool g3) {
+    if constexpr (F == FMT_SELL_PACK || is_line_fmt(F)) {
+        if (g3) {
+            launch_batch_g3<PHASE, F, N, true>(s, m, map, ntiles, step_off, tpb, xcd);
+            return;
+        }
+    }
+    launch_batch_g3<PHASE, F, N, false>(s, m, map, ntiles, step_off, tpb, xcd);
 }
 
 int batch_heun_launch(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,
-                      uint32_t step_off, int phase, uint32_t tpb, int xcd) {
+                      uint32_t step_off, int phase, uint32_t tpb, int xcd, bool g3) {
     if (ntiles == 0) return 0;
     if (tpb != 128 && tpb != 256 && tpb != 512) SLLG_FAIL(LLG_E_INVALID, "batch tile of %u threads", tpb);
     const BatchMember* m = static_cast<const BatchMember*>(dmem);
     const uint32_t* map = static_cast<const uint32_t*>(dmap);
 #define SLLG_L(F, N)                                                                   \
     do {                                                                               \
-        if (phase == 0) launch_batch_tpb<0, F, N>(s, m, map, ntiles, step_off, tpb, xcd); \
-        else launch_batch_tpb<1, F, N>(s, m, map, ntiles, step_off, tpb, xcd);            \
+        if (phase == 0) launch_batch_tpb<0, F, N>(s, m, map, ntiles, step_off, tpb, xcd, g3); \
+        else launch_batch_tpb<1, F, N>(s, m, map, ntiles, step_off, tpb, xcd, g3);            \
     } while (0)
     SLLG_DISPATCH_FMT_NC(fmt, nc, SLLG_L);
 #undef SLLG_L

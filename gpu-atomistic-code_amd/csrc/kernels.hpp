@@ -27,10 +27,12 @@ int fused_detfield(llg_ctx* c, const float* Sp, double* out);
 int fused_rk_init_tables();
 // Batched Heun over several FP32 handles of one (format, ncomp) class (k_fused.hip).
 size_t batch_member_size();
+// g3: the kernels gather from AoS3 copies of S and Y (built here where stale)
 int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<uint64_t>& t0, uint32_t tpb,
-                      DevBuf& dmem, DevBuf& dmap, uint32_t& ntiles, hipStream_t s);
+                      bool g3, DevBuf& dmem, DevBuf& dmap, uint32_t& ntiles, hipStream_t s);
 int batch_heun_launch(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,
-                      uint32_t step_off, int phase, uint32_t tpb, int xcd);
+                      uint32_t step_off, int phase, uint32_t tpb, int xcd, bool g3);
+bool batch_needs_y_soa(const llg_ctx* c);  // the member's predictor must also write the SoA Y
 int dipolar_set_stream(llg_ctx* c);
 // Handle over an existing coupling store (capi.hip).
 int create_with_store(const llg_desc* d, const std::shared_ptr<ExchStore>& store, llg_ctx** out);

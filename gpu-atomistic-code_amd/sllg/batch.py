@@ -43,7 +43,8 @@ class ServeStats(C.Structure):
 
 class BatchTune(C.Structure):
     _fields_ = [("tpb", C.c_int), ("xcd", C.c_int), ("ms_per_step", C.c_double),
-                ("ms_per_step_default", C.c_double), ("candidates", C.c_int), ("line_mode", C.c_int)]
+                ("ms_per_step_default", C.c_double), ("candidates", C.c_int), ("line_mode", C.c_int),
+                ("gather_copy", C.c_int)]
 
 
 _bound = False
@@ -70,7 +71,7 @@ def _bind():
     L.llg_batch_get_spins.argtypes = [P, C.c_int, dp, dp, dp]
     L.llg_batch_get_step.argtypes = [P, C.c_int, C.POINTER(C.c_uint64)]
     L.llg_batch_get_info.argtypes = [P, C.POINTER(BatchInfo)]
-    L.llg_batch_set_launch.argtypes = [P, C.c_uint32, C.c_int, C.c_int]
+    L.llg_batch_set_launch.argtypes = [P, C.c_uint32, C.c_int, C.c_int, C.c_int]
     L.llg_batch_autotune.argtypes = [P, C.c_uint32, C.POINTER(BatchTune)]
     L.llg_serve_run.argtypes = [C.POINTER(ServeConfig), C.POINTER(Request), C.c_uint32, C.POINTER(Result), dp,
                                 C.POINTER(C.c_uint64), dp, C.POINTER(C.c_uint64), C.POINTER(ServeStats)]
@@ -127,12 +128,13 @@ class Batch:
     def step(self, nsteps):
         _check(self.L.llg_batch_step(self.h, int(nsteps)), self.L)
 
-    def set_launch(self, tpb, xcd=1, line_mode=1):
-        _check(self.L.llg_batch_set_launch(self.h, int(tpb), int(xcd), int(line_mode)), self.L)
+    def set_launch(self, tpb, xcd=1, line_mode=1, gather_copy=0):
+        _check(self.L.llg_batch_set_launch(self.h, int(tpb), int(xcd), int(line_mode), int(gather_copy)), self.L)
 
     def autotune(self, nsteps=10):
-        """Time every tile size and block order on the live members (their spins and step
-        counters are restored) and keep the fastest; returns the choice."""
+        """Time every tile size, block order and gather copy (and both readers when a member
+        has a line encoding) on the live members (their spins and step counters are
+        restored) and keep the fastest; returns the choice."""
         r = BatchTune()
         _check(self.L.llg_batch_autotune(self.h, int(nsteps), C.byref(r)), self.L)
         return {f: getattr(r, f) for f, _ in BatchTune._fields_}

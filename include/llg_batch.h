@@ -58,15 +58,17 @@ int llg_batch_set_applied(llg_batch* b, int member, const double B[3]);
 int llg_batch_step(llg_batch* b, uint32_t nsteps);
 
 /* Launch shape of the batched kernels: tpb threads per tile (one block per tile: 128, 256
- * or 512), the XCD-aware block order (1) or plain order (0), and for members with a line
- * encoding the line reader (1) or their packed entries (0).  Tile size and block order do
- * not change results (each thread owns one spin of one member); the reader changes the
- * order of the sums to FP32 rounding. */
-int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd, int line_mode);
+ * or 512), the XCD-aware block order (1) or plain order (0), for members with a line
+ * encoding the line reader (1) or their packed entries (0), and gather_copy: neighbours
+ * (and a row's own spin) read from AoS3 copies of S and Y that the kernels keep current (1)
+ * or from the SoA arrays (0; packed and line readers only).  Tile size, block order and the
+ * gather copy do not change results (each thread owns one spin of one member, same sums);
+ * the reader changes the order of the sums to FP32 rounding. */
+int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd, int line_mode, int gather_copy);
 
 /* The batch's own autotuner (no reference counterpart; the reference takes
  * threadsperblock from its config, src/cuda.cu:322-324): times nsteps steps of the live
- * members for every tile size and block order (with hipGraph chunks of nsteps steps when
+ * members for every tile size, block order and gather copy (with hipGraph chunks of nsteps steps when
  * the batch replays graphs) and, when a member has a line encoding, for both readers;
  * keeps the fastest, and restores every member's spins and step counter. */
 typedef struct llg_batch_tune {
@@ -75,6 +77,7 @@ typedef struct llg_batch_tune {
     double ms_per_step_default;  /* 256 threads, XCD order, line reader */
     int candidates;
     int line_mode;
+    int gather_copy;
 } llg_batch_tune;
 int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out);
 

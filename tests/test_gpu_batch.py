@@ -183,15 +183,16 @@ def test_serve_graph_chunks_equal_eager(gpu):
 
 def test_batch_autotune_restores_and_shapes_agree(gpu):
     """The batch autotuner restores every member's spins and step counter, and every tile
-    size / block order gives bit-identical trajectories (one thread per spin)."""
+    size / block order / gather copy gives bit-identical trajectories (one thread per spin,
+    the same sums)."""
     res = []
-    for shape in (None, (128, 0), (512, 1), "tune"):
+    for shape in (None, (128, 0), (512, 1), (256, 1, 1, 1), (128, 0, 1, 1), "tune"):
         bt = SB.Batch(flags=SB.BATCH_GRAPH)
         ids = [bt.add(s, s0, T, B) for s, s0, T, B in _members()]
         before = [bt.get_spins(k) for k in ids]
         if shape == "tune":
             t = bt.autotune(5)
-            assert t["candidates"] == 6 and t["tpb"] in (128, 256, 512)
+            assert t["candidates"] == 12 and t["tpb"] in (128, 256, 512) and t["gather_copy"] in (0, 1)
             for k, s0 in zip(ids, before):
                 assert np.array_equal(bt.get_spins(k), s0)
                 assert bt.get_step(k) == 0
