@@ -64,6 +64,11 @@ struct DipolarPlan {
     DevBuf tw;              // exp(-2 pi i m / Gx), m < Gx / 2 (FP32 pairs from FP64)
     std::shared_ptr<DevBuf> NkX;  // N(k) in the x kernel's block order [group][kx][6][COLS]
     DevBuf NkXc;                  // FFT exchange, half grid: [group][kx][nms^2 9][COLS] complex
+    // half grid with the (y, z) plane kernels in place of hipFFT's 2D transforms and the
+    // scatter / gather (power-of-two Gy, Gz at most 64): no real grids, no hipFFT plans
+    bool yz = false;
+    DevBuf map;       // [species][x][y][z] over the physical extents: device spin or ~0
+    DevBuf twy, twz;  // exp(-2 pi i m / Gy), m < Gy / 2; exp(-2 pi i m / Gz), m < Gz
     ~DipolarPlan() {
         if (have_fwd) hipfftDestroy(fwd);
         if (have_inv) hipfftDestroy(inv);
@@ -174,9 +179,10 @@ __device__ __forceinline__ void xf_load(float2* buf, float2* w, const float2* __
     __syncthreads();
 }
 
-template <int GX, int NSEQ>
-__device__ __forceinline__ void xf_single_stage(float2* buf) {  // length 2: w[0] = 1
-    for (uint32_t q = threadIdx.x; q < (uint32_t)(NSEQ * GX / 2); q += kXT) {
+// nseq sequences of GX elements (a compile-time constant at the x kernels' call sites)
+template <int GX>
+__device__ __forceinline__ void xf_single_stage(float2* buf, uint32_t nseq) {  // length 2: w[0] = 1
+    for (uint32_t q = threadIdx.x; q < nseq * (GX / 2); q += kXT) {
         const uint32_t sq = q / (GX / 2), i0 = xf_at<GX>(sq, 2 * (q % (GX / 2))), i1 = i0 + 1;
         const float2 u = buf[i0], v = buf[i1];
         buf[i0] = cadd(u, v);
@@ -188,14 +194,14 @@ __device__ __forceinline__ void xf_single_stage(float2* buf) {  // length 2: w[0
 // forward DIT: an odd stage count starts with one length-2 stage; then stages L and 2L run
 // as one pass, a thread taking the quartet pos, pos + L/2, pos + L, pos + 3L/2 of a group
 // of 2L (both butterflies of both stages in registers)
-template <int GX, int NSEQ>
-__device__ __forceinline__ void xf_forward(float2* buf, const float2* w) {
+template <int GX>
+__device__ __forceinline__ void xf_forward(float2* buf, const float2* w, uint32_t nseq) {
     constexpr int LG = ilog2c(GX);
-    if constexpr (LG & 1) xf_single_stage<GX, NSEQ>(buf);
+    if constexpr (LG & 1) xf_single_stage<GX>(buf, nseq);
 #pragma unroll
     for (int L = (LG & 1) ? 4 : 2; 2 * L <= GX; L <<= 2) {
         const int qh = L >> 1;
-        for (uint32_t q = threadIdx.x; q < (uint32_t)(NSEQ * GX / 4); q += kXT) {
+        for (uint32_t q = threadIdx.x; q < nseq * (GX / 4); q += kXT) {
             const uint32_t sq = q / (GX / 4), k = q % (GX / 4), pos = k % qh;
             const uint32_t b = (k / qh) * 2 * L + pos;
             const uint32_t i0 = xf_at<GX>(sq, b), i1 = xf_at<GX>(sq, b + qh), i2 = xf_at<GX>(sq, b + L),
@@ -219,13 +225,13 @@ __device__ __forceinline__ void xf_forward(float2* buf, const float2* w) {
 
 // inverse DIF (twiddles conjugated): stages 2L and L as one pass from the top; an odd stage
 // count ends with one length-2 stage
-template <int GX, int NSEQ>
-__device__ __forceinline__ void xf_inverse(float2* buf, const float2* w) {
+template <int GX>
+__device__ __forceinline__ void xf_inverse(float2* buf, const float2* w, uint32_t nseq) {
     constexpr int LG = ilog2c(GX);
 #pragma unroll
     for (int L = GX / 2; L >= ((LG & 1) ? 4 : 2); L >>= 2) {
         const int qh = L >> 1;
-        for (uint32_t q = threadIdx.x; q < (uint32_t)(NSEQ * GX / 4); q += kXT) {
+        for (uint32_t q = threadIdx.x; q < nseq * (GX / 4); q += kXT) {
             const uint32_t sq = q / (GX / 4), k = q % (GX / 4), pos = k % qh;
             const uint32_t b = (k / qh) * 2 * L + pos;
             const uint32_t i0 = xf_at<GX>(sq, b), i1 = xf_at<GX>(sq, b + qh), i2 = xf_at<GX>(sq, b + L),
@@ -241,7 +247,7 @@ __device__ __forceinline__ void xf_inverse(float2* buf, const float2* w) {
         }
         __syncthreads();
     }
-    if constexpr (LG & 1) xf_single_stage<GX, NSEQ>(buf);
+    if constexpr (LG & 1) xf_single_stage<GX>(buf, nseq);
 }
 
 template <int GX, int NS, int COLS>
@@ -251,6 +257,118 @@ __device__ __forceinline__ void xf_store(const float2* buf, float2* __restrict__
         const uint32_t col = e % COLS, x = (e / COLS) % LX, sc = e / (COLS * LX), j = j0 + col;
         if (j < M) spec[((size_t)sc * LX + x) * M + j] = buf[xf_at<GX>(sc * COLS + col, __brev(x) >> (32 - LG))];
     }
+}
+
+// (y, z) plane transforms of the half grid, fused with the scatter and the gather.  A block
+// takes one plane: sequence sc = species * 3 + component, x < Gx/2.  Forward: the plane's
+// occupied points y < Gy/2, z < Gz/2 (mesh point -> device spin from `map`, times the
+// moment for the dipolar convolution) into LDS, a direct DFT along z to kz = 0 .. Gz/2 (the
+// zero half of z costs nothing), a radix-2 FFT along y over the Gz/2 + 1 kz sequences, and
+// the [ky][kz] half spectrum out in hipFFT's R2C layout.  Inverse: the spectrum in, the
+// inverse y FFT, and at each occupied point the real inverse DFT along z (Hermitian half
+// spectrum: X0 + X_{Gz/2} (-1)^z + 2 Re sum X_kz e^{+i...}) written to the spin's field.
+// Dynamic LDS: (Gz/2 + 1) sequences of Gy (+1/32) complex, the plane's real points, the y
+// and z twiddles.
+struct YZArgs {
+    uint32_t n, lx, ly, lz, gz, kz;  // spins; physical extents (G/2); Gz; Gz/2 + 1
+    const uint32_t* map;             // [species][x][y][z] over the physical extents: device spin or ~0
+    const float* mu;                 // moments (dipolar) or null (FFT exchange)
+    const float2* twy;               // exp(-2 pi i m / Gy), m < Gy / 2
+    const float2* twz;               // exp(-2 pi i m / Gz), m < Gz
+    float2* spec;                    // [sc][x < Gx/2][Gy (Gz/2 + 1)]
+};
+
+template <int GY>
+__global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restrict__ S) {
+    constexpr int LG = ilog2c(GY);
+    extern __shared__ float2 yz_lds[];
+    const uint32_t kz = a.kz, sc = blockIdx.x / a.lx, x = blockIdx.x % a.lx, sp = sc / 3, c = sc % 3;
+    float2* const buf = yz_lds;                                        // kz (GY + GY/32)
+    float2* const wy = buf + (size_t)kz * (GY + GY / 32);               // GY / 2
+    float2* const wz = wy + GY / 2;                                     // gz
+    float* const R = reinterpret_cast<float*>(wz + a.gz);               // ly lz
+    for (uint32_t t = threadIdx.x; t < (uint32_t)(GY / 2); t += kXT) wy[t] = a.twy[t];
+    for (uint32_t t = threadIdx.x; t < a.gz; t += kXT) wz[t] = a.twz[t];
+    const uint32_t* const mp = a.map + ((size_t)sp * a.lx + x) * a.ly * a.lz;
+    for (uint32_t e = threadIdx.x; e < a.ly * a.lz; e += kXT) {
+        const uint32_t i = mp[e];
+        R[e] = i == 0xFFFFFFFFu ? 0.f : (a.mu ? a.mu[i] : 1.f) * S[(size_t)c * a.n + i];
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * kz; e += kXT) {
+        const uint32_t y = e / kz, k = e % kz;
+        float2 X = make_float2(0.f, 0.f);
+        if (y < a.ly) {
+            const float* const r = R + (size_t)y * a.lz;
+            uint32_t m = 0;  // k z mod gz
+            for (uint32_t z = 0; z < a.lz; ++z) {
+                const float2 t = wz[m];
+                X.x += r[z] * t.x;
+                X.y += r[z] * t.y;
+                m += k;
+                if (m >= a.gz) m -= a.gz;
+            }
+        }
+        buf[xf_at<GY>(k, __brev(y) >> (32 - LG))] = X;
+    }
+    __syncthreads();
+    xf_forward<GY>(buf, wy, kz);
+    float2* const out = a.spec + ((size_t)sc * a.lx + x) * GY * kz;
+    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * kz; e += kXT) out[e] = buf[xf_at<GY>(e % kz, e / kz)];
+}
+
+template <int GY>
+__global__ __launch_bounds__(kXT) void k_yz_inv(YZArgs a, float invP, float* __restrict__ out) {
+    constexpr int LG = ilog2c(GY);
+    extern __shared__ float2 yz_lds[];
+    const uint32_t kz = a.kz, sc = blockIdx.x / a.lx, x = blockIdx.x % a.lx, sp = sc / 3, c = sc % 3;
+    float2* const buf = yz_lds;
+    float2* const wy = buf + (size_t)kz * (GY + GY / 32);
+    float2* const wz = wy + GY / 2;
+    for (uint32_t t = threadIdx.x; t < (uint32_t)(GY / 2); t += kXT) wy[t] = a.twy[t];
+    for (uint32_t t = threadIdx.x; t < a.gz; t += kXT) wz[t] = a.twz[t];
+    const float2* const in = a.spec + ((size_t)sc * a.lx + x) * GY * kz;
+    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * kz; e += kXT) buf[xf_at<GY>(e % kz, e / kz)] = in[e];
+    __syncthreads();
+    xf_inverse<GY>(buf, wy, kz);
+    const uint32_t* const mp = a.map + ((size_t)sp * a.lx + x) * a.ly * a.lz;
+    const uint32_t nyq = a.gz / 2;
+    for (uint32_t e = threadIdx.x; e < a.ly * a.lz; e += kXT) {
+        const uint32_t i = mp[e];
+        if (i == 0xFFFFFFFFu) continue;
+        const uint32_t y = e / a.lz, z = e % a.lz, yb = __brev(y) >> (32 - LG);
+        float v = buf[xf_at<GY>(0, yb)].x + ((z & 1u) ? -1.f : 1.f) * buf[xf_at<GY>(nyq, yb)].x;
+        float acc = 0.f;
+        uint32_t m = z;  // k z mod gz, from k = 1
+        for (uint32_t k = 1; k < nyq; ++k) {
+            const float2 X = buf[xf_at<GY>(k, yb)], t = wz[m];  // Re(X conj(t))
+            acc += X.x * t.x + X.y * t.y;
+            m += z;
+            if (m >= a.gz) m -= a.gz;
+        }
+        out[(size_t)c * a.n + i] = (v + 2.f * acc) * invP;
+    }
+}
+
+static size_t yz_lds_bytes(int gy, uint32_t gz, uint32_t ly, uint32_t lz) {
+    return ((size_t)(gz / 2 + 1) * (gy + gy / 32) + gy / 2 + gz) * sizeof(float2) + (size_t)ly * lz * sizeof(float);
+}
+
+static int launch_yz(bool fwd, int gy, uint32_t planes, const YZArgs& a, const float* S, float invP, float* out,
+                     hipStream_t s) {
+    const size_t lds = yz_lds_bytes(gy, a.gz, a.ly, a.lz);
+#define SLLG_YZ(G_)                                                                   \
+    case G_:                                                                          \
+        if (fwd) k_yz_fwd<G_><<<planes, kXT, lds, s>>>(a, S);                         \
+        else k_yz_inv<G_><<<planes, kXT, lds, s>>>(a, invP, out);                     \
+        break;
+    switch (gy) {
+        SLLG_YZ(16) SLLG_YZ(32) SLLG_YZ(64) SLLG_YZ(128) SLLG_YZ(256) SLLG_YZ(512) SLLG_YZ(1024)
+        default: SLLG_FAIL(LLG_E_UNSUPPORTED, "y extent %d has no plane transform", gy);
+    }
+#undef SLLG_YZ
+    SLLG_HIP(hipGetLastError());
+    return 0;
 }
 
 // Dipolar: 3 sequences per column, product with the real symmetric N(k) (6 comps, stored
@@ -274,7 +392,7 @@ __global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __re
         for (int q = 0; q < 6; ++q) nreg[r][q] = ng[(size_t)kx * 6 * COLS + q * COLS + col];
     }
     xf_load<GX, 3, COLS>(buf, w, tw, spec, M, j0);
-    xf_forward<GX, 3 * COLS>(buf, w);
+    xf_forward<GX>(buf, w, 3 * COLS);
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
         const uint32_t e = threadIdx.x + r * kXT, col = e % COLS, kx = e / COLS, j = j0 + col;
@@ -290,7 +408,7 @@ __global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __re
         *bz = make_float2(nxz * sx.x + nyz * sy.x + nzz * sz.x, nxz * sx.y + nyz * sy.y + nzz * sz.y);
     }
     __syncthreads();
-    xf_inverse<GX, 3 * COLS>(buf, w);
+    xf_inverse<GX>(buf, w, 3 * COLS);
     xf_store<GX, 3, COLS>(buf, spec, M, j0);
 }
 
@@ -307,7 +425,7 @@ __global__ __launch_bounds__(kXT) void k_fx_xfft(uint32_t M, const float2* __res
     const uint32_t grp = xf_group(), j0 = grp * COLS;
     const float2* const ng = NkX + (size_t)grp * GX * NN * COLS;
     xf_load<GX, 3 * NMS, COLS>(buf, w, tw, spec, M, j0);
-    xf_forward<GX, 3 * NMS * COLS>(buf, w);
+    xf_forward<GX>(buf, w, 3 * NMS * COLS);
     for (uint32_t e = threadIdx.x; e < (uint32_t)(COLS * GX); e += kXT) {
         const uint32_t col = e % COLS, kx = e / COLS, j = j0 + col;
         if (j >= M) continue;
@@ -329,7 +447,7 @@ __global__ __launch_bounds__(kXT) void k_fx_xfft(uint32_t M, const float2* __res
             }
     }
     __syncthreads();
-    xf_inverse<GX, 3 * NMS * COLS>(buf, w);
+    xf_inverse<GX>(buf, w, 3 * NMS * COLS);
     xf_store<GX, 3 * NMS, COLS>(buf, spec, M, j0);
 }
 
@@ -420,6 +538,58 @@ static int upload_twiddles(DipolarPlan* p) {
     SLLG_TRY(p->tw.alloc(tw.size() * sizeof(float2)));
     SLLG_HIP(hipMemcpy(p->tw.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
     return 0;
+}
+
+static int upload_tw(DevBuf& b, int g, int count) {
+    std::vector<float2> tw(count);
+    for (int m = 0; m < count; ++m) {
+        const double ang = -2.0 * M_PI * (double)m / (double)g;
+        tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+    }
+    SLLG_TRY(b.alloc(tw.size() * sizeof(float2)));
+    SLLG_HIP(hipMemcpy(b.p, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// The (y, z) plane kernels apply on the half grid when Gy is a power of two up to 1024, Gz
+// is at most 64 (direct DFT along z) and a plane fits the LDS.
+static bool yz_ok(const DipolarPlan* p) {
+    const int gy = p->G[1], gz = p->G[2];
+    return p->xh && gy >= 16 && gy <= 1024 && (gy & (gy - 1)) == 0 && gz <= 64 &&
+           yz_lds_bytes(gy, (uint32_t)gz, (uint32_t)gy / 2, (uint32_t)gz / 2) <= 60 * 1024;
+}
+
+// mesh point -> device spin per species, and the y / z twiddles
+static int yz_setup(llg_ctx* c, DipolarPlan* p, const llg_dipolar& d, const uint32_t* spin_species, uint32_t nms) {
+    const size_t lx = p->G[0] / 2, ly = p->G[1] / 2, lz = p->G[2] / 2;
+    std::vector<uint32_t> map((size_t)nms * lx * ly * lz, 0xFFFFFFFFu);
+    for (uint32_t i = 0; i < c->n; ++i) {
+        const size_t m0 = d.mesh[(size_t)i * 3], m1 = d.mesh[(size_t)i * 3 + 1], m2 = d.mesh[(size_t)i * 3 + 2];
+        const uint32_t sp = nms > 1 ? (spin_species ? spin_species[i] : (i % c->ns)) : 0;
+        map[(((size_t)sp * lx + m0) * ly + m1) * lz + m2] = c->perm.to_dev(i);
+    }
+    SLLG_TRY(p->map.alloc(map.size() * sizeof(uint32_t)));
+    SLLG_HIP(hipMemcpy(p->map.p, map.data(), map.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    SLLG_TRY(upload_tw(p->twy, p->G[1], p->G[1] / 2));
+    SLLG_TRY(upload_tw(p->twz, p->G[2], p->G[2]));
+    p->yz = true;
+    return 0;
+}
+
+static YZArgs yz_args(const llg_ctx* c, const DipolarPlan* p) {
+    YZArgs a;
+    a.n = c->n;
+    a.lx = (uint32_t)p->G[0] / 2;
+    a.ly = (uint32_t)p->G[1] / 2;
+    a.lz = (uint32_t)p->G[2] / 2;
+    a.gz = (uint32_t)p->G[2];
+    a.kz = a.gz / 2 + 1;
+    a.map = p->map.as<uint32_t>();
+    a.mu = p->general ? nullptr : p->mu.as<float>();
+    a.twy = p->twy.as<float2>();
+    a.twz = p->twz.as<float2>();
+    a.spec = p->spec.as<float2>();
+    return a;
 }
 
 template <class R>
@@ -655,6 +825,10 @@ static int fftexch_create(llg_ctx* c, DipolarPlan* p, const llg_dipolar& d, cons
         SLLG_HIP(hipDeviceSynchronize());
         p->Nkc.release();
         SLLG_TRY(upload_twiddles(p));
+        if (yz_ok(p)) {
+            SLLG_TRY(p->spec.alloc((size_t)3 * nms * (gx / 2) * p->Mc * 2 * rs));
+            return yz_setup(c, p, d, spin_species, nms);
+        }
         SLLG_TRY(p->grid.alloc((size_t)3 * nms * p->Pc * rs));
         SLLG_TRY(p->hgrid.alloc((size_t)3 * nms * p->Pc * rs));
         SLLG_TRY(p->spec.alloc((size_t)3 * nms * (gx / 2) * p->Mc * 2 * rs));
@@ -793,6 +967,10 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
         }
         p->Nk.reset();  // the [6][Ph] layout is not read on this path
         SLLG_TRY(upload_twiddles(p));
+        if (yz_ok(p)) {
+            SLLG_TRY(p->spec.alloc(3 * (size_t)(gx / 2) * p->Mc * 2 * rs));
+            return yz_setup(c, p, d, spin_species, 1);
+        }
         SLLG_TRY(p->grid.alloc(3 * p->Pc * rs));
         SLLG_TRY(p->hgrid.alloc(3 * p->Pc * rs));
         SLLG_TRY(p->spec.alloc(3 * (size_t)(gx / 2) * p->Mc * 2 * rs));
@@ -827,8 +1005,8 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
 
 int dipolar_set_stream(llg_ctx* c) {
     if (!c->dplan) return 0;
-    SLLG_FFT(hipfftSetStream(c->dplan->fwd, c->stream));
-    SLLG_FFT(hipfftSetStream(c->dplan->inv, c->stream));
+    if (c->dplan->have_fwd) SLLG_FFT(hipfftSetStream(c->dplan->fwd, c->stream));
+    if (c->dplan->have_inv) SLLG_FFT(hipfftSetStream(c->dplan->inv, c->stream));
     return 0;
 }
 
@@ -843,7 +1021,8 @@ uint64_t dipolar_bytes(const llg_ctx* c) {
     const DipolarPlan* p = c->dplan;
     if (!p) return 0;
     uint64_t b = p->lin.bytes + p->mu.bytes + p->grid.bytes + p->spec.bytes + p->hgrid.bytes + p->sp.bytes +
-                 p->Nkc.bytes + p->hspec.bytes + p->tw.bytes + (p->Nk ? p->Nk->bytes : 0) +
+                 p->Nkc.bytes + p->hspec.bytes + p->tw.bytes + p->map.bytes + p->twy.bytes + p->twz.bytes +
+                 (p->Nk ? p->Nk->bytes : 0) +
                  (p->NkX ? p->NkX->bytes : 0) + p->NkXc.bytes;
     size_t w = 0;
     if (p->have_fwd && hipfftGetSize(p->fwd, &w) == HIPFFT_SUCCESS) b += w;
@@ -870,6 +1049,13 @@ int dipolar_refresh(llg_ctx* c, const void* S, void* out) {
             SLLG_FFT(hipfftExecZ2D(p->inv, p->hspec.as<hipfftDoubleComplex>(), p->hgrid.as<double>()));
             k_fx_gather<double><<<gn, b, 0, c->stream>>>(n, p->P, 1.0 / (double)p->P, lin, sp, p->hgrid.as<double>(),
                                                          (double*)out);
+        } else if (p->yz) {
+            const YZArgs ya = yz_args(c, p);
+            const uint32_t planes = 3 * p->nms * ya.lx;
+            SLLG_TRY(launch_yz(true, p->G[1], planes, ya, (const float*)S, 0.f, nullptr, c->stream));
+            SLLG_TRY(launch_fx_xfft(p->G[0], p->nms, (uint32_t)p->Mc, p->tw.as<float2>(), p->NkXc.as<float2>(),
+                                    p->spec.as<float2>(), c->stream));
+            SLLG_TRY(launch_yz(false, p->G[1], planes, ya, nullptr, 1.0f / (float)p->P, (float*)out, c->stream));
         } else if (p->xh) {
             k_fx_scatter<float><<<gn, b, 0, c->stream>>>(n, p->Pc, lin, sp, (const float*)S, p->grid.as<float>());
             SLLG_HIP(hipGetLastError());
@@ -905,6 +1091,13 @@ int dipolar_refresh(llg_ctx* c, const void* S, void* out) {
         SLLG_FFT(hipfftExecZ2D(p->inv, p->spec.as<hipfftDoubleComplex>(), p->hgrid.as<double>()));
         k_dip_gather<double><<<gn, b, 0, c->stream>>>(n, p->P, 1.0 / (double)p->P, p->lin.as<uint32_t>(),
                                                       p->hgrid.as<double>(), (double*)out);
+    } else if (p->yz) {
+        const YZArgs ya = yz_args(c, p);
+        const uint32_t planes = 3 * ya.lx;
+        SLLG_TRY(launch_yz(true, p->G[1], planes, ya, (const float*)S, 0.f, nullptr, c->stream));
+        SLLG_TRY(launch_dip_xfft(p->G[0], (uint32_t)p->Mc, p->tw.as<float2>(), p->NkX->as<float>(),
+                                 p->spec.as<float2>(), c->stream));
+        SLLG_TRY(launch_yz(false, p->G[1], planes, ya, nullptr, 1.0f / (float)p->P, (float*)out, c->stream));
     } else if (p->xh) {
         k_dip_scatter<float><<<gn, b, 0, c->stream>>>(n, p->Pc, p->lin.as<uint32_t>(), p->mu.as<float>(),
                                                       (const float*)S, p->grid.as<float>());
