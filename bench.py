@@ -414,7 +414,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "(replicas and requests would be placed whole per GPU, one large system split into slabs "
+              "with a halo exchange per field evaluation); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     modes = {"copy-probe": copy_probe, "accuracy": accuracy, "rk4": rk4_dmi, "batched": batched,
              "serve": serve_mode, "headline": headline, "fft-exchange": fft_exchange}
