@@ -413,7 +413,7 @@ def main():
     ap.add_argument("--start", default="both", choices=["random", "ordered", "both"], help="accuracy: initial state")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(replicas and requests would be placed whole per GPU, one large system split into slabs "
               "with a halo exchange per field evaluation); run with --gpus 1", file=sys.stderr)
         sys.exit(2)

@@ -51,7 +51,7 @@ typedef enum llg_status {
     LLG_E_INVALID = -1,     /* malformed descriptor or argument */
     LLG_E_NOMEM = -2,       /* device allocation failed, or over the HBM budget */
     LLG_E_DEVICE = -3,      /* HIP runtime error */
-    LLG_E_UNSUPPORTED = -4, /* valid request that this build does not implement */
+    LLG_E_UNSUPPORTED = -4, /* valid request that this library does not implement */
     LLG_E_NODEVICE = -5,    /* no GPU visible */
     LLG_E_STATE = -6,       /* call out of order (e.g. no injected noise left) */
     LLG_E_NONFINITE = -7    /* NaN/Inf detected in spins */
