@@ -278,13 +278,17 @@ struct YZArgs {
     float2* spec;                    // [sc][x < Gx/2][Gy (Gz/2 + 1)]
 };
 
+// The forward pass splits a plane's kz range over `parts` blocks (kzp sequences each; the
+// plane's spins are read by each part) for parallelism: the plane count alone is small.
 template <int GY>
-__global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restrict__ S) {
+__global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restrict__ S, uint32_t parts, uint32_t kzp) {
     constexpr int LG = ilog2c(GY);
     extern __shared__ float2 yz_lds[];
-    const uint32_t kz = a.kz, sc = blockIdx.x / a.lx, x = blockIdx.x % a.lx, sp = sc / 3, c = sc % 3;
-    float2* const buf = yz_lds;                                        // kz (GY + GY/32)
-    float2* const wy = buf + (size_t)kz * (GY + GY / 32);               // GY / 2
+    const uint32_t kz = a.kz, plane = blockIdx.x / parts, part = blockIdx.x % parts;
+    const uint32_t sc = plane / a.lx, x = plane % a.lx, sp = sc / 3, c = sc % 3;
+    const uint32_t k0 = part * kzp, nk = (k0 + kzp <= kz ? kzp : kz - k0);  // this block's kz range
+    float2* const buf = yz_lds;                                        // kzp (GY + GY/32)
+    float2* const wy = buf + (size_t)kzp * (GY + GY / 32);              // GY / 2
     float2* const wz = wy + GY / 2;                                     // gz
     float* const R = reinterpret_cast<float*>(wz + a.gz);               // ly lz
     for (uint32_t t = threadIdx.x; t < (uint32_t)(GY / 2); t += kXT) wy[t] = a.twy[t];
@@ -295,8 +299,8 @@ __global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restric
         R[e] = i == 0xFFFFFFFFu ? 0.f : (a.mu ? a.mu[i] : 1.f) * S[(size_t)c * a.n + i];
     }
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * kz; e += kXT) {
-        const uint32_t y = e / kz, k = e % kz;
+    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * nk; e += kXT) {
+        const uint32_t y = e / nk, kk = e % nk, k = k0 + kk;
         float2 X = make_float2(0.f, 0.f);
         if (y < a.ly) {
             const float* const r = R + (size_t)y * a.lz;
@@ -309,12 +313,13 @@ __global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restric
                 if (m >= a.gz) m -= a.gz;
             }
         }
-        buf[xf_at<GY>(k, __brev(y) >> (32 - LG))] = X;
+        buf[xf_at<GY>(kk, __brev(y) >> (32 - LG))] = X;
     }
     __syncthreads();
-    xf_forward<GY>(buf, wy, kz);
-    float2* const out = a.spec + ((size_t)sc * a.lx + x) * GY * kz;
-    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * kz; e += kXT) out[e] = buf[xf_at<GY>(e % kz, e / kz)];
+    xf_forward<GY>(buf, wy, nk);
+    float2* const out = a.spec + ((size_t)sc * a.lx + x) * GY * kz + k0;
+    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * nk; e += kXT)
+        out[(size_t)(e / nk) * kz + e % nk] = buf[xf_at<GY>(e % nk, e / nk)];
 }
 
 template <int GY>
@@ -350,16 +355,18 @@ __global__ __launch_bounds__(kXT) void k_yz_inv(YZArgs a, float invP, float* __r
     }
 }
 
-static size_t yz_lds_bytes(int gy, uint32_t gz, uint32_t ly, uint32_t lz) {
-    return ((size_t)(gz / 2 + 1) * (gy + gy / 32) + gy / 2 + gz) * sizeof(float2) + (size_t)ly * lz * sizeof(float);
+// nseq kz sequences (gz / 2 + 1 for the inverse, a forward part's kzp)
+static size_t yz_lds_bytes(int gy, uint32_t gz, uint32_t ly, uint32_t lz, uint32_t nseq) {
+    return ((size_t)nseq * (gy + gy / 32) + gy / 2 + gz) * sizeof(float2) + (size_t)ly * lz * sizeof(float);
 }
 
 static int launch_yz(bool fwd, int gy, uint32_t planes, const YZArgs& a, const float* S, float invP, float* out,
                      hipStream_t s) {
-    const size_t lds = yz_lds_bytes(gy, a.gz, a.ly, a.lz);
+    const uint32_t parts = fwd ? 2u : 1u, kzp = (a.kz + parts - 1) / parts;  // a.kz >= 2: no empty part
+    const size_t lds = yz_lds_bytes(gy, a.gz, a.ly, a.lz, kzp);
 #define SLLG_YZ(G_)                                                                   \
     case G_:                                                                          \
-        if (fwd) k_yz_fwd<G_><<<planes, kXT, lds, s>>>(a, S);                         \
+        if (fwd) k_yz_fwd<G_><<<planes * parts, kXT, lds, s>>>(a, S, parts, kzp);    \
         else k_yz_inv<G_><<<planes, kXT, lds, s>>>(a, invP, out);                     \
         break;
     switch (gy) {
@@ -556,7 +563,7 @@ static int upload_tw(DevBuf& b, int g, int count) {
 static bool yz_ok(const DipolarPlan* p) {
     const int gy = p->G[1], gz = p->G[2];
     return p->xh && gy >= 16 && gy <= 1024 && (gy & (gy - 1)) == 0 && gz <= 64 &&
-           yz_lds_bytes(gy, (uint32_t)gz, (uint32_t)gy / 2, (uint32_t)gz / 2) <= 60 * 1024;
+           yz_lds_bytes(gy, (uint32_t)gz, (uint32_t)gy / 2, (uint32_t)gz / 2, (uint32_t)gz / 2 + 1) <= 60 * 1024;
 }
 
 // mesh point -> device spin per species, and the y / z twiddles
