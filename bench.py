@@ -8,8 +8,11 @@ spins; value = spin-updates/s = n * K / (timed seconds) with the state resident 
 
 Also measured in the same run: the FP64 plain path (reference kernel structure, CSR with
 3 FP64 values per nnz as the reference stores diagonal exchange) on the same lattice,
-the per-kernel mean launch time from HIP events on the handle's stream (roofline), and
-the FP64 oracle on one host core over a bounded sample (cpu_baseline).
+the per-kernel mean launch time from HIP events on the handle's stream (roofline), the
+FP64 oracle on one host core over a bounded sample (cpu_baseline), and the serving half of
+the metric: 96 M(T) / field-ramp / spin-wave requests (L 10-40, 1000-5000 steps, Poisson
+arrivals at 200/s) through the batched FP32 request loop, solves/s and p50/p99 latency,
+against the sequential FP64 plain solves of the same requests (`serving`).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|copy-probe|accuracy|rk4|batched|serve|fft-exchange]
 """
@@ -145,6 +148,18 @@ def headline(args):
     p.close()
 
     cpu = None if args.no_cpu else cpu_baseline(L, args.cpu_budget)
+    serving = None
+    if not args.no_serve:
+        # the metric's serving half: 96 requests (L 10-40, 1000-5000 steps, 200/s), batched
+        # FP32 loop vs the sequential FP64 plain solves of the same requests
+        reqs = serve_requests(args.seed, 96, 200.0, 10, 40, 1000, 5000)
+        srv = serve_run(reqs, args.budget_gb, True)
+        b, q = srv["batched_fp32"], srv["sequential_fp64_plain"]
+        serving = {"requests": 96, "rate_per_s": 200.0, "L": [10, 40], "steps": [1000, 5000],
+                   "types": "M(T), field ramp, spin wave", "batched_solves_per_s": b["solves_per_s"],
+                   "p50_latency_s": b["p50_latency_s"], "p99_latency_s": b["p99_latency_s"],
+                   "sequential_fp64_plain_solves_per_s": q["solves_per_s"],
+                   "sequential_p99_latency_s": q["p99_latency_s"], "solves_per_s_ratio": srv["solves_per_s_ratio"]}
     line = {
         "metric": METRIC,
         "value": value,
@@ -172,6 +187,7 @@ def headline(args):
         "kernels_ms": prof,
         "launch_tune": tune,
         "unshared_encoding": unshared,
+        "serving": serving,
         "wall_s_timed": wall,
     }
     print(json.dumps(line), flush=True)
@@ -306,37 +322,51 @@ def batched(args):
                       "m_z_lowest_T": float(m[0][2]), "m_z_highest_T": float(m[-1][2])}), flush=True)
 
 
-def serve_mode(args):
+def serve_requests(seed, nreq, rate, lmin, lmax, smin, smax):
     """Poisson stream of simple-cubic M(T), field-ramp and spin-wave requests (one third
     each; spin-wave requests sample S(k) at (q, 0, 0) every `update` steps), L log-uniform in
-    [10, 100], T uniform in [0, 1.5 Tc], steps uniform in [smin, smax]; batched FP32 loop
-    (graph-captured chunks) vs sequential FP64 plain solves, p50/p99 latency."""
+    [lmin, lmax], T uniform in [0, 1.5 Tc], steps uniform in [smin, smax] (whole hundreds)."""
     import sllg
     from sllg import batch as SB
 
-    rng = np.random.default_rng(args.seed)
+    rng = np.random.default_rng(seed)
     Tc = 1.44 * 3e-21 / sllg.K_B
     specs = []
     t = 0.0
-    for k in range(args.requests):
-        t += rng.exponential(1.0 / args.rate)
-        L = int(round(10 ** rng.uniform(np.log10(args.lmin), np.log10(args.lmax))))
+    for k in range(nreq):
+        t += rng.exponential(1.0 / rate)
+        L = int(round(10 ** rng.uniform(np.log10(lmin), np.log10(lmax))))
         typ = (SB.REQ_MT, SB.REQ_RAMP, SB.REQ_SW)[int(rng.integers(3))]
-        specs.append(dict(type=typ, L=L, steps=int(rng.integers(args.smin, args.smax + 1)) // 100 * 100,
+        specs.append(dict(type=typ, L=L, steps=int(rng.integers(smin, smax + 1)) // 100 * 100,
                           update=100, T=float(rng.uniform(0, 1.5 * Tc)), arrival_s=t, seed=1000 + k,
                           B0=(0, 0, -0.5) if typ == SB.REQ_RAMP else (0, 0, 0), B1=(0, 0, 0.5) if typ == SB.REQ_RAMP else (0, 0, 0)))
-    reqs = SB.make_requests(specs)
-    # untimed warm-up: the first serve call in a process pays module loading and first launches
+    return SB.make_requests(specs)
+
+
+def serve_run(reqs, budget_gb, sequential):
+    """Batched FP32 request loop (graph-captured chunks) after an untimed warm-up (the first
+    serve call in a process pays module loading and first launches); optionally the
+    sequential FP64 plain solves of the same requests."""
+    import sllg
+    from sllg import batch as SB
+
     SB.serve(SB.make_requests([dict(L=8, steps=200, update=100, T=100.0, arrival_s=0.0, seed=1),
                                dict(L=12, steps=200, update=100, T=100.0, arrival_s=0.0, seed=2)]), graph=True)
-    # hipGraph chunks of `update` steps (captured again when the members change)
-    _, st, _ = SB.serve(reqs, budget_bytes=int(args.budget_gb * 2**30), graph=True)
-    out = {"mode": "serve", "requests": args.requests, "rate_per_s": args.rate,
-           "batched_fp32": st}
-    if args.sequential:
+    _, st, _ = SB.serve(reqs, budget_bytes=int(budget_gb * 2**30), graph=True)
+    out = {"batched_fp32": st}
+    if sequential:
         _, sq, _ = SB.serve(reqs, sequential=True, precision=sllg.FP64_PLAIN)
         out["sequential_fp64_plain"] = sq
         out["solves_per_s_ratio"] = st["solves_per_s"] / sq["solves_per_s"]
+    return out
+
+
+def serve_mode(args):
+    """Request stream of serve_requests: batched FP32 loop vs sequential FP64 plain solves,
+    p50/p99 latency."""
+    reqs = serve_requests(args.seed, args.requests, args.rate, args.lmin, args.lmax, args.smin, args.smax)
+    out = {"mode": "serve", "requests": args.requests, "rate_per_s": args.rate}
+    out.update(serve_run(reqs, args.budget_gb, args.sequential))
     print(json.dumps(out), flush=True)
 
 
@@ -409,6 +439,7 @@ def main():
     ap.add_argument("--smax", type=int, default=20000)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--sequential", action="store_true", help="serve: also run the sequential FP64 plain comparator")
+    ap.add_argument("--no-serve", action="store_true", help="headline: skip the serving measurement")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--start", default="both", choices=["random", "ordered", "both"], help="accuracy: initial state")
     args = ap.parse_args()
