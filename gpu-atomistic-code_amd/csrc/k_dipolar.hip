@@ -883,15 +883,20 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
     p->Ph = (size_t)p->G[0] * p->G[1] * (p->G[2] / 2 + 1);
     if (fx) return fftexch_create(c, p, d, spin_species, *fx, dip_terms);
     const uint32_t n = c->n;
-    // grid point of each spin, indexed in the handle's device order
+    // grid point of each spin, indexed in the handle's device order; one spin per point (two
+    // moments at one point have no finite dipolar coupling)
     std::vector<uint32_t> lin(n);
+    std::vector<uint8_t> used(p->P, 0);
     for (uint32_t i = 0; i < n; ++i) {
         uint32_t m[3];
         for (int a = 0; a < 3; ++a) {
             m[a] = d.mesh[(size_t)i * 3 + a];
             if (m[a] >= d.dim[a] * d.nk[a]) SLLG_FAIL(LLG_E_INVALID, "mesh coordinate of spin %u outside dim*nk", i);
         }
-        lin[c->perm.to_dev(i)] = (uint32_t)(((size_t)m[0] * p->G[1] + m[1]) * p->G[2] + m[2]);
+        const size_t g = ((size_t)m[0] * p->G[1] + m[1]) * p->G[2] + m[2];
+        if (used[g]) SLLG_FAIL(LLG_E_INVALID, "two spins share a mesh point (spin %u)", i);
+        used[g] = 1;
+        lin[c->perm.to_dev(i)] = (uint32_t)g;
     }
     const size_t rs = p->f64 ? sizeof(double) : sizeof(float);
     SLLG_TRY(p->lin.alloc(n * sizeof(uint32_t)));

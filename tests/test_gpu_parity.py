@@ -200,6 +200,17 @@ def test_dipolar_half_grid_equals_full_fft(gpu):
         assert ds <= 1e-5 and rel <= 1e-5
 
 
+def test_dipolar_rejects_shared_mesh_point(gpu):
+    """Two spins on one mesh point have no finite dipolar coupling: rejected at create."""
+    sysd, _ = W.sc_film_dip(Lx=8, Ly=6, Lz=4, precision="fp32")
+    mesh = np.array(sysd.dip["mesh"]).copy()
+    mesh[3] = mesh[0]
+    sysd.dip = {**sysd.dip, "mesh": mesh}
+    with pytest.raises(sllg.LLGError) as e:
+        sllg.Context(sysd)
+    assert e.value.code == -1 and "share a mesh point" in str(e.value)
+
+
 def test_invalid_descriptor_rejected(gpu):
     sysd, s0, _ = W.sc_ferro(L=4, precision="fp32")
     bad = dict(sysd.exch)
