@@ -66,7 +66,7 @@ struct DipolarPlan {
     DevBuf NkXc;                  // FFT exchange, half grid: [group][kx][nms^2 9][COLS] complex
     // half grid with the (y, z) plane kernels in place of hipFFT's 2D transforms and the
     // scatter / gather (power-of-two Gy, Gz at most 64): no real grids, no hipFFT plans
-    bool yz = false;
+    bool yz = false, yz_dense = false;  // yz_dense: the map below is the identity
     DevBuf map;       // [species][x][y][z] over the physical extents: device spin or ~0
     DevBuf twy, twz;  // exp(-2 pi i m / Gy), m < Gy / 2; exp(-2 pi i m / Gz), m < Gz
     ~DipolarPlan() {
@@ -272,6 +272,7 @@ __device__ __forceinline__ void xf_store(const float2* buf, float2* __restrict__
 struct YZArgs {
     uint32_t n, lx, ly, lz, gz, kz;  // spins; physical extents (G/2); Gz; Gz/2 + 1
     const uint32_t* map;             // [species][x][y][z] over the physical extents: device spin or ~0
+    uint32_t dense;                  // the map is the identity (one species, every point, mesh order)
     const float* mu;                 // moments (dipolar) or null (FFT exchange)
     const float2* twy;               // exp(-2 pi i m / Gy), m < Gy / 2
     const float2* twz;               // exp(-2 pi i m / Gz), m < Gz
@@ -294,8 +295,9 @@ __global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restric
     for (uint32_t t = threadIdx.x; t < (uint32_t)(GY / 2); t += kXT) wy[t] = a.twy[t];
     for (uint32_t t = threadIdx.x; t < a.gz; t += kXT) wz[t] = a.twz[t];
     const uint32_t* const mp = a.map + ((size_t)sp * a.lx + x) * a.ly * a.lz;
+    const uint32_t i0 = x * a.ly * a.lz;  // first spin of the plane when the map is the identity
     for (uint32_t e = threadIdx.x; e < a.ly * a.lz; e += kXT) {
-        const uint32_t i = mp[e];
+        const uint32_t i = a.dense ? i0 + e : mp[e];
         R[e] = i == 0xFFFFFFFFu ? 0.f : (a.mu ? a.mu[i] : 1.f) * S[(size_t)c * a.n + i];
     }
     __syncthreads();
@@ -337,9 +339,9 @@ __global__ __launch_bounds__(kXT) void k_yz_inv(YZArgs a, float invP, float* __r
     __syncthreads();
     xf_inverse<GY>(buf, wy, kz);
     const uint32_t* const mp = a.map + ((size_t)sp * a.lx + x) * a.ly * a.lz;
-    const uint32_t nyq = a.gz / 2;
+    const uint32_t nyq = a.gz / 2, i0 = x * a.ly * a.lz;
     for (uint32_t e = threadIdx.x; e < a.ly * a.lz; e += kXT) {
-        const uint32_t i = mp[e];
+        const uint32_t i = a.dense ? i0 + e : mp[e];
         if (i == 0xFFFFFFFFu) continue;
         const uint32_t y = e / a.lz, z = e % a.lz, yb = __brev(y) >> (32 - LG);
         float v = buf[xf_at<GY>(0, yb)].x + ((z & 1u) ? -1.f : 1.f) * buf[xf_at<GY>(nyq, yb)].x;
@@ -575,6 +577,10 @@ static int yz_setup(llg_ctx* c, DipolarPlan* p, const llg_dipolar& d, const uint
         const uint32_t sp = nms > 1 ? (spin_species ? spin_species[i] : (i % c->ns)) : 0;
         map[(((size_t)sp * lx + m0) * ly + m1) * lz + m2] = c->perm.to_dev(i);
     }
+    // identity map: one species, every mesh point holds the spin of its own linear index
+    bool dense = nms == 1 && (size_t)c->n == map.size();
+    for (size_t e = 0; dense && e < map.size(); ++e) dense = map[e] == (uint32_t)e;
+    p->yz_dense = dense;
     SLLG_TRY(p->map.alloc(map.size() * sizeof(uint32_t)));
     SLLG_HIP(hipMemcpy(p->map.p, map.data(), map.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     SLLG_TRY(upload_tw(p->twy, p->G[1], p->G[1] / 2));
@@ -592,6 +598,7 @@ static YZArgs yz_args(const llg_ctx* c, const DipolarPlan* p) {
     a.gz = (uint32_t)p->G[2];
     a.kz = a.gz / 2 + 1;
     a.map = p->map.as<uint32_t>();
+    a.dense = p->yz_dense ? 1u : 0u;
     a.mu = p->general ? nullptr : p->mu.as<float>();
     a.twy = p->twy.as<float2>();
     a.twz = p->twz.as<float2>();
