@@ -133,6 +133,13 @@ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 
 constexpr int kXT = 512;  // threads of k_dip_xfft
 
+// (y, kz) columns per block of the x kernels: 3 nms COLS sequences of Gx, about 12.5 KB of
+// LDS, so blocks outnumber the resident slots and one block's loads overlap another's LDS
+// stages (2048 / Gx columns per block: 33 us, 1024 / Gx: 27 us, 512 / Gx: 25 us for the
+// 256 x 256 x 32 grid); at least one column and whole (column, kx) pairs per thread.  Host
+// permutes use the same function.
+constexpr int xf_cols(int gx, int nms) { return 512 / (gx * nms) > 0 ? 512 / (gx * nms) : 1; }
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 w, float2 v) {
@@ -148,8 +155,8 @@ __device__ __forceinline__ float2 cmulc(float2 w, float2 v) {
 // Gx/2 to Gx, runs a radix-2 DIT forward FFT along x (bit-reversed load, natural-order
 // result, two stages per LDS pass), forms the k-space product, runs a radix-2 DIF inverse
 // FFT (natural-order input, bit-reversed result) and writes back x < Gx/2 only.
-// Unnormalised both ways (the gather scales by 1/P).  NS COLS = 3 * 2048 / Gx sequences:
-// about 50 KB of LDS for every Gx.  A sequence's element i sits at i + i / 32 (one pad slot
+// Unnormalised both ways (the gather scales by 1/P).  NS COLS sequences of Gx, COLS from
+// xf_cols: about 12.5 KB of LDS (more where one column is already that much).  A sequence's element i sits at i + i / 32 (one pad slot
 // per 32): the bit-reversed load and store and the column-major product would otherwise
 // hit one LDS bank pair.
 template <int GX>
@@ -386,7 +393,7 @@ static int launch_yz(bool fwd, int gy, uint32_t planes, const YZArgs& a, const f
 template <int GX>
 __global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __restrict__ tw,
                                                   const float* __restrict__ NkX, float2* __restrict__ spec) {
-    constexpr int COLS = 2048 / GX, LG = ilog2c(GX), NPT = COLS * GX / kXT;
+    constexpr int COLS = xf_cols(GX, 1), LG = ilog2c(GX), NPT = COLS * GX / kXT;
     static_assert(COLS >= 1 && (1 << LG) == GX, "power-of-two x extent of at most 2048");
     static_assert((COLS * GX) % kXT == 0, "whole (column, kx) pairs per thread");
     __shared__ float2 buf[3 * COLS * (GX + GX / 32)];
@@ -427,7 +434,7 @@ __global__ __launch_bounds__(kXT) void k_dip_xfft(uint32_t M, const float2* __re
 template <int GX, int NMS>
 __global__ __launch_bounds__(kXT) void k_fx_xfft(uint32_t M, const float2* __restrict__ tw,
                                                  const float2* __restrict__ NkX, float2* __restrict__ spec) {
-    constexpr int COLS = 2048 / (GX * NMS), LG = ilog2c(GX), NN = NMS * NMS * 9;
+    constexpr int COLS = xf_cols(GX, NMS), LG = ilog2c(GX), NN = NMS * NMS * 9;
     static_assert(COLS >= 1 && (1 << LG) == GX, "power-of-two x extent, GX NMS at most 2048");
     __shared__ float2 buf[3 * NMS * COLS * (GX + GX / 32)];
     __shared__ float2 w[GX / 2];
@@ -490,7 +497,7 @@ __global__ __launch_bounds__(kTPB) void k_nkc_to_x(uint32_t M, uint32_t gx, uint
 static int launch_dip_xfft(int gx, uint32_t M, const float2* tw, const float* Nk, float2* spec, hipStream_t s) {
 #define SLLG_XF(G_)                                                                          \
     case G_:                                                                                 \
-        k_dip_xfft<G_><<<(unsigned)((M + 2048 / G_ - 1) / (2048 / G_)), kXT, 0, s>>>(M, tw, Nk, spec); \
+        k_dip_xfft<G_><<<(unsigned)((M + xf_cols(G_, 1) - 1) / xf_cols(G_, 1)), kXT, 0, s>>>(M, tw, Nk, spec); \
         break;
     switch (gx) {
         SLLG_XF(16) SLLG_XF(32) SLLG_XF(64) SLLG_XF(128) SLLG_XF(256) SLLG_XF(512) SLLG_XF(1024) SLLG_XF(2048)
@@ -505,7 +512,7 @@ static int launch_fx_xfft(int gx, uint32_t nms, uint32_t M, const float2* tw, co
                           hipStream_t s) {
 #define SLLG_XF(G_, S_)                                                                              \
     case G_:                                                                                         \
-        k_fx_xfft<G_, S_><<<(unsigned)((M + 2048 / (G_ * S_) - 1) / (2048 / (G_ * S_))), kXT, 0, s>>>(M, tw, Nk, \
+        k_fx_xfft<G_, S_><<<(unsigned)((M + xf_cols(G_, S_) - 1) / xf_cols(G_, S_)), kXT, 0, s>>>(M, tw, Nk, \
                                                                                                    spec); \
         break;
     if (nms == 1) {
@@ -828,7 +835,7 @@ static int fftexch_create(llg_ctx* c, DipolarPlan* p, const llg_dipolar& d, cons
         // half grid: [nms][3][Gx/2][Gy][Gz] real, [nms][3][Gx/2][Mc] complex (x slowest, so
         // lin is unchanged); the kernel in k_fx_xfft's block order replaces Nkc
         const int gx = p->G[0];
-        const uint32_t cols = 2048u / ((uint32_t)gx * nms), nn = nms * nms * 9;
+        const uint32_t cols = (uint32_t)xf_cols(gx, (int)nms), nn = nms * nms * 9;
         p->Pc = p->P / 2;
         p->Mc = (size_t)p->G[1] * (p->G[2] / 2 + 1);
         const size_t groups = (p->Mc + cols - 1) / cols, total = groups * (size_t)gx * nn * cols;
@@ -973,7 +980,7 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
         p->Pc = p->P / 2;
         p->Mc = (size_t)p->G[1] * (p->G[2] / 2 + 1);
         if (!p->NkX) {  // N(k) in the x kernel's block order, shared like N(k) itself
-            const uint32_t cols = 2048u / (uint32_t)gx;
+            const uint32_t cols = (uint32_t)xf_cols(gx, 1);
             const size_t groups = (p->Mc + cols - 1) / cols, total = groups * (size_t)gx * 6 * cols;
             p->NkX = std::make_shared<DevBuf>();
             SLLG_TRY(p->NkX->alloc(total * sizeof(float)));
