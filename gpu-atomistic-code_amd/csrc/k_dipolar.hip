@@ -286,8 +286,9 @@ struct YZArgs {
     float2* spec;                    // [sc][x < Gx/2][Gy (Gz/2 + 1)]
 };
 
-// The forward pass splits a plane's kz range over `parts` blocks (kzp sequences each; the
-// plane's spins are read by each part) for parallelism: the plane count alone is small.
+// The forward pass splits a plane's kz range over `parts` blocks (kzp sequences each, none
+// empty; the plane's spins are read by each part) for parallelism: the plane count alone is
+// small.
 template <int GY>
 __global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restrict__ S, uint32_t parts, uint32_t kzp) {
     constexpr int LG = ilog2c(GY);
@@ -371,7 +372,10 @@ static size_t yz_lds_bytes(int gy, uint32_t gz, uint32_t ly, uint32_t lz, uint32
 
 static int launch_yz(bool fwd, int gy, uint32_t planes, const YZArgs& a, const float* S, float invP, float* out,
                      hipStream_t s) {
-    const uint32_t parts = fwd ? 2u : 1u, kzp = (a.kz + parts - 1) / parts;  // a.kz >= 2: no empty part
+    // forward: two kz parts (kz >= 2, so neither is empty; four parts measured slower, the
+    // plane's spins being read once per part)
+    const uint32_t parts = fwd ? 2u : 1u;
+    const uint32_t kzp = (a.kz + parts - 1) / parts;
     const size_t lds = yz_lds_bytes(gy, a.gz, a.ly, a.lz, kzp);
 #define SLLG_YZ(G_)                                                                   \
     case G_:                                                                          \
