@@ -908,6 +908,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     const int line0 = c->tune_line;
     double best = 1e300;
     TuneChoice b{256, 1, 2, c->tune_thc, c->tune_g3, c->tune_line};
+    std::vector<std::pair<double, TuneChoice>> timed;  // every candidate's first timing
     hipEvent_t e0, e1;
     SLLG_HIP(hipEventCreate(&e0));
     SLLG_HIP(hipEventCreate(&e1));
@@ -938,11 +939,48 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
                 if (tpbs[ti] == 256 && xcd == 1 && unrs[ui] == 2 && hi == 0 && gi == 0 && li == 0)
                     out->ms_per_step_default = per;
                 out->candidates++;
+                timed.push_back({per, TuneChoice{tpbs[ti], xcd, unrs[ui], c->tune_thc, c->tune_g3, c->tune_line}});
                 if (per < best) {
                     best = per;
-                    b = TuneChoice{tpbs[ti], xcd, unrs[ui], c->tune_thc, c->tune_g3, c->tune_line};
+                    b = timed.back().second;
                 }
             }
+    // The minimum of many short noisy timings favours a lucky candidate: the fastest few are
+    // timed again, in turn, over four times as many steps, and the best of those is kept.
+    if (!rc && timed.size() > 1) {
+        std::sort(timed.begin(), timed.end(),
+                  [](const std::pair<double, TuneChoice>& x, const std::pair<double, TuneChoice>& y) {
+                      return x.first < y.first;
+                  });
+        const size_t top = std::min<size_t>(4, timed.size());
+        std::vector<double> acc(top, 0.0);
+        for (int pass = 0; pass < 2 && !rc; ++pass)
+            for (size_t k = 0; k < top && !rc; ++k) {
+                const TuneChoice& t = timed[k].second;
+                c->tune_tpb = t.tpb;
+                c->tune_xcd = t.xcd;
+                c->tune_unr = t.unr;
+                c->tune_thc = t.thc;
+                c->tune_g3 = t.g3;
+                c->tune_line = t.line;
+                rc = do_steps(c, 0, 2);
+                if (rc) break;
+                (void)hipEventRecord(e0, c->stream);
+                rc = do_steps(c, 2, 2 * nsteps);
+                (void)hipEventRecord(e1, c->stream);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                acc[k] += ms / (2.0 * nsteps);
+            }
+        if (!rc) {
+            size_t kb = 0;
+            for (size_t k = 1; k < top; ++k)
+                if (acc[k] < acc[kb]) kb = k;
+            b = timed[kb].second;
+            best = acc[kb] / 2.0;
+        }
+    }
     c->noise_mode = noise;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
