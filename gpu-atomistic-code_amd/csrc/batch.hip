@@ -390,6 +390,12 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
     double best = 1e300;
     uint32_t btpb = 256;
     int bxcd = 1, bline = line0, bg3 = 0, rc = 0;
+    struct Cand {
+        double per;
+        uint32_t tpb;
+        int xcd, line, g3;
+    };
+    std::vector<Cand> timed;  // every candidate's first timing
     for (int li = 0; li < nline && !rc; ++li)
     for (int gi = 0; gi < 2 && !rc; ++gi)
     for (int ti = 0; ti < 3 && !rc; ++ti)
@@ -409,6 +415,7 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
             const double per = (double)ms / nsteps;
             if (tpbs[ti] == 256 && xcd == 1 && li == 0 && gi == 0) out->ms_per_step_default = per;
             out->candidates++;
+            timed.push_back(Cand{per, tpbs[ti], xcd, line, gi});
             if (per < best) {
                 best = per;
                 btpb = tpbs[ti];
@@ -417,6 +424,37 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
                 bg3 = gi;
             }
         }
+    // the fastest few are timed again, in turn, over twice as many steps in two passes (the
+    // minimum of many short noisy timings favours a lucky candidate)
+    if (!rc && timed.size() > 1) {
+        std::sort(timed.begin(), timed.end(), [](const Cand& x, const Cand& y) { return x.per < y.per; });
+        const size_t top = std::min<size_t>(3, timed.size());
+        std::vector<double> acc(top, 0.0);
+        for (int pass = 0; pass < 2 && !rc; ++pass)
+            for (size_t k = 0; k < top && !rc; ++k) {
+                const Cand& t = timed[k];
+                rc = llg_batch_set_launch(b, t.tpb, t.xcd, t.line, t.g3);
+                if (!rc) rc = llg_batch_step(b, 2);
+                if (!rc) rc = llg_batch_step(b, 2 * nsteps);  // captures a graph of this length
+                if (!rc && hipEventRecord(e0, b->stream) != hipSuccess) rc = LLG_E_DEVICE;
+                if (!rc) rc = llg_batch_step(b, 2 * nsteps);
+                if (!rc && hipEventRecord(e1, b->stream) != hipSuccess) rc = LLG_E_DEVICE;
+                if (!rc && hipEventSynchronize(e1) != hipSuccess) rc = LLG_E_DEVICE;
+                float ms = 0.f;
+                if (!rc && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = LLG_E_DEVICE;
+                acc[k] += (double)ms / (2.0 * nsteps);
+            }
+        if (!rc) {
+            size_t kb = 0;
+            for (size_t k = 1; k < top; ++k)
+                if (acc[k] < acc[kb]) kb = k;
+            btpb = timed[kb].tpb;
+            bxcd = timed[kb].xcd;
+            bline = timed[kb].line;
+            bg3 = timed[kb].g3;
+            best = acc[kb] / 2.0;
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     // restore the members' spins and step counters
