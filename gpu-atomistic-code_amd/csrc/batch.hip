@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <string>
 #include <memory>
 #include <thread>
 #include <vector>
@@ -25,6 +26,8 @@ struct Member {
     uint64_t t = 0;
     uint64_t bytes = 0;
     bool live = false;
+    int status = 0;  // LLG_E_NONFINITE once a finite check failed: kept, no longer stepped
+    bool active() const { return live && status == 0; }
 };
 
 struct Group {
@@ -47,6 +50,9 @@ struct llg_batch {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     uint32_t gsteps = 0;
+    // dipolar refreshes captured in the graph: per live dipolar member (slot, t mod refresh,
+    // every-stage); a chunk whose refreshes fall elsewhere is captured anew
+    std::vector<uint64_t> gsig;
     uint32_t graphs_built = 0;
     uint32_t tpb = 256;  // threads per tile (one block per tile)
     int xcd = 1;         // XCD-aware block order
@@ -54,17 +60,22 @@ struct llg_batch {
     int g3 = 0;          // gather from AoS3 copies of S and Y (packed and line readers)
     DevBuf red;
     std::vector<std::shared_ptr<ExchStore>> pinned;  // coupling stores kept for future members
+    std::map<const ExchStore*, uint64_t> store_est;  // admission estimate of each coupling store
 };
 
 namespace {
-// Device bytes in use: member state plus every distinct coupling store, counted once.
+// Device bytes in use: member state plus every distinct coupling store, counted once, at
+// the larger of its admission estimate and its size (a store that encodes smaller than
+// estimated is still charged what admitting it was checked against, so a budget of one
+// request's estimate holds exactly one such request at a time).
 uint64_t used_bytes(const llg_batch* b) {
     uint64_t u = b->used;
     std::vector<const ExchStore*> seen;
     auto add = [&](const ExchStore* s) {
         if (s && std::find(seen.begin(), seen.end(), s) == seen.end()) {
             seen.push_back(s);
-            u += s->bytes();
+            const auto it = b->store_est.find(s);
+            u += std::max<uint64_t>(s->bytes(), it == b->store_est.end() ? 0 : it->second);
         }
     };
     for (auto& mm : b->m)
@@ -82,6 +93,21 @@ void drop_graph(llg_batch* b) {
     b->gexec = nullptr;
     b->graph = nullptr;
     b->gsteps = 0;
+    b->gsig.clear();
+}
+
+// Where the dipolar refreshes of the next chunk fall: they depend only on each dipolar
+// member's step counter modulo its refresh interval (and on every-stage refreshes).
+std::vector<uint64_t> refresh_signature(const llg_batch* b) {
+    std::vector<uint64_t> sig;
+    for (size_t k = 0; k < b->m.size(); ++k) {
+        const Member& mm = b->m[k];
+        if (!mm.active() || !mm.c->dip) continue;
+        sig.push_back(k);
+        sig.push_back(mm.t % mm.c->dip_refresh);
+        sig.push_back(mm.c->dip_every_stage ? 1u : 0u);
+    }
+    return sig;
 }
 
 int regroup(llg_batch* b) {
@@ -89,7 +115,7 @@ int regroup(llg_batch* b) {
     b->groups.clear();
     std::map<std::pair<int, int>, Group*> by;
     for (size_t k = 0; k < b->m.size(); ++k) {
-        if (!b->m[k].live) continue;
+        if (!b->m[k].active()) continue;
         const llg_ctx* c = b->m[k].c;
         const int f = (is_line_fmt(c->ex32.fmt) && !b->line) ? (int)FMT_SELL_PACK : c->ex32.fmt;
         const auto key = std::make_pair(f, c->ex32.ncomp);
@@ -128,13 +154,13 @@ int launch_chunk(llg_batch* b, uint32_t nsteps, bool dipolar) {
     for (uint32_t k = 0; k < nsteps; ++k) {
         if (dipolar)
             for (auto& mm : b->m)
-                if (mm.live && mm.c->dip && (mm.t + k) % mm.c->dip_refresh == 0)
+                if (mm.active() && mm.c->dip && (mm.t + k) % mm.c->dip_refresh == 0)
                     SLLG_TRY(dipolar_refresh(mm.c, mm.c->S.p, mm.c->DEM.p));
         for (auto& g : b->groups)
             SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 0, b->tpb, b->xcd, b->g3 != 0));
         if (dipolar)
             for (auto& mm : b->m)
-                if (mm.live && mm.c->dip && mm.c->dip_every_stage)
+                if (mm.active() && mm.c->dip && mm.c->dip_every_stage)
                     SLLG_TRY(dipolar_refresh(mm.c, mm.c->Y.p, mm.c->DEM.p));
         for (auto& g : b->groups)
             SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 1, b->tpb, b->xcd, b->g3 != 0));
@@ -268,6 +294,7 @@ static int batch_add_impl(llg_batch* b, const llg_desc* d, const std::shared_ptr
     mm.t = t0;
     mm.bytes = std::max<uint64_t>(est_state, info.device_bytes - c->xs->bytes());  // state only
     mm.live = true;
+    if (!store) b->store_est[c->xs.get()] = est - est_state;
     int slot = -1;
     for (size_t k = 0; k < b->m.size(); ++k)
         if (!b->m[k].live && !b->m[k].c) { slot = (int)k; break; }
@@ -322,14 +349,20 @@ int llg_batch_step(llg_batch* b, uint32_t nsteps) {
     if (nsteps == 0) return 0;
     SLLG_HIP(hipSetDevice(b->device));
     if (b->dirty) SLLG_TRY(regroup(b));
-    bool dip = false;
-    for (auto& mm : b->m) dip = dip || (mm.live && mm.c->dip);
+    bool dip = false, dip_graph = true;
+    for (auto& mm : b->m)
+        if (mm.active() && mm.c->dip) {
+            dip = true;
+            // refreshes made of this library's own kernels only (no hipFFT plan) are captured
+            dip_graph = dip_graph && dipolar_capturable(mm.c);
+        }
     SLLG_TRY(upload_tables(b));
-    if ((b->flags & LLG_BATCH_GRAPH) && !dip) {
-        if (!b->gexec || b->gsteps != nsteps) {
+    if ((b->flags & LLG_BATCH_GRAPH) && dip_graph) {
+        std::vector<uint64_t> sig = refresh_signature(b);
+        if (!b->gexec || b->gsteps != nsteps || sig != b->gsig) {
             drop_graph(b);
             SLLG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            const int rc = launch_chunk(b, nsteps, false);
+            const int rc = launch_chunk(b, nsteps, dip);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) {
@@ -340,6 +373,7 @@ int llg_batch_step(llg_batch* b, uint32_t nsteps) {
             b->graph = g;
             SLLG_HIP(hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0));
             b->gsteps = nsteps;
+            b->gsig.swap(sig);
             b->graphs_built++;
         }
         SLLG_HIP(hipGraphLaunch(b->gexec, b->stream));
@@ -347,7 +381,29 @@ int llg_batch_step(llg_batch* b, uint32_t nsteps) {
         SLLG_TRY(launch_chunk(b, nsteps, dip));
     }
     for (auto& mm : b->m)
-        if (mm.live) mm.t += nsteps;
+        if (mm.active()) mm.t += nsteps;
+    // members that asked for finite checks (LLG_FLAG_CHECK_FINITE): one with a NaN/Inf spin is
+    // taken out of the launches, keeps its state and step counter, and reports the failure
+    std::string bad;
+    for (size_t k = 0; k < b->m.size(); ++k) {
+        Member& mm = b->m[k];
+        if (!mm.active() || !(mm.c->flags & LLG_FLAG_CHECK_FINITE)) continue;
+        int nf = 0;
+        SLLG_TRY(check_finite(mm.c, &nf));
+        if (nf) {
+            mm.status = LLG_E_NONFINITE;
+            b->dirty = true;
+            bad += (bad.empty() ? "" : ", ") + std::to_string(k);
+        }
+    }
+    if (!bad.empty()) SLLG_FAIL(LLG_E_NONFINITE, "non-finite spins in batch member(s) %s", bad.c_str());
+    return 0;
+}
+
+int llg_batch_member_status(llg_batch* b, int member, int* status) {
+    SLLG_TRY(check_member(b, member));
+    if (!status) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    *status = b->m[member].status;
     return 0;
 }
 
@@ -369,19 +425,29 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
     std::memset(out, 0, sizeof(*out));
     SLLG_HIP(hipSetDevice(b->device));
     // save every live member's spins and step counter; the timed steps are undone afterwards
-    std::vector<DevBuf> save(b->m.size());
+    // (and the dipolar field cached between refreshes); every candidate starts from the saved
+    // step counters, so its timed window holds the same dipolar refreshes as every other's
+    std::vector<DevBuf> save(b->m.size()), save_dem(b->m.size());
     std::vector<uint64_t> t_save(b->m.size());
     for (size_t k = 0; k < b->m.size(); ++k) {
-        if (!b->m[k].live) continue;
+        if (!b->m[k].active()) continue;
         llg_ctx* c = b->m[k].c;
         SLLG_TRY(save[k].alloc(c->S.bytes));
         SLLG_HIP(hipMemcpyAsync(save[k].p, c->S.p, c->S.bytes, hipMemcpyDeviceToDevice, b->stream));
+        if (c->dip) {
+            SLLG_TRY(save_dem[k].alloc(c->DEM.bytes));
+            SLLG_HIP(hipMemcpyAsync(save_dem[k].p, c->DEM.p, c->DEM.bytes, hipMemcpyDeviceToDevice, b->stream));
+        }
         t_save[k] = b->m[k].t;
     }
+    auto rewind = [&]() {
+        for (size_t k = 0; k < b->m.size(); ++k)
+            if (b->m[k].active()) b->m[k].t = t_save[k];
+    };
     const uint32_t tpb0 = b->tpb;
     const int xcd0 = b->xcd, line0 = b->line, g30 = b->g3;
     bool any_line = false;
-    for (auto& mm : b->m) any_line = any_line || (mm.live && is_line_fmt(mm.c->ex32.fmt));
+    for (auto& mm : b->m) any_line = any_line || (mm.active() && is_line_fmt(mm.c->ex32.fmt));
     const int nline = any_line ? 2 : 1;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     SLLG_HIP(hipEventCreate(&e0));
@@ -402,6 +468,7 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
         for (int xcd = 1; xcd >= 0 && !rc; --xcd) {
             const int line = nline == 2 ? 1 - li : line0;
             rc = llg_batch_set_launch(b, tpbs[ti], xcd, line, gi);
+            rewind();
             if (!rc) rc = llg_batch_step(b, 2);  // warm-up (and graph capture when enabled)
             if (!rc) rc = llg_batch_step(b, nsteps);  // a graph of this length is captured here
             if (rc) break;
@@ -434,6 +501,7 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
             for (size_t k = 0; k < top && !rc; ++k) {
                 const Cand& t = timed[k];
                 rc = llg_batch_set_launch(b, t.tpb, t.xcd, t.line, t.g3);
+                rewind();
                 if (!rc) rc = llg_batch_step(b, 2);
                 if (!rc) rc = llg_batch_step(b, 2 * nsteps);  // captures a graph of this length
                 if (!rc && hipEventRecord(e0, b->stream) != hipSuccess) rc = LLG_E_DEVICE;
@@ -459,10 +527,14 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
     (void)hipEventDestroy(e1);
     // restore the members' spins and step counters
     for (size_t k = 0; k < b->m.size(); ++k) {
-        if (!b->m[k].live) continue;
+        if (!b->m[k].active()) continue;
         llg_ctx* c = b->m[k].c;
-        const hipError_t e = hipMemcpyAsync(c->S.p, save[k].p, c->S.bytes, hipMemcpyDeviceToDevice, b->stream);
+        hipError_t e = hipMemcpyAsync(c->S.p, save[k].p, c->S.bytes, hipMemcpyDeviceToDevice, b->stream);
         if (e != hipSuccess && !rc) rc = LLG_E_DEVICE;
+        if (c->dip) {
+            e = hipMemcpyAsync(c->DEM.p, save_dem[k].p, c->DEM.bytes, hipMemcpyDeviceToDevice, b->stream);
+            if (e != hipSuccess && !rc) rc = LLG_E_DEVICE;
+        }
         c->g3_valid = false;  // S changed under the AoS3 copy: rebuilt before the next gather-copy launch
         b->m[k].t = t_save[k];
     }

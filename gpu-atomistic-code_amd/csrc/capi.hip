@@ -514,33 +514,36 @@ int llg_set_staggered(llg_ctx* c, const double* hx, const double* hy, const doub
     if (!c) SLLG_FAIL(LLG_E_INVALID, "NULL handle");
     SLLG_HIP(hipSetDevice(c->device));
     const uint32_t n = c->n;
-    std::vector<double>& s0 = c->stag0_host;
     if (!hx && !hy && !hz) {
-        s0.clear();
         c->stag_set = false;
+        c->STG0.release();
         SLLG_HIP(hipMemsetAsync(c->STG.p, 0, c->STG.bytes, c->stream));
         SLLG_HIP(hipStreamSynchronize(c->stream));
         return 0;
     }
     if (!hx || !hy || !hz) SLLG_FAIL(LLG_E_INVALID, "staggered field: give all three components or none");
-    s0.assign(3 * (size_t)n, 0.0);
-    std::memcpy(s0.data(), hx, n * 8);
-    std::memcpy(s0.data() + n, hy, n * 8);
-    std::memcpy(s0.data() + 2 * (size_t)n, hz, n * 8);
+    // H0 stays on the device in FP64 (device order) for llg_scale_staggered; H = 1 * H0
+    const double* src[3] = {hx, hy, hz};
+    std::vector<double> h0(3 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const size_t d = c->perm.to_dev(i);
+        for (int a = 0; a < 3; ++a) h0[(size_t)a * n + d] = src[a][i];
+    }
+    if (c->STG0.bytes != h0.size() * 8) SLLG_TRY(c->STG0.alloc(h0.size() * 8));
+    SLLG_HIP(hipMemcpyAsync(c->STG0.p, h0.data(), h0.size() * 8, hipMemcpyHostToDevice, c->stream));
+    SLLG_TRY(stag_scale(c, 1.0));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
     c->stag_set = true;
-    return upload_state(c, c->STG, hx, hy, hz);
+    return 0;
 }
 
-// cullg::rampStag / cufields::CincStagFields (src/cuda.cu:38-41, src/cufields.cu:46-56): H = r H0
+// cullg::rampStag / cufields::CincStagFields (src/cuda.cu:38-41, src/cufields.cu:46-56):
+// H = r H0, one kernel on the handle's stream (asynchronous, like the reference's launch).
 int llg_scale_staggered(llg_ctx* c, double r) {
     if (!c || !std::isfinite(r)) SLLG_FAIL(LLG_E_INVALID, "bad argument");
-    std::vector<double>& s0 = c->stag0_host;
-    if (s0.empty()) SLLG_FAIL(LLG_E_STATE, "no staggered field set");
-    const uint32_t n = c->n;
-    std::vector<double> v(3 * (size_t)n);
-    for (size_t k = 0; k < v.size(); ++k) v[k] = r * s0[k];
+    if (!c->stag_set || !c->STG0.p) SLLG_FAIL(LLG_E_STATE, "no staggered field set");
     SLLG_HIP(hipSetDevice(c->device));
-    return upload_state(c, c->STG, v.data(), v.data() + n, v.data() + 2 * (size_t)n);
+    return stag_scale(c, r);
 }
 
 int llg_push_noise(llg_ctx* c, uint32_t nsteps, const double* xi) {
@@ -822,8 +825,14 @@ std::string tune_key(const llg_ctx* c) {
     hipDeviceProp_t p{};
     (void)hipGetDeviceProperties(&p, c->device);
     char buf[256];
-    std::snprintf(buf, sizeof buf, "%s|%d|%d|%u|%llu|%u", p.gcnArchName, c->ex32.fmt, c->ex32.ncomp, c->n,
-                  (unsigned long long)c->ex32.nent, c->perm.na);
+    // what changes the kernels that run: the thermal field (drawn only at T > 0 with counter
+    // noise), the dipolar field operand, the four-spin term, a per-spin species index, and a
+    // reader pinned by the caller
+    const int thermal = c->T > 0.0 && c->noise_mode == LLG_NOISE_COUNTER;
+    std::snprintf(buf, sizeof buf, "%s|%d|%d|%u|%llu|%u|T%d|D%d|F%d|M%d|R%d", p.gcnArchName, c->ex32.fmt,
+                  c->ex32.ncomp, c->n, (unsigned long long)c->ex32.nent, c->perm.na, thermal, c->dip ? 1 : 0,
+                  c->ex32.fs_xadj ? 1 : 0, c->multi_species ? 1 : 0,
+                  (c->flags & LLG_FLAG_TUNE_FIXED_READER) ? 1 + c->tune_line : 0);
     return buf;
 }
 
@@ -893,13 +902,19 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
             return 0;
         }
     }
-    // keep the state: time on a copy of the spins, restore afterwards
-    DevBuf save;
+    // keep the state: time on a copy of the spins (and of the dipolar field cached between
+    // refreshes, which the timed steps overwrite), restore afterwards
+    DevBuf save, save_dem;
     SLLG_TRY(save.alloc(c->S.bytes));
     SLLG_HIP(hipMemcpyAsync(save.p, c->S.p, c->S.bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (c->dip) {
+        SLLG_TRY(save_dem.alloc(c->DEM.bytes));
+        SLLG_HIP(hipMemcpyAsync(save_dem.p, c->DEM.p, c->DEM.bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    const TuneChoice caller{c->tune_tpb, c->tune_xcd, c->tune_unr, c->tune_thc, c->tune_g3, c->tune_line};
     const int tpbs[3] = {256, 512, 1024};
-    // line encodings: line reader or the packed entries kept beside it
-    const int nline = is_line_fmt(c->ex32.fmt) ? 2 : 1;
+    // line encodings: line reader or the packed entries kept beside it (unless pinned)
+    const int nline = (is_line_fmt(c->ex32.fmt) && !(c->flags & LLG_FLAG_TUNE_FIXED_READER)) ? 2 : 1;
     const int unrs[2] = {2, 4};
     // handing the thermal field over only matters at T > 0 with counter noise
     const int nthc = (c->T > 0.0 && c->noise_mode == LLG_NOISE_COUNTER) ? 2 : 1;
@@ -985,10 +1000,19 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     SLLG_HIP(hipMemcpyAsync(c->S.p, save.p, c->S.bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (c->dip) SLLG_HIP(hipMemcpyAsync(c->DEM.p, save_dem.p, c->DEM.bytes, hipMemcpyDeviceToDevice, c->stream));
     SLLG_HIP(hipStreamSynchronize(c->stream));
     c->det_valid = false;
     c->g3_valid = false;
-    if (rc) return rc;
+    if (rc) {  // the caller's launch shape stays in force
+        c->tune_tpb = caller.tpb;
+        c->tune_xcd = caller.xcd;
+        c->tune_unr = caller.unr;
+        c->tune_thc = caller.thc;
+        c->tune_g3 = caller.g3;
+        c->tune_line = caller.line;
+        return rc;
+    }
     c->tune_tpb = out->tpb = b.tpb;
     c->tune_xcd = out->xcd = b.xcd;
     c->tune_unr = out->unroll = b.unr;

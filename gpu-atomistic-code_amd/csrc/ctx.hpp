@@ -205,7 +205,8 @@ struct llg_ctx {
     bool y_soa_valid = true;  // Y (SoA) holds the last predictor output (else only Y3 does)
 
     // device state
-    sllg::DevBuf S, Y, A, H, FN, DET, XI, STG, STG0, DEM;  // SoA 3n (type per path)
+    sllg::DevBuf S, Y, A, H, FN, DET, XI, STG, DEM;  // SoA 3n (type per path)
+    sllg::DevBuf STG0;  // staggered field as last given, FP64 SoA 3n in device order (ramped by llg_scale_staggered)
     sllg::DevBuf HT;  // FP32 Heun: thermal field of the current step, predictor -> corrector
     sllg::DevBuf S3, Y3;  // FP32 Heun: AoS3 gather copies [3 i + c] of S and Y
     sllg::DevBuf K[6];                                     // RK stage increments
@@ -231,8 +232,6 @@ struct llg_ctx {
     sllg::DipolarPlan* dplan = nullptr;
     // injected noise queue (host)
     std::deque<std::vector<double>> noise_q;
-    // staggered field as last given (host, SoA 3n), scaled by llg_scale_staggered
-    std::vector<double> stag0_host;
     // timing
     sllg::KernelTimer* timer = nullptr;
 };

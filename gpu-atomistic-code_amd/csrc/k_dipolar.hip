@@ -1042,6 +1042,10 @@ int dipolar_set_stream(llg_ctx* c) {
 
 bool dipolar_is_exchange(const llg_ctx* c) { return c->dplan && c->dplan->general; }
 
+// The FP32 half-grid plane path runs only kernels of this file (no hipFFT plans, no memsets),
+// so a stream capture of a refresh replays exactly.
+bool dipolar_capturable(const llg_ctx* c) { return c->dplan && !c->dplan->f64 && c->dplan->yz; }
+
 void dipolar_destroy(llg_ctx* c) {
     delete c->dplan;
     c->dplan = nullptr;

@@ -337,4 +337,26 @@ int plain_detfield(llg_ctx* c, const double* Sp, double* out) {
     return 0;
 }
 
+// Staggered-field ramp on the device, cufields::CincStagFields (src/cufields.cu:46-56):
+// H_stag = r * H0 with H0 the field last given, kept on the device in FP64 (device order,
+// SoA).  The product is taken in FP64 and rounded once to the path's precision.
+template <class P>
+__global__ __launch_bounds__(kTPB) void k_stag_scale(size_t n3, double r, const double* __restrict__ h0,
+                                                     P* __restrict__ h) {
+    const size_t k = (size_t)blockIdx.x * kTPB + threadIdx.x;
+    if (k < n3) h[k] = (P)(r * h0[k]);
+}
+
+int stag_scale(llg_ctx* c, double r) {
+    const size_t n3 = 3 * (size_t)c->n;
+    if (c->prec == LLG_FP64_PLAIN)
+        k_stag_scale<double><<<blocks_for(n3, kTPB), kTPB, 0, c->stream>>>(n3, r, c->STG0.as<double>(),
+                                                                           c->STG.as<double>());
+    else
+        k_stag_scale<float><<<blocks_for(n3, kTPB), kTPB, 0, c->stream>>>(n3, r, c->STG0.as<double>(),
+                                                                          c->STG.as<float>());
+    SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
 }  // namespace sllg

@@ -13,6 +13,7 @@ int plain_rk_init_tables();
 int plain_rk_stage(llg_ctx* c, int m);
 int plain_rk_comb(llg_ctx* c);
 int plain_detfield(llg_ctx* c, const double* Sp, double* out);
+int stag_scale(llg_ctx* c, double r);  // STG = r * STG0 (FP64 H0 kept on the device)
 
 // FP32 fused path (k_fused.hip)
 struct StepSched {
@@ -56,6 +57,7 @@ int dipolar_create(llg_ctx* c, const llg_dipolar& d, const uint32_t* spin_specie
 void dipolar_destroy(llg_ctx* c);
 uint64_t dipolar_bytes(const llg_ctx* c);  // plan buffers and FFT work areas (a shared kernel counted in full)
 bool dipolar_is_exchange(const llg_ctx* c);  // the plan is an FFT-exchange plan
+bool dipolar_capturable(const llg_ctx* c);   // a refresh is this library's kernels only (no hipFFT plan)
 int dipolar_refresh(llg_ctx* c, const void* S, void* out);  // spins of the path's precision -> out (3n)
 
 // Encodings (encode.cpp)

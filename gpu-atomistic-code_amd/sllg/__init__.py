@@ -32,6 +32,7 @@ FLAG_NO_SUBLATTICE_ORDER = 0x200
 FLAG_NO_SLICE_SHARING = 0x400
 FLAG_NO_LINE_MODE = 0x800
 FLAG_DIPOLAR_FULL_FFT = 0x1000
+FLAG_TUNE_FIXED_READER = 0x2000
 
 STATUS = {0: "OK", -1: "INVALID", -2: "NOMEM", -3: "DEVICE", -4: "UNSUPPORTED", -5: "NODEVICE",
           -6: "STATE", -7: "NONFINITE"}

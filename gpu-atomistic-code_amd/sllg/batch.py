@@ -67,6 +67,7 @@ def _bind():
     L.llg_batch_set_temperature.argtypes = [P, C.c_int, C.c_double]
     L.llg_batch_set_applied.argtypes = [P, C.c_int, dp]
     L.llg_batch_step.argtypes = [P, C.c_uint32]
+    L.llg_batch_member_status.argtypes = [P, C.c_int, C.POINTER(C.c_int)]
     L.llg_batch_reduce.argtypes = [P, C.POINTER(C.c_int), C.c_uint32, C.c_uint32, dp]
     L.llg_batch_get_spins.argtypes = [P, C.c_int, dp, dp, dp]
     L.llg_batch_get_step.argtypes = [P, C.c_int, C.POINTER(C.c_uint64)]
@@ -127,6 +128,12 @@ class Batch:
 
     def step(self, nsteps):
         _check(self.L.llg_batch_step(self.h, int(nsteps)), self.L)
+
+    def member_status(self, member):
+        """0, or the status (-7, NONFINITE) of a member taken out by a failed finite check."""
+        st = C.c_int(0)
+        _check(self.L.llg_batch_member_status(self.h, member, C.byref(st)), self.L)
+        return st.value
 
     def set_launch(self, tpb, xcd=1, line_mode=1, gather_copy=0):
         _check(self.L.llg_batch_set_launch(self.h, int(tpb), int(xcd), int(line_mode), int(gather_copy)), self.L)

@@ -54,8 +54,15 @@ int llg_batch_remove(llg_batch* b, int member);
 int llg_batch_set_temperature(llg_batch* b, int member, double T);
 int llg_batch_set_applied(llg_batch* b, int member, const double B[3]);
 
-/* Advance every member nsteps steps (each from its own step counter). */
+/* Advance every member nsteps steps (each from its own step counter).  Members created with
+ * LLG_FLAG_CHECK_FINITE are checked for NaN/Inf spins after the steps; a member that fails is
+ * taken out of later launches (its spins and step counter stay as they were found, its status
+ * becomes LLG_E_NONFINITE) and the call returns LLG_E_NONFINITE naming it.  The other members
+ * have advanced normally and go on unaffected: no member reads another's data.  This replaces
+ * the reference's error exit (src/error.cpp:271-281) with a per-member status. */
 int llg_batch_step(llg_batch* b, uint32_t nsteps);
+/* LLG_OK, or LLG_E_NONFINITE for a member taken out by a failed finite check. */
+int llg_batch_member_status(llg_batch* b, int member, int* status);
 
 /* Launch shape of the batched kernels: tpb threads per tile (one block per tile: 128, 256
  * or 512), the XCD-aware block order (1) or plain order (0), for members with a line

@@ -92,6 +92,10 @@ typedef enum llg_noise {
  * transforms of its x < Gx/2 half plus the fused x-transform / k-space product kernel
  * (same result to FP32 rounding). */
 #define LLG_FLAG_DIPOLAR_FULL_FFT 0x1000u
+/* llg_autotune keeps the reader set by llg_set_launch (line reader or packed entries) instead
+ * of timing both: the two sum a row's couplings in different orders, so with this flag the
+ * FP32 trajectory does not depend on which reader a timing run happened to favour. */
+#define LLG_FLAG_TUNE_FIXED_READER 0x2000u
 
 /* Per-species (unit-cell atom) constants; reference geom::ucm (inc/unitcell.h) and
  * the per-spin arrays filled in src/geom.cpp:239-247 and src/llg.cpp:591-592,626-631. */
