@@ -6,22 +6,37 @@ FP32 fused field assembly + integrator, T = 300 K counter-based thermal noise, s
 random from seed 42.  A "step" is one full Heun step (predictor + corrector) of all
 spins; value = spin-updates/s = n * K / (timed seconds) with the state resident in HBM.
 
-Also measured in the same run: the FP64 plain path (reference kernel structure, CSR with
-3 FP64 values per nnz as the reference stores diagonal exchange) on the same lattice,
-the per-kernel mean launch time from HIP events on the handle's stream (roofline), the
-FP64 oracle on one host core over a bounded sample (cpu_baseline), and the serving half of
-the metric: 96 M(T) / field-ramp / spin-wave requests (L 10-40, 1000-5000 steps, Poisson
-arrivals at 200/s) through the batched FP32 request loop, solves/s and p50/p99 latency,
-against the sequential FP64 plain solves of the same requests (`serving`).
+Also measured in the same run and carried in the same JSON line:
+  roofline       the corrector's algorithmic bytes / its share of the timed step, and its HBM
+                 traffic per launch from two rocprofv3 PMC child runs of this file (FETCH_SIZE,
+                 WRITE_SIZE), each scaled by the same counter on a 1 GiB device copy
+  by_temperature the same system at T = 0, 300 and 600 K
+  beyond_cache   fcc 192 x 192 x 64 x 4 = 9,437,184 spins, whose stepping state (S, Y, A: 340 MB)
+                 exceeds the 256 MB Infinity Cache
+  fp64_plain     the FP64 plain path (reference kernel structure, CSR with 3 FP64 values per
+                 nnz as the reference stores diagonal exchange) on the 1M-spin lattice
+  cpu_baseline   the FP64 oracle on one host core over a bounded sample
+  serving        the serving stream: 4,096 Poisson requests at 200/s of simple-cubic M(T)
+                 and field-ramp solves, L log-uniform in [10, 100], 2,000-20,000 Heun steps,
+                 through the batched FP32 request loop (solves/s, p50/p99 latency), a
+                 sub-capacity stream (256 requests at 10/s), and the sequential FP64 plain
+                 solves of a seeded sample of the stream (every 256th request, run back to back)
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|copy-probe|accuracy|rk4|batched|serve|fft-exchange]
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+                  [--mode headline|copy-probe|traffic-probe|accuracy|rk4|batched|serve|fft-exchange]
+  python bench.py --summarise trace|pmc <rocprofv3 results.db> [kernel ...]
 """
 import argparse
+import glob
 import json
 import os
 import platform
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
+from collections import defaultdict
 
 import numpy as np
 
@@ -30,7 +45,10 @@ sys.path.insert(0, os.path.join(ROOT, "gpu-atomistic-code_amd"))
 
 PEAK_HBM_GBS = 8000.0
 METRIC = "spin-updates/s @1M spins FP32 vs FP64 plain; batched solves/s + p99 latency"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_heun_fp32_fcc1M.json")
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def cpu_model():
@@ -42,6 +60,43 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor()
+
+
+# ---------------------------------------------------------------- rocprofv3 results
+
+
+def rp_trace(db):
+    """Kernel trace of a rocprofv3 results database: (name, calls, mean, min, max us)."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, count(*), avg(duration), min(duration), max(duration) from kernels "
+                       "group by name order by sum(duration) desc").fetchall()
+    return [(name, cnt, avg / 1e3, mn / 1e3, mx / 1e3) for name, cnt, avg, mn, mx in rows]
+
+
+def rp_pmc(db, kernels=None):
+    """Mean counter value per dispatch: {(kernel name, counter): (mean, dispatches)}."""
+    con = sqlite3.connect(db)
+    acc = defaultdict(list)
+    for name, counter, value in con.execute("select kernel_name, counter_name, value from counters_collection"):
+        if kernels and not any(k in name for k in kernels):
+            continue
+        acc[(name, counter)].append(value)
+    return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
+
+
+def summarise(argv):
+    mode, db = argv[0], argv[1]
+    if mode == "trace":
+        print(f"{'kernel':60s} {'calls':>6s} {'mean_us':>10s} {'min_us':>10s} {'max_us':>10s}")
+        for name, cnt, avg, mn, mx in rp_trace(db):
+            print(f"{name[:60]:60s} {cnt:6d} {avg:10.3f} {mn:10.3f} {mx:10.3f}")
+    else:
+        # full names: template arguments tell a kernel's instances apart
+        for (name, counter), (mean, cnt) in sorted(rp_pmc(db, argv[2:]).items()):
+            print(f"{name} {counter} dispatches={cnt} mean={mean:.1f}")
+
+
+# ---------------------------------------------------------------- timing helpers
 
 
 def time_steps(ctx, torch, t0, steps):
@@ -82,6 +137,86 @@ def cpu_baseline(L, budget_s=15.0):
                       f"{steps} steps after 1 warm-up step ({dt:.1f} s) on one core of {cpu_model()}"}
 
 
+LAUNCH_KEYS = ("tpb", "xcd", "unroll", "thermal_cache", "gather_copy", "line_mode")
+
+
+def launch_of(tune):
+    return [int(tune[k]) for k in LAUNCH_KEYS]
+
+
+# ---------------------------------------------------------------- HBM traffic (PMC child runs)
+
+
+def traffic_probe(args):
+    """Child of the PMC runs: the headline system at the given launch shape for a few steps,
+    then a 1 GiB device copy (the counter scale)."""
+    import torch
+    import sllg
+    from sllg import workloads as W
+
+    sysd, s0 = W.fcc_3shell(L=args.L, precision="fp32")
+    g = sllg.Context(sysd)
+    g.set_spins(s0)
+    g.set_temperature(args.T)
+    g.set_launch(*args.launch)
+    g.step(0, 3)
+    g.step(3, 10)
+    g.sync()
+    g.close()
+    nbytes = 1 << 30
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    x.fill_(1.0)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print("traffic probe done", flush=True)
+
+
+def measure_traffic(args, tune):
+    """HBM bytes per launch of the Heun kernels from two rocprofv3 PMC runs of traffic_probe
+    (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads
+    for the 1 GiB copy.  Returns (dict, None) or (None, reason)."""
+    out = {}
+    with tempfile.TemporaryDirectory(prefix="sllg_pmc_") as d:
+        env = dict(os.environ, TMPDIR=d)
+        for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+            od = os.path.join(d, ctr)
+            cmd = ["rocprofv3", "--pmc", ctr, "-d", od, "-o", ctr.lower(), "--", sys.executable,
+                   os.path.join(ROOT, "bench.py"), "--mode", "traffic-probe", "--L", str(args.L), "--T", str(args.T),
+                   "--launch", *[str(v) for v in launch_of(tune)]]
+            log(f"bench: PMC run {ctr}")
+            try:
+                r = subprocess.run(cmd, cwd=d, env=env, capture_output=True, text=True, timeout=180)
+            except (OSError, subprocess.TimeoutExpired) as e:
+                return None, f"rocprofv3 {ctr}: {e}"
+            dbs = glob.glob(os.path.join(od, "**", "*.db"), recursive=True)
+            if "traffic probe done" not in r.stdout or not dbs:
+                return None, f"rocprofv3 {ctr}: rc {r.returncode}, no results ({r.stderr[-300:]})"
+            vals = defaultdict(list)
+            for db in dbs:
+                for (name, counter), (mean, cnt) in rp_pmc(db).items():
+                    if counter != ctr:
+                        continue
+                    key = ("copy" if "copyBuffer" in name else "corr" if "k_heun_corr" in name
+                           else "pred" if "k_heun_pred" in name else None)
+                    if key:
+                        vals[key].append(mean)
+            if not all(vals.get(k) for k in ("copy", "corr", "pred")):
+                return None, f"{ctr}: kernels missing from the PMC results ({sorted(vals)})"
+            scale = (1 << 30) / (max(vals["copy"]) * 1024.0)
+            out[ctr] = {"scale": scale, "corr_KiB": float(np.mean(vals["corr"])), "pred_KiB": float(np.mean(vals["pred"]))}
+    f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]
+    res = {"fetch_scale": f["scale"], "write_scale": w["scale"]}
+    for k in ("corr", "pred"):
+        res[k] = {"fetch_bytes": f[k + "_KiB"] * 1024 * f["scale"], "write_bytes": w[k + "_KiB"] * 1024 * w["scale"]}
+        res[k]["hbm_bytes"] = res[k]["fetch_bytes"] + res[k]["write_bytes"]
+    return res, None
+
+
+# ---------------------------------------------------------------- headline
+
+
 def headline(args):
     import torch
     import sllg
@@ -95,20 +230,35 @@ def headline(args):
     g.set_spins(s0)
     g.set_temperature(T)
     tune = None if args.no_tune else g.autotune(20)
+    log(f"bench: launch {tune}")
     g.step(0, args.warmup)
     sec, wall = time_steps(g, torch, args.warmup, args.steps)
     value = sysd.n * args.steps / sec
-    prof = g.profile_kernels(args.warmup + args.steps, 20)
+    ms_step = sec * 1e3 / args.steps
+    # per-kernel launch times from events around each launch (they add a little per launch);
+    # the corrector's duration is its share of the timed step in the ratio of those times
+    prof = g.profile_kernels(args.warmup + args.steps, 50)
     info = g.info()
-    # dominant kernel: the corrector (same algorithmic bytes as the predictor)
-    dom_ms = prof.get("heun_corrector", max(prof.values()))
+    pk, ck = prof.get("heun_predictor"), prof.get("heun_corrector")
+    dom_ms = ms_step * ck / (pk + ck) if (pk and ck) else max(prof.values())
     bytes_launch = info["dominant_kernel_bytes_per_spin"] * sysd.n
     achieved = bytes_launch / (dom_ms / 1e3) / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    # the same system at the three temperatures of the case
+    by_T = {}
+    t_next = args.warmup + args.steps + 50
+    for TT in (0.0, 300.0, 600.0):
+        g.set_temperature(TT)
+        g.step(t_next, 5)
+        tsec, _ = time_steps(g, torch, t_next + 5, 100)
+        t_next += 105
+        by_T[f"{TT:g}K"] = {"spin_updates_per_s": sysd.n * 100 / tsec, "ms_per_step": tsec * 10}
     g.close()
+    log(f"bench: headline {value:.3e} su/s, {ms_step * 1e3:.2f} us/step")
+
+    traffic = traffic_err = None
+    if not args.no_pmc and tune is not None:
+        traffic, traffic_err = measure_traffic(args, tune)
+        log(f"bench: traffic {traffic if traffic else traffic_err}")
 
     # Same system with every slice's entries stored (no sharing of repeated slices): the
     # coupling stream then comes from HBM on every launch.
@@ -125,7 +275,8 @@ def headline(args):
         usec, _ = time_steps(u, torch, args.warmup, usteps)
         uprof = u.profile_kernels(args.warmup + usteps, 20)
         uinfo = u.info()
-        u_ms = uprof.get("heun_corrector", max(uprof.values()))
+        upk, uck = uprof.get("heun_predictor"), uprof.get("heun_corrector")
+        u_ms = usec * 1e3 / usteps * uck / (upk + uck)
         u_bytes = uinfo["dominant_kernel_bytes_per_spin"] * sysd.n
         u_ach = u_bytes / (u_ms / 1e3) / 1e9
         unshared = {"value": sysd.n * usteps / usec, "ms_per_step": usec * 1e3 / usteps,
@@ -134,6 +285,31 @@ def headline(args):
                     "achieved_GBs": u_ach, "frac": u_ach / PEAK_HBM_GBS}
         u.close()
         sysd.flags = 0
+
+    # A lattice whose stepping state does not fit the 256 MB Infinity Cache.
+    beyond = None
+    if not args.no_beyond:
+        bd = (192, 192, 64)
+        bsys, bs0 = W.fcc_3shell(dims=bd, precision="fp32")
+        bg = sllg.Context(bsys)
+        bg.set_spins(bs0)
+        bg.set_temperature(T)
+        btune = bg.autotune(10) if tune is not None else None
+        bg.step(0, 5)
+        bsteps = max(20, args.steps // 4)
+        bsec, _ = time_steps(bg, torch, 5, bsteps)
+        bprof = bg.profile_kernels(5 + bsteps, 20)
+        binfo = bg.info()
+        bpk, bck = bprof.get("heun_predictor"), bprof.get("heun_corrector")
+        b_ms = bsec * 1e3 / bsteps * bck / (bpk + bck)
+        b_ach = binfo["dominant_kernel_bytes_per_spin"] * bsys.n / (b_ms / 1e3) / 1e9
+        beyond = {"workload": f"fcc {bd[0]}x{bd[1]}x{bd[2]} conventional x4 = {bsys.n} spins, same couplings",
+                  "stepping_state_MB": 36 * bsys.n / 1e6, "infinity_cache_MB": 256,
+                  "value": bsys.n * bsteps / bsec, "ms_per_step": bsec * 1e3 / bsteps, "kernel_ms": b_ms,
+                  "achieved_GBs": b_ach, "frac": b_ach / PEAK_HBM_GBS, "exch_encoding": binfo["exch_encoding"],
+                  "launch_tune": btune}
+        bg.close()
+        log(f"bench: beyond cache {beyond['value']:.3e} su/s")
 
     # FP64 plain path (reference structure) on the same lattice: CSR xx, yy, zz per nnz.
     sysp, sp0 = W.fcc_3shell(L=L, precision="fp64", ncomp=3)
@@ -148,18 +324,7 @@ def headline(args):
     p.close()
 
     cpu = None if args.no_cpu else cpu_baseline(L, args.cpu_budget)
-    serving = None
-    if not args.no_serve:
-        # the metric's serving half: 96 requests (L 10-40, 1000-5000 steps, 200/s), batched
-        # FP32 loop vs the sequential FP64 plain solves of the same requests
-        reqs = serve_requests(args.seed, 96, 200.0, 10, 40, 1000, 5000)
-        srv = serve_run(reqs, args.budget_gb, True)
-        b, q = srv["batched_fp32"], srv["sequential_fp64_plain"]
-        serving = {"requests": 96, "rate_per_s": 200.0, "L": [10, 40], "steps": [1000, 5000],
-                   "types": "M(T), field ramp, spin wave", "batched_solves_per_s": b["solves_per_s"],
-                   "p50_latency_s": b["p50_latency_s"], "p99_latency_s": b["p99_latency_s"],
-                   "sequential_fp64_plain_solves_per_s": q["solves_per_s"],
-                   "sequential_p99_latency_s": q["p99_latency_s"], "solves_per_s_ratio": srv["solves_per_s_ratio"]}
+    serving = None if args.no_serve else serving_block(args)
     line = {
         "metric": METRIC,
         "value": value,
@@ -167,7 +332,7 @@ def headline(args):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": sec * 1e3 / args.steps,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -178,19 +343,96 @@ def headline(args):
                                f"T={T} K counter noise, Heun, FP32 field+integrator, FP64 reductions",
                    "n_spins": sysd.n, "exch_encoding": info["exch_encoding"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": traffic["corr"]["hbm_bytes"] if traffic else None,
                      "kernel": "k_heun_corr", "kernel_ms": dom_ms,
-                     "algorithmic_bytes_per_spin": info["dominant_kernel_bytes_per_spin"]},
+                     "algorithmic_bytes_per_spin": info["dominant_kernel_bytes_per_spin"],
+                     "traffic_detail": traffic if traffic else traffic_err},
         "cpu_baseline": cpu,
         "fp64_plain": {"value": plain, "unit": "spin-updates/s", "ms_per_step": psec * 1e3 / psteps,
                        "fp32_over_fp64_plain": value / plain, "kernels_ms": pprof},
-        "kernels_ms": prof,
+        "kernels_ms_events": prof,
+        "by_temperature": by_T,
         "launch_tune": tune,
         "unshared_encoding": unshared,
+        "beyond_cache": beyond,
         "serving": serving,
         "wall_s_timed": wall,
     }
     print(json.dumps(line), flush=True)
+
+
+# ---------------------------------------------------------------- serving
+
+
+def config5_specs(seed, nreq, rate, lmin=10, lmax=100, smin=2000, smax=20000, types=(0, 1)):
+    """Poisson arrivals at `rate`/s; L = round(10^U(log10 lmin, log10 lmax)); T ~ U(0, 1.5 Tc) with
+    Tc = 1.44 J / k_B; steps ~ U{smin..smax}; the request type drawn from `types` (0 M(T) at
+    fixed T, 1 field ramp from -0.5 T to +0.5 T along z, 2 spin wave with S(k) samples); a
+    seed per request; <M> sampled every 100 steps."""
+    rng = np.random.default_rng(seed)
+    tc = 1.44 * 3e-21 / 1.38e-23
+    t = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    specs = []
+    for k in range(nreq):
+        typ = int(types[int(rng.integers(len(types)))])
+        L = int(round(10 ** rng.uniform(np.log10(lmin), np.log10(lmax))))
+        specs.append(dict(type=typ, L=L, steps=int(rng.integers(smin, smax + 1)), update=100,
+                          T=float(rng.uniform(0, 1.5 * tc)), arrival_s=float(t[k]), seed=1000 + k,
+                          B0=(0.0, 0.0, -0.5) if typ == 1 else (0.0, 0.0, 0.0),
+                          B1=(0.0, 0.0, 0.5) if typ == 1 else (0.0, 0.0, 0.0)))
+    return specs
+
+
+def _serve_warmup():
+    from sllg import batch as SB
+
+    SB.serve(SB.make_requests([dict(L=8, steps=200, update=100, T=100.0, arrival_s=0.0, seed=1),
+                               dict(L=12, steps=200, update=100, T=100.0, arrival_s=0.0, seed=2)]), graph=True)
+
+
+def _progress(tag):
+    return lambda done, total, wall: log(f"bench: {tag} {done}/{total} solves done, {wall:.1f} s")
+
+
+def _stats(st, keys=("solves_per_s", "p50_latency_s", "p99_latency_s", "mean_latency_s", "spin_updates_per_s",
+                     "completed", "max_live", "wall_s")):
+    return {k: st[k] for k in keys}
+
+
+def serving_block(args):
+    """The serving half of the metric on the serving stream (batched FP32 request loop), a
+    sub-capacity stream, and the sequential FP64 plain comparator on a seeded sample."""
+    import sllg
+    from sllg import batch as SB
+
+    specs = config5_specs(args.seed, args.requests, args.rate, args.lmin, args.lmax, args.smin, args.smax)
+    _serve_warmup()
+    _, st, _ = SB.serve(SB.make_requests(specs), budget_bytes=int(args.budget_gb * 2**30), graph=True,
+                        progress=_progress("serving stream"), progress_every=256)
+    mean_su = float(np.mean([s["L"] ** 3 * s["steps"] for s in specs]))
+    out = {"workload": f"{args.requests} Poisson requests at {args.rate:g}/s, simple-cubic M(T) and field-ramp "
+                       f"solves, L log-uniform in [{args.lmin}, {args.lmax}], {args.smin}-{args.smax} Heun steps, "
+                       f"T uniform in [0, 1.5 Tc], NN J=3e-21 J, K=1e-23 J, FP32, seed {args.seed}",
+           "batched_fp32": _stats(st), "mean_spin_updates_per_request": mean_su}
+    # sub-capacity: the same kind of stream at an arrival rate the device keeps up with
+    sub = config5_specs(args.seed + 1, args.sub_requests, args.sub_rate, args.lmin, args.lmax, args.smin, args.smax)
+    _, ss, _ = SB.serve(SB.make_requests(sub), budget_bytes=int(args.budget_gb * 2**30), graph=True,
+                        progress=_progress("sub-capacity stream"), progress_every=64)
+    out["sub_capacity"] = {"requests": args.sub_requests, "rate_per_s": args.sub_rate, "batched_fp32": _stats(ss)}
+    # sequential FP64 plain solves of a seeded sample of the stream, back to back
+    if args.seq_every > 0:
+        sample = [dict(specs[k], arrival_s=0.0) for k in range(0, len(specs), args.seq_every)]
+        _, sq, _ = SB.serve(SB.make_requests(sample), sequential=True, precision=sllg.FP64_PLAIN,
+                            progress=_progress("sequential FP64 sample"), progress_every=1)
+        proj = sq["spin_updates_per_s"] / mean_su
+        out["sequential_fp64_plain"] = {
+            "sample": f"{len(sample)} requests (every {args.seq_every}th of the stream), run back to back",
+            "sample_solves_per_s": sq["solves_per_s"], "spin_updates_per_s": sq["spin_updates_per_s"],
+            "projected_stream_solves_per_s": proj,
+            "note": "projected = the sample's spin-updates/s over the stream's mean spin-updates per request"}
+        out["batched_over_sequential_solves_per_s"] = st["solves_per_s"] / proj
+    return out
 
 
 def accuracy(args):
@@ -287,8 +529,8 @@ def rk4_dmi(args):
 
 def batched(args):
     """64 replicas of the 128x128x16 film (NN exchange + FFT dipolar on the 256x256x32 padded
-    grid) at T_r = 900 r/63 K, Heun FP32, one batched launch per half step (hipGraph when
-    no dipolar refresh falls in the chunk), dipolar refreshed every `update` = 100 steps."""
+    grid) at T_r = 900 r/63 K, Heun FP32, one batched launch per half step, chunks of 100 steps
+    replayed from hipGraphs that hold the dipolar refreshes (every `update` = 100 steps)."""
     import sllg
     from sllg import batch as SB
     from sllg import workloads as W
@@ -322,51 +564,25 @@ def batched(args):
                       "m_z_lowest_T": float(m[0][2]), "m_z_highest_T": float(m[-1][2])}), flush=True)
 
 
-def serve_requests(seed, nreq, rate, lmin, lmax, smin, smax):
-    """Poisson stream of simple-cubic M(T), field-ramp and spin-wave requests (one third
-    each; spin-wave requests sample S(k) at (q, 0, 0) every `update` steps), L log-uniform in
-    [lmin, lmax], T uniform in [0, 1.5 Tc], steps uniform in [smin, smax] (whole hundreds)."""
-    import sllg
-    from sllg import batch as SB
-
-    rng = np.random.default_rng(seed)
-    Tc = 1.44 * 3e-21 / sllg.K_B
-    specs = []
-    t = 0.0
-    for k in range(nreq):
-        t += rng.exponential(1.0 / rate)
-        L = int(round(10 ** rng.uniform(np.log10(lmin), np.log10(lmax))))
-        typ = (SB.REQ_MT, SB.REQ_RAMP, SB.REQ_SW)[int(rng.integers(3))]
-        specs.append(dict(type=typ, L=L, steps=int(rng.integers(smin, smax + 1)) // 100 * 100,
-                          update=100, T=float(rng.uniform(0, 1.5 * Tc)), arrival_s=t, seed=1000 + k,
-                          B0=(0, 0, -0.5) if typ == SB.REQ_RAMP else (0, 0, 0), B1=(0, 0, 0.5) if typ == SB.REQ_RAMP else (0, 0, 0)))
-    return SB.make_requests(specs)
-
-
-def serve_run(reqs, budget_gb, sequential):
-    """Batched FP32 request loop (graph-captured chunks) after an untimed warm-up (the first
-    serve call in a process pays module loading and first launches); optionally the
-    sequential FP64 plain solves of the same requests."""
-    import sllg
-    from sllg import batch as SB
-
-    SB.serve(SB.make_requests([dict(L=8, steps=200, update=100, T=100.0, arrival_s=0.0, seed=1),
-                               dict(L=12, steps=200, update=100, T=100.0, arrival_s=0.0, seed=2)]), graph=True)
-    _, st, _ = SB.serve(reqs, budget_bytes=int(budget_gb * 2**30), graph=True)
-    out = {"batched_fp32": st}
-    if sequential:
-        _, sq, _ = SB.serve(reqs, sequential=True, precision=sllg.FP64_PLAIN)
-        out["sequential_fp64_plain"] = sq
-        out["solves_per_s_ratio"] = st["solves_per_s"] / sq["solves_per_s"]
-    return out
-
-
 def serve_mode(args):
-    """Request stream of serve_requests: batched FP32 loop vs sequential FP64 plain solves,
-    p50/p99 latency."""
-    reqs = serve_requests(args.seed, args.requests, args.rate, args.lmin, args.lmax, args.smin, args.smax)
-    out = {"mode": "serve", "requests": args.requests, "rate_per_s": args.rate}
-    out.update(serve_run(reqs, args.budget_gb, args.sequential))
+    """A request stream (config5_specs with the three request types, spin-wave requests
+    sampling S(k)) through the batched FP32 loop; optionally the sequential FP64 plain solves
+    of the same requests in arrival order."""
+    import sllg
+    from sllg import batch as SB
+
+    specs = config5_specs(args.seed, args.requests, args.rate, args.lmin, args.lmax, args.smin, args.smax,
+                          types=(0, 1, 2))
+    reqs = SB.make_requests(specs)
+    _serve_warmup()
+    _, st, _ = SB.serve(reqs, budget_bytes=int(args.budget_gb * 2**30), graph=True,
+                        progress=_progress("stream"), progress_every=64)
+    out = {"mode": "serve", "requests": args.requests, "rate_per_s": args.rate, "batched_fp32": _stats(st)}
+    if args.sequential:
+        _, sq, _ = SB.serve(reqs, sequential=True, precision=sllg.FP64_PLAIN, progress=_progress("sequential"),
+                            progress_every=16)
+        out["sequential_fp64_plain"] = _stats(sq)
+        out["solves_per_s_ratio"] = st["solves_per_s"] / sq["solves_per_s"]
     print(json.dumps(out), flush=True)
 
 
@@ -420,7 +636,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--mode", default="headline",
-                    choices=["headline", "copy-probe", "accuracy", "rk4", "batched", "serve", "fft-exchange"])
+                    choices=["headline", "copy-probe", "traffic-probe", "accuracy", "rk4", "batched", "serve",
+                             "fft-exchange"])
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--T", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -431,7 +648,14 @@ def main():
     ap.add_argument("--err-steps", type=int, default=2000, help="rk4: steps for the FP32-vs-FP64 error")
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--film", type=int, nargs=3, default=[128, 128, 16])
-    ap.add_argument("--requests", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=4096)
+    ap.add_argument("--sub-requests", type=int, default=256, help="headline: requests of the sub-capacity stream")
+    ap.add_argument("--sub-rate", type=float, default=10.0, help="headline: arrival rate of the sub-capacity stream")
+    ap.add_argument("--seq-every", type=int, default=256,
+                    help="headline: the sequential FP64 plain sample takes every N-th request (0: none)")
+    ap.add_argument("--no-pmc", action="store_true", help="headline: skip the rocprofv3 PMC traffic runs")
+    ap.add_argument("--no-beyond", action="store_true", help="headline: skip the lattice beyond the Infinity Cache")
+    ap.add_argument("--launch", type=int, nargs=6, default=None, help="traffic-probe: launch shape")
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--lmin", type=int, default=10)
     ap.add_argument("--lmax", type=int, default=100)
@@ -442,14 +666,17 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="headline: skip the serving measurement")
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--start", default="both", choices=["random", "ordered", "both"], help="accuracy: initial state")
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarise":
+        summarise(sys.argv[2:])
+        return
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(replicas and requests would be placed whole per GPU, one large system split into slabs "
               "with a halo exchange per field evaluation); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    modes = {"copy-probe": copy_probe, "accuracy": accuracy, "rk4": rk4_dmi, "batched": batched,
-             "serve": serve_mode, "headline": headline, "fft-exchange": fft_exchange}
+    modes = {"copy-probe": copy_probe, "traffic-probe": traffic_probe, "accuracy": accuracy, "rk4": rk4_dmi,
+             "batched": batched, "serve": serve_mode, "headline": headline, "fft-exchange": fft_exchange}
     modes[args.mode](args)
 
 

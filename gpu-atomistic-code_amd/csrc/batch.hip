@@ -432,6 +432,7 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
     for (size_t k = 0; k < b->m.size(); ++k) {
         if (!b->m[k].active()) continue;
         llg_ctx* c = b->m[k].c;
+        SLLG_TRY(sync_spins(c));
         SLLG_TRY(save[k].alloc(c->S.bytes));
         SLLG_HIP(hipMemcpyAsync(save[k].p, c->S.p, c->S.bytes, hipMemcpyDeviceToDevice, b->stream));
         if (c->dip) {
@@ -536,6 +537,7 @@ int llg_batch_autotune(llg_batch* b, uint32_t nsteps, llg_batch_tune* out) {
             if (e != hipSuccess && !rc) rc = LLG_E_DEVICE;
         }
         c->g3_valid = false;  // S changed under the AoS3 copy: rebuilt before the next gather-copy launch
+        c->s_soa_valid = true;
         b->m[k].t = t_save[k];
     }
     if (hipStreamSynchronize(b->stream) != hipSuccess && !rc) rc = LLG_E_DEVICE;
@@ -753,7 +755,15 @@ extern "C" int llg_serve_run(const llg_serve_config* cfg, const llg_request* req
     const auto t_start = Clock::now();
     std::vector<double> lat_hist;
     uint64_t su = 0;
-    uint32_t completed = 0, max_live = 0;
+    uint32_t completed = 0, max_live = 0, reported = 0;
+    // the caller's progress hook, after every progress_every-th completion and the last one
+    auto progress = [&]() {
+        if (!cfg->progress || !cfg->progress_every) return;
+        if (completed == nreq || completed / cfg->progress_every > reported / cfg->progress_every) {
+            cfg->progress(completed, nreq, since(t_start), cfg->progress_user);
+            reported = completed;
+        }
+    };
 
     if (cfg->sequential) {
         // one request at a time, in arrival order, alone on the device
@@ -804,6 +814,7 @@ extern "C" int llg_serve_run(const llg_serve_config* cfg, const llg_request* req
             su += (uint64_t)d.n * r.steps;
             completed++;
             max_live = 1;
+            progress();
         }
     } else {
         if (cfg->precision != LLG_FP32) SLLG_FAIL(LLG_E_UNSUPPORTED, "batched serving runs the FP32 path");
@@ -818,8 +829,13 @@ extern "C" int llg_serve_run(const llg_serve_config* cfg, const llg_request* req
         const uint32_t chunk_cap = cfg->chunk ? cfg->chunk : min_update;
         while (completed < nreq) {
             const double now = since(t_start);
-            // admission: arrived requests, FIFO, while slots and memory allow
-            while (next < nreq && reqs[order[next]].arrival_s <= now) {
+            // admission: arrived requests, FIFO, while slots and memory allow, and only while every
+            // live request sits at one of its sampling steps: a request admitted then starts in
+            // phase with the others, so launch chunks keep the full sampling interval instead of
+            // shrinking to the gaps between the requests' sampling steps
+            bool in_phase = true;
+            for (const auto& lv : live) in_phase = in_phase && lv.done % reqs[lv.req].update == 0;
+            while (in_phase && next < nreq && reqs[order[next]].arrival_s <= now) {
                 if (cfg->max_members && live.size() >= cfg->max_members) break;
                 const uint32_t i = order[next];
                 const llg_request& r = reqs[i];
@@ -923,6 +939,7 @@ extern "C" int llg_serve_run(const llg_serve_config* cfg, const llg_request* req
                     }
                 }
                 live.swap(keep);
+                progress();
             }
         }
     }

@@ -80,7 +80,10 @@ size_t prec_bytes(const llg_ctx* c) { return c->prec == LLG_FP64_PLAIN ? sizeof(
 int upload_state(llg_ctx* c, DevBuf& dst, const double* x, const double* y, const double* z) {
     const uint32_t n = c->n;
     const double* src[3] = {x, y, z};
-    if (&dst == &c->S) c->g3_valid = false;  // the AoS3 gather copy no longer matches
+    if (&dst == &c->S) {
+        c->g3_valid = false;  // the AoS3 gather copy no longer matches
+        c->s_soa_valid = true;
+    }
     if (&dst == &c->Y) c->y_soa_valid = true;
     if (c->prec == LLG_FP64_PLAIN) {
         for (int a = 0; a < 3; ++a)
@@ -193,7 +196,10 @@ int step_plain(llg_ctx* c, uint64_t t) {
 }
 
 int step_fused(llg_ctx* c, uint64_t t) {
-    if (c->dip && t % c->dip_refresh == 0) SLLG_TIMED("dipolar_refresh", dipolar_refresh(c, c->S.p, c->DEM.p));
+    if (c->dip && t % c->dip_refresh == 0) {
+        SLLG_TRY(sync_spins(c));
+        SLLG_TIMED("dipolar_refresh", dipolar_refresh(c, c->S.p, c->DEM.p));
+    }
     StepSched s{t, nullptr};
     SLLG_TRY(take_noise(c, &s.xi));
     if (c->scheme == LLG_HEUN) {
@@ -216,6 +222,7 @@ int step_fused(llg_ctx* c, uint64_t t) {
 
 int do_steps(llg_ctx* c, uint64_t t0, uint32_t nsteps) {
     if (c->scheme != LLG_HEUN) {
+        if (c->prec != LLG_FP64_PLAIN) SLLG_TRY(sync_spins(c));  // RK stages read and write S
         c->g3_valid = false;  // RK stages write S without its gather copy
         c->y_soa_valid = true;
     }
@@ -492,6 +499,7 @@ int llg_set_spins(llg_ctx* c, const double* sx, const double* sy, const double* 
 int llg_get_spins(llg_ctx* c, double* sx, double* sy, double* sz) {
     if (!c || !sx || !sy || !sz) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
     SLLG_HIP(hipSetDevice(c->device));
+    if (c->prec != LLG_FP64_PLAIN) SLLG_TRY(sync_spins(c));
     return download_state(c, c->S.p, c->prec == LLG_FP64_PLAIN, sx, sy, sz);
 }
 
@@ -594,12 +602,14 @@ int llg_get_det_field(llg_ctx* c, double* hx, double* hy, double* hz) {
         SLLG_TRY(fused_y_soa(c));
         return field_at(c, c->Y.p, hx, hy, hz);
     }
+    if (c->prec != LLG_FP64_PLAIN) SLLG_TRY(sync_spins(c));
     return field_at(c, c->S.p, hx, hy, hz);
 }
 
 int llg_get_field(llg_ctx* c, double* hx, double* hy, double* hz) {
     if (!c || !hx || !hy || !hz) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
     SLLG_HIP(hipSetDevice(c->device));
+    if (c->prec != LLG_FP64_PLAIN) SLLG_TRY(sync_spins(c));
     return field_at(c, c->S.p, hx, hy, hz);
 }
 
@@ -905,6 +915,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     // keep the state: time on a copy of the spins (and of the dipolar field cached between
     // refreshes, which the timed steps overwrite), restore afterwards
     DevBuf save, save_dem;
+    SLLG_TRY(sync_spins(c));
     SLLG_TRY(save.alloc(c->S.bytes));
     SLLG_HIP(hipMemcpyAsync(save.p, c->S.p, c->S.bytes, hipMemcpyDeviceToDevice, c->stream));
     if (c->dip) {
@@ -1004,6 +1015,7 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     SLLG_HIP(hipStreamSynchronize(c->stream));
     c->det_valid = false;
     c->g3_valid = false;
+    c->s_soa_valid = true;
     if (rc) {  // the caller's launch shape stays in force
         c->tune_tpb = caller.tpb;
         c->tune_xcd = caller.xcd;

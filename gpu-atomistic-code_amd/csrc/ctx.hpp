@@ -26,12 +26,12 @@ constexpr int kSlice = 64;      // sliced-ELL slice height = one wavefront
 constexpr int kMaxSub = 16;     // sublattices reduced on the device
 constexpr int kDictLds = 2048;  // floats of coupling dictionary staged in LDS (8 KB)
 
-// Line-mode group records: line_group(nc) neighbour lines per record, line_words(nc) words
-// (this group's line words, 3 shifts x nc couplings per line, the next group's line words
-// last; padded to whole 16-byte units).
+// Line-mode group records: line_group(nc) neighbour windows per record, line_words(nc) words
+// (the group's shift mask, its window starts, 3 shifts x nc couplings per window, the next
+// group's window starts last; padded to whole 16-byte units).
 constexpr uint32_t kLineFallback = 0xFFFFFFFFu;  // line-mode slice read from the packed entries
 constexpr int line_group(int nc) { return nc == 1 ? 3 : (nc == 3 ? 2 : 1); }
-constexpr int line_words(int nc) { return (2 * line_group(nc) + 3 * line_group(nc) * nc + 3) / 4 * 4; }
+constexpr int line_words(int nc) { return (1 + 2 * line_group(nc) + 3 * line_group(nc) * nc + 3) / 4 * 4; }
 
 enum ExchFmt : int {
     FMT_NONE = 0,
@@ -202,6 +202,7 @@ struct llg_ctx {
     int tune_g3 = 1;     // gather neighbours from the AoS3 copy of the spins (llg_autotune)
     int tune_line = 1;   // use the line reader when the couplings have a line encoding (llg_autotune)
     bool g3_valid = false;   // S3 holds the current S
+    bool s_soa_valid = true;  // S (SoA) holds the current spins (else only S3 does: see sync_spins)
     bool y_soa_valid = true;  // Y (SoA) holds the last predictor output (else only Y3 does)
 
     // device state

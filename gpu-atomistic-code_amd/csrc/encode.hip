@@ -189,16 +189,22 @@ struct TupleHash {
 // zero-filling shift (open) across the wave (len 64) or within each 16-lane row (len 16).  Slices where it does not hold (a window
 // that would wrap a periodic boundary inside the slice) keep the packed entries
 // (kLineFallback).  A slice becomes a list of group records read with scalar loads,
-// G = line_group(ncomp) lines each, line_words(ncomp) words:
-//   [0, G)           line words: (L - slice base) | mask, L a multiple of len, mask bit 0/1/2
-//                    = couplings present for dz = 0 / +1 / -1 (0 for an unused slot)
-//   [G, G + 3 G nc)  couplings [line][dz 0, +1, -1][component], FP32, zero where absent
-//   [W - G, W)       the next group's line words (zero after the last group); W is
-//                    rounded up to whole 16-byte units, the words between are zero
-// so the reader issues the next group's line loads before it uses the current group.
-// Entries of one row that name the same (line, dz) are summed (in FP64, rounded once).
+// G = line_group(ncomp) windows each, line_words(ncomp) words:
+//   [0]                  shift mask of the group: bit 0/1/2 = some window of the group has
+//                        couplings for dz = 0 / +1 / -1
+//   [1, 1 + G)           window starts L - slice base (int32, multiples of len)
+//   [1 + G, 1 + G + 3 G nc)  couplings [window][dz 0, +1, -1][component], FP32, zero where
+//                        the window has none
+//   [W - G, W)           the next group's window starts (zero after the last group); W is
+//                        rounded up to whole 16-byte units, the words between are zero
+// so the reader issues the next group's window loads before it uses the current group.
+// Windows are grouped by their own shift masks (full groups of one mask first, the rest
+// together under the union of their masks), so a group tests each shift once for all its
+// windows; an unused slot repeats the group's first window with zero couplings.  The reader
+// sums each shift's contributions unshifted and rotates the three partial sums once.
+// Entries of one row that name the same (window, dz) are summed (in FP64, rounded once).
 // Identical record lists are stored once; the first record is a zero record for empty
-// slices.  Entries are summed line by line, not in stored CSR order.
+// slices.  Entries are summed window by window, not in stored CSR order.
 bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vector<uint32_t>& idx,
                  const std::vector<TupleKey>& tuples, uint32_t len, bool open, uint32_t& stored,
                  uint32_t& lines_row, uint32_t& nfall) {
@@ -298,21 +304,42 @@ bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vec
         }
         ++lined;
         st.clear();
-        const size_t ng = (lines.size() + (size_t)G - 1) / (size_t)G;
-        auto line_word = [&](size_t l) -> uint32_t {
-            return l < lines.size() ? ((uint32_t)lines[l].lrel | lines[l].mask) : 0u;
+        // groups of G windows: full groups of one shift mask, then the remainders together
+        std::vector<std::vector<int>> grp;  // window indices (-1: unused slot)
+        {
+            std::vector<int> rest;
+            for (uint32_t m = 7; m >= 1; --m) {
+                std::vector<int> cls;
+                for (size_t l = 0; l < lines.size(); ++l)
+                    if (lines[l].mask == m) cls.push_back((int)l);
+                size_t k = 0;
+                for (; k + (size_t)G <= cls.size(); k += (size_t)G)
+                    grp.emplace_back(cls.begin() + (ptrdiff_t)k, cls.begin() + (ptrdiff_t)k + G);
+                rest.insert(rest.end(), cls.begin() + (ptrdiff_t)k, cls.end());
+            }
+            for (size_t k = 0; k < rest.size(); k += (size_t)G) {
+                std::vector<int> g(rest.begin() + (ptrdiff_t)k, rest.begin() + (ptrdiff_t)std::min(k + G, rest.size()));
+                g.resize((size_t)G, -1);
+                grp.push_back(g);
+            }
+        }
+        const size_t ng = grp.size();
+        auto start_of = [&](const std::vector<int>& g, int u) -> uint32_t {
+            return (uint32_t)lines[(size_t)(g[(size_t)u] >= 0 ? g[(size_t)u] : g[0])].lrel;
         };
         for (size_t gi = 0; gi < ng; ++gi) {
+            const std::vector<int>& g = grp[gi];
             std::vector<uint32_t> rec((size_t)W, 0u);
             for (int u = 0; u < G; ++u) {
-                const size_t l = gi * (size_t)G + (size_t)u;
-                rec[(size_t)u] = line_word(l);
-                if (gi + 1 < ng) rec[(size_t)(W - G + u)] = line_word(l + (size_t)G);
-                if (l >= lines.size()) continue;
+                rec[(size_t)(1 + u)] = start_of(g, u);
+                if (gi + 1 < ng) rec[(size_t)(W - G + u)] = start_of(grp[gi + 1], u);
+                if (g[(size_t)u] < 0) continue;
+                const Line& ln = lines[(size_t)g[(size_t)u]];
+                rec[0] |= ln.mask;
                 for (int k = 0; k < 3; ++k)
                     for (int a = 0; a < nc; ++a) {
-                        const float v = (float)lines[l].v[k][a];
-                        std::memcpy(&rec[(size_t)(G + (u * 3 + k) * nc + a)], &v, 4);
+                        const float v = (float)ln.v[k][a];
+                        std::memcpy(&rec[(size_t)(1 + G + (u * 3 + k) * nc + a)], &v, 4);
                     }
             }
             st.insert(st.end(), rec.begin(), rec.end());

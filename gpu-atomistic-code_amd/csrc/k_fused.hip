@@ -163,27 +163,18 @@ __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_
     }
 }
 
-// Wave rotations by one lane (DPP): rot_next gives lane l the value of lane l+1, rot_prev
-// the value of lane l-1 (both wrap around the 64 lanes).
-// Every lane is written (a full-wave rotation with all rows and banks enabled), so no old
-// value is needed.
-__device__ __forceinline__ float rot_next(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float rot_prev(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x13C, 0xF, 0xF, false));
-}
-
 // Line-mode group record, read with scalar loads by the whole wave (constant indices only,
 // so it stays in SGPRs):
-//   w[0, G)            this group's line words: start row of the 64-row window relative to
-//                      the slice's first row (int32, a multiple of the line length, 64 or 16)
-//                      | mask (bit 0: unshifted, 1: next along the line, 2: previous)
-//   w[G, G + 3 G NC)   couplings [line][shift][component], FP32 (then zero padding)
-//   w[W - G, W)        the next group's line words (zero after the last group)
+//   w[0]               shift mask of the group (bit 0: unshifted, 1: next along the line,
+//                      2: previous): the shifts some window of the group has couplings for
+//   w[1, 1 + G)        window starts relative to the slice's first row (int32, multiples of
+//                      the line length, 64 or 16)
+//   w[CO, CO + 3 G NC) couplings [window][shift][component], FP32, zero where a window has
+//                      none (CO = 1 + G)
+//   w[W - G, W)        the next group's window starts (zero after the last group)
 template <int NC>
 struct LineRecW {
-    static constexpr int G = line_group(NC), W = line_words(NC), NX = W - G;
+    static constexpr int G = line_group(NC), W = line_words(NC), NX = W - G, CO = 1 + G;
     uint32_t w[W];
 };
 // Whole 16-byte units, so the record arrives in few wide scalar loads.
@@ -202,16 +193,7 @@ __device__ __forceinline__ LineRecW<NC> load_line_rec(const uint32_t* __restrict
     return o;
 }
 
-// Couplings of line u, shift k of the record against the spin vector s.
-template <int NC>
-__device__ __forceinline__ void line_entry(const LineRecW<NC>& r, int u, int k, const float s[3], float dot[3]) {
-    float v[NC];
-#pragma unroll
-    for (int a = 0; a < NC; ++a) v[a] = __uint_as_float(r.w[LineRecW<NC>::G + (u * 3 + k) * NC + a]);
-    accum<NC, float>(v, s, dot);
-}
-
-// A loaded line stays one 96-bit value across the pipelined loop (an aligned VGPR tuple,
+// A loaded window stays one 96-bit value across the pipelined loop (an aligned VGPR tuple,
 // so the loads write their final registers and need no copies).
 typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
 
@@ -229,13 +211,13 @@ __device__ __forceinline__ u32x3 line_v(const SpinBuf& sb, uint32_t lane, uint32
     }
 }
 
-// The G lines named by line words lw, loaded into x.
+// The G windows whose starts are ws[0, G), loaded into x.
 template <int NC, bool G3>
-__device__ __forceinline__ void line_loads(const SpinBuf& sb, uint32_t lane, int32_t base, const uint32_t* lw,
+__device__ __forceinline__ void line_loads(const SpinBuf& sb, uint32_t lane, int32_t base, const uint32_t* ws,
                                            u32x3* x) {
 #pragma unroll
     for (int u = 0; u < line_group(NC); ++u)
-        x[u] = line_v<G3>(sb, lane, __builtin_amdgcn_readfirstlane((uint32_t)(base + (int32_t)(lw[u] & ~7u))));
+        x[u] = line_v<G3>(sb, lane, __builtin_amdgcn_readfirstlane((uint32_t)(base + (int32_t)ws[u])));
 }
 
 // Neighbours along lines of LEN cells: lane l takes lane l + 1 (next) or l - 1 (prev) of
@@ -254,30 +236,38 @@ __device__ __forceinline__ float line_prev(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xF, 0xF, OPEN));
 }
 
-// One group's arithmetic: per line, the unshifted and the two rotated neighbours.
-template <int NC, int LEN, bool OPEN>
-__device__ __forceinline__ void line_group_sum(const LineRecW<NC>& r, const u32x3* x, float dot[3]) {
+// Shift K of one group: every window's couplings for that shift against its unshifted
+// spins, into the partial sum of shift K.
+template <int NC, int K>
+__device__ __forceinline__ void line_shift_sum(const LineRecW<NC>& r, const u32x3* x, float p[3]) {
+    using RW = LineRecW<NC>;
 #pragma unroll
-    for (int u = 0; u < LineRecW<NC>::G; ++u) {
-        const uint32_t m = r.w[u];
-        const float s0[3] = {__uint_as_float(x[u].x), __uint_as_float(x[u].y), __uint_as_float(x[u].z)};
-        if (m & 1u) line_entry<NC>(r, u, 0, s0, dot);
-        if (m & 2u) {
-            const float xs[3] = {line_next<LEN, OPEN>(s0[0]), line_next<LEN, OPEN>(s0[1]), line_next<LEN, OPEN>(s0[2])};
-            line_entry<NC>(r, u, 1, xs, dot);
-        }
-        if (m & 4u) {
-            const float xs[3] = {line_prev<LEN, OPEN>(s0[0]), line_prev<LEN, OPEN>(s0[1]), line_prev<LEN, OPEN>(s0[2])};
-            line_entry<NC>(r, u, 2, xs, dot);
-        }
+    for (int u = 0; u < RW::G; ++u) {
+        const float s[3] = {__uint_as_float(x[u].x), __uint_as_float(x[u].y), __uint_as_float(x[u].z)};
+        float v[NC];
+#pragma unroll
+        for (int a = 0; a < NC; ++a) v[a] = __uint_as_float(r.w[RW::CO + (u * 3 + K) * NC + a]);
+        accum<NC, float>(v, s, p);
     }
 }
 
-// Line-mode reader (FMT_SELL_LINE / LINE16): per group of G neighbour windows, one
-// 64-row window load each, the +-1 shifted neighbours as rotations along the lines of the
-// loaded window.  Software pipelined: a group's record names the next group's windows,
-// whose loads are issued before the current group's arithmetic.  A slice marked
-// kLineFallback is read from the packed entries.
+// One group's arithmetic: one uniform test per shift.
+template <int NC>
+__device__ __forceinline__ void line_group_sum(const LineRecW<NC>& r, const u32x3* x, float p[3][3]) {
+    const uint32_t m = r.w[0];
+    if (m & 1u) line_shift_sum<NC, 0>(r, x, p[0]);
+    if (m & 2u) line_shift_sum<NC, 1>(r, x, p[1]);
+    if (m & 4u) line_shift_sum<NC, 2>(r, x, p[2]);
+}
+
+// Line-mode reader (FMT_SELL_LINE / LINE16 / open variants): per group of G neighbour
+// windows, one 64-row window load each.  The shifted neighbours' contributions are summed
+// per shift at the unshifted lanes, and the two shifted partial sums are moved to the
+// lanes that own them once per row (a shift along the line is linear), so a row costs
+// six lane moves whatever its number of windows.  Software pipelined: a group's record
+// names the next group's windows, whose loads are issued (with the scalar load of the
+// next record) before the current group's arithmetic.  A slice marked kLineFallback is
+// read from the packed entries.
 // LFB: the encoding has slices marked kLineFallback (compiled in only then: the packed
 // reader's registers would otherwise cost the line kernels occupancy).
 template <int NC, bool G3, int LEN, bool OPEN, bool LFB>
@@ -293,38 +283,38 @@ __device__ __forceinline__ void exch_row_line(const Exch<float>& e, uint32_t i, 
             return;
         }
     }
-    dot[0] = dot[1] = dot[2] = 0.0f;
     const uint32_t* __restrict__ rp = e.lent + sd.x;
     const uint32_t ngrp = sd.y, lane = i & 63u;
     const int32_t base = (int32_t)(sl << 6);
-    if (ngrp == 0) return;
-    const SpinBuf sb = spin_buf(S, e.n);
-    // Two line buffers used in turn (no register copies, so no early waits on the loads in
-    // flight).  Pairs of groups that both have a successor run in the loop; the last one or
-    // two groups follow it, and the last group issues no look-ahead loads.
-    u32x3 xa[G], xb[G];
-    {
-        const RW r0 = load_line_rec<NC>(rp);
-        line_loads<NC, G3>(sb, lane, base, r0.w, xa);
+    float p[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    if (ngrp != 0) {
+        const SpinBuf sb = spin_buf(S, e.n);
+        // two window buffers used in turn (no register copies, so no early waits on the
+        // loads in flight); pairs of groups run in the loop, the last one or two after it
+        u32x3 xa[G], xb[G];
+        RW c0 = load_line_rec<NC>(rp);
+        line_loads<NC, G3>(sb, lane, base, c0.w + 1, xa);
+        uint32_t r = 0;
+        for (; r + 2 < ngrp; r += 2) {
+            const RW c1 = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
+            line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
+            line_group_sum<NC>(c0, xa, p);
+            const RW c2 = load_line_rec<NC>(rp + (size_t)(r + 2) * W);
+            line_loads<NC, G3>(sb, lane, base, c1.w + NX, xa);
+            line_group_sum<NC>(c1, xb, p);
+            c0 = c2;
+        }
+        if (r + 1 < ngrp) {
+            const RW c1 = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
+            line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
+            line_group_sum<NC>(c0, xa, p);
+            line_group_sum<NC>(c1, xb, p);
+        } else {
+            line_group_sum<NC>(c0, xa, p);
+        }
     }
-    uint32_t r = 0;
-    for (; r + 2 < ngrp; r += 2) {
-        const RW c0 = load_line_rec<NC>(rp + (size_t)r * W);
-        line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
-        line_group_sum<NC, LEN, OPEN>(c0, xa, dot);
-        const RW c1 = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
-        line_loads<NC, G3>(sb, lane, base, c1.w + NX, xa);
-        line_group_sum<NC, LEN, OPEN>(c1, xb, dot);
-    }
-    const RW c0 = load_line_rec<NC>(rp + (size_t)r * W);
-    if (r + 1 < ngrp) {
-        line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
-        line_group_sum<NC, LEN, OPEN>(c0, xa, dot);
-        const RW c1 = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
-        line_group_sum<NC, LEN, OPEN>(c1, xb, dot);
-    } else {
-        line_group_sum<NC, LEN, OPEN>(c0, xa, dot);
-    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dot[c] = p[0][c] + line_next<LEN, OPEN>(p[1][c]) + line_prev<LEN, OPEN>(p[2][c]);
 }
 
 template <int FMT, int NC, class R, class V, class SpinT>
@@ -532,7 +522,8 @@ inline size_t line_lds_bytes(const Exch<float>& e) {
 // With G3 the kernels gather neighbours (and read the row's own spin) from the AoS3 copy
 // (S3 / Y3, [3 i + c]) and keep it current: the predictor writes Y3 (and Y only when Y is
 // non-null: the four-spin term or a dipolar refresh at the predicted spins reads it), the
-// corrector S3 beside S.
+// corrector S3 (and S only when S is non-null: the four-spin term reads it; every other
+// reader of the spins first rebuilds S from S3, sync_spins).
 
 template <int FMT, int NC, int TPB, int UNR, bool G3, bool LFB>
 __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __restrict__ S,
@@ -576,8 +567,10 @@ __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __r
 #pragma unroll
     for (int c = 0; c < 3; ++c) p[c] = av[c] + a.half_rdt * f[c];
     const float r = rsqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    if (!G3 || S) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) S[(size_t)c * n + i] = p[c] * r;
+        for (int c = 0; c < 3; ++c) S[(size_t)c * n + i] = p[c] * r;
+    }
     if constexpr (G3) reinterpret_cast<Spin3*>(S3)[i] = Spin3{p[0] * r, p[1] * r, p[2] * r};
 }
 
@@ -951,6 +944,16 @@ static int heun_bufs(llg_ctx* c, bool g3, HeunBufs& b) {
     return 0;
 }
 
+// S (SoA) from S3 when the corrector left only the AoS3 copy current.  Every reader of
+// c->S outside the Heun kernels calls this first.
+int sync_spins(llg_ctx* c) {
+    if (c->s_soa_valid) return 0;
+    k_from_aos3<<<blocks_for(c->n, kTPB), kTPB, 0, c->stream>>>(c->n, c->S3.as<float>(), c->S.as<float>());
+    SLLG_HIP(hipGetLastError());
+    c->s_soa_valid = true;
+    return 0;
+}
+
 int fused_y_soa(llg_ctx* c) {
     if (c->y_soa_valid) return 0;
     k_from_aos3<<<blocks_for(c->n, kTPB), kTPB, 0, c->stream>>>(c->n, c->Y3.as<float>(), c->Y.as<float>());
@@ -969,6 +972,7 @@ int fused_heun_pred(llg_ctx* c, const StepSched& s) {
         SLLG_HIP(hipGetLastError());
         c->g3_valid = true;
     }
+    if (!g3 || c->ex32.fs_xadj) SLLG_TRY(sync_spins(c));  // this launch reads the SoA spins
     if (cache_thermal(c)) {
         if (c->HT.bytes < 3 * (size_t)c->n * sizeof(float)) SLLG_TRY(c->HT.alloc(3 * (size_t)c->n * sizeof(float)));
         a.th_out = c->HT.as<float>();
@@ -997,6 +1001,10 @@ int fused_heun_corr(llg_ctx* c, const StepSched& s) {
     HeunBufs b;
     SLLG_TRY(heun_bufs(c, g3, b));
     if (cache_thermal(c) && c->HT.p) a.th_in = c->HT.as<float>();
+    // with the AoS3 copy the SoA spins are written only for the four-spin term, which reads
+    // them; the other readers rebuild them on demand (sync_spins)
+    const bool ssoa = !g3 || c->ex32.fs_xadj;
+    if (!ssoa) b.S = nullptr;
     const int tpb = c->tune_tpb, unr = c->tune_unr;
 #define SLLG_L(F, N)                                                                          \
     do {                                                                                      \
@@ -1007,7 +1015,8 @@ int fused_heun_corr(llg_ctx* c, const StepSched& s) {
     SLLG_DISPATCH_FMT_NC(exec_fmt(c), c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
     SLLG_HIP(hipGetLastError());
-    c->g3_valid = g3;  // S3 now matches S only if the corrector wrote it
+    c->g3_valid = g3;  // S3 is current only if the corrector wrote it
+    c->s_soa_valid = ssoa;
     return 0;
 }
 
@@ -1100,6 +1109,7 @@ int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<ui
     uint32_t tile = 0;
     for (size_t k = 0; k < members.size(); ++k) {
         llg_ctx* c = members[k];
+        SLLG_TRY(sync_spins(c));  // the batch kernels read and write the SoA spins
         const StepSched sch{t0[k], nullptr};
         hm[k].a = fused_args(c, sch);
         hm[k].S = c->S.as<float>();

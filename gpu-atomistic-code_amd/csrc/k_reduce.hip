@@ -313,6 +313,7 @@ static int ensure_red(llg_ctx* c) {
 static unsigned red_blocks(uint32_t n) { return std::min<unsigned>(kRedBlocks, std::max<unsigned>(1, blocks_for(n, kTPB))); }
 
 int reduce_mag(llg_ctx* c, double* dev_out) {
+    SLLG_TRY(sync_spins(c));
     SLLG_TRY(ensure_red(c));
     const unsigned nb = red_blocks(c->n);
     const uint8_t* sub = (c->nsub > 1) ? c->sub.as<uint8_t>() : nullptr;
@@ -332,6 +333,7 @@ int reduce_mag(llg_ctx* c, double* dev_out) {
 }
 
 int reduce_energy(llg_ctx* c, const void* dem_now, double* dev_out) {
+    SLLG_TRY(sync_spins(c));
     SLLG_TRY(ensure_red(c));
     const unsigned nb = red_blocks(c->n);
     EnergyArgs a{};
@@ -390,6 +392,7 @@ int noise_sample(llg_ctx* c, uint64_t t, double* dev_out) {
 }
 
 int check_finite(llg_ctx* c, int* bad) {
+    SLLG_TRY(sync_spins(c));
     SLLG_TRY(ensure_red(c));
     int* flag = reinterpret_cast<int*>(c->red_out.as<double>() + 63);
     SLLG_HIP(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
@@ -447,6 +450,7 @@ int set_bins(llg_ctx* c, uint32_t nbins, const uint32_t* bin, const double* weig
 }
 
 int reduce_bins(llg_ctx* c, double* dev_out) {
+    SLLG_TRY(sync_spins(c));
     if (c->nbins == 0) return 0;
     const double* w = c->bin_w.p ? c->bin_w.as<double>() : nullptr;
     if (c->nbin_chunks) {
@@ -512,6 +516,7 @@ int set_sq(llg_ctx* c, const uint32_t grid[3], const uint32_t* mesh, uint32_t nk
 }
 
 int sample_sq(llg_ctx* c, double* dev_out) {
+    SLLG_TRY(sync_spins(c));
     if (c->sq_nk == 0 || c->sq_ngroups == 0) return 0;
     if (c->sq_nch) {
         SqArgs a{};

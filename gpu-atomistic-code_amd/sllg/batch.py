@@ -30,9 +30,14 @@ class Result(C.Structure):
                 ("samples", C.c_uint32), ("pad_", C.c_uint32)]
 
 
+ProgressFn = C.CFUNCTYPE(None, C.c_uint32, C.c_uint32, C.c_double, C.c_void_p)
+
+
 class ServeConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("flags", C.c_uint32), ("hbm_budget_bytes", C.c_uint64),
-                ("max_members", C.c_uint32), ("chunk", C.c_uint32), ("sequential", C.c_int), ("precision", C.c_int)]
+                ("max_members", C.c_uint32), ("chunk", C.c_uint32), ("sequential", C.c_int), ("precision", C.c_int),
+                ("progress", ProgressFn), ("progress_user", C.c_void_p), ("progress_every", C.c_uint32),
+                ("pad_", C.c_uint32)]
 
 
 class ServeStats(C.Structure):
@@ -194,16 +199,18 @@ def make_requests(specs):
 
 
 def serve(requests, device=0, budget_bytes=0, max_members=0, chunk=0, sequential=False, precision=1, graph=False,
-          want_series=False):
+          want_series=False, progress=None, progress_every=0):
     """Run llg_serve_run; returns (results list of dicts, stats dict, series list or None).
 
     With want_series, series[i] is request i's <M> samples (ceil(steps/update) x 3) and, for
     a spin-wave request, results[i]["sq"] its S(k) samples (ceil(steps/update) x (L/2+1))
-    as complex numbers at k = (q, 0, 0)."""
+    as complex numbers at k = (q, 0, 0).  progress(completed, nreq, wall_s) is called after
+    every progress_every-th completion and after the last one."""
     L = _bind()
     n = len(requests)
+    hook = ProgressFn(lambda done, total, wall, user: progress(done, total, wall)) if progress else ProgressFn()
     cfg = ServeConfig(device, BATCH_GRAPH if graph else 0, int(budget_bytes), max_members, chunk,
-                      1 if sequential else 0, precision)
+                      1 if sequential else 0, precision, hook, None, progress_every if progress else 0, 0)
     res = (Result * n)()
     st = ServeStats()
     series = offs = sq = sq_offs = None

@@ -140,6 +140,13 @@ typedef struct llg_serve_config {
     uint32_t chunk;           /* steps per batched launch chunk; 0: the smallest update interval */
     int sequential;           /* 1: no batching - one request at a time in arrival order */
     int precision;            /* LLG_FP32 (batched) or LLG_FP64_PLAIN (sequential only) */
+    /* optional: called on the serving thread after every progress_every-th completion and
+     * after the last one, with the completions so far, nreq and the seconds since the start
+     * (NULL or progress_every 0: never) */
+    void (*progress)(uint32_t completed, uint32_t nreq, double wall_s, void* user);
+    void* progress_user;
+    uint32_t progress_every;
+    uint32_t pad_;
 } llg_serve_config;
 
 typedef struct llg_serve_stats {

@@ -154,8 +154,9 @@ int main(void) {
          offsetof(llg_tune_result, candidates), sizeof(llg_fourspin), offsetof(llg_desc, four));
   printf("%zu %zu %zu %zu\n", sizeof(llg_fft_interaction), offsetof(llg_fft_interaction, J), sizeof(llg_fftexch),
          offsetof(llg_desc, fftx));
-  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(llg_batch_tune), offsetof(llg_batch_tune, candidates),
-         sizeof(llg_request), offsetof(llg_request, seed), sizeof(llg_result), sizeof(llg_serve_stats));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(llg_batch_tune), offsetof(llg_batch_tune, candidates),
+         sizeof(llg_request), offsetof(llg_request, seed), sizeof(llg_result), sizeof(llg_serve_stats),
+         sizeof(llg_serve_config), offsetof(llg_serve_config, progress_every));
   return 0;
 }
 '''
@@ -181,7 +182,8 @@ int main(void) {
     from sllg import batch as SB
     assert [int(x) for x in c4.split()] == [C.sizeof(SB.BatchTune), SB.BatchTune.candidates.offset,
                                             C.sizeof(SB.Request), SB.Request.seed.offset, C.sizeof(SB.Result),
-                                            C.sizeof(SB.ServeStats)]
+                                            C.sizeof(SB.ServeStats), C.sizeof(SB.ServeConfig),
+                                            SB.ServeConfig.progress_every.offset]
 
 
 def _create_rc(sysd):
