@@ -193,17 +193,20 @@ def measure_traffic(args, tune):
             dbs = glob.glob(os.path.join(od, "**", "*.db"), recursive=True)
             if "traffic probe done" not in r.stdout or not dbs:
                 return None, f"rocprofv3 {ctr}: rc {r.returncode}, no results ({r.stderr[-300:]})"
-            vals = defaultdict(list)
+            vals = defaultdict(list)  # per dispatch
             for db in dbs:
-                for (name, counter), (mean, cnt) in rp_pmc(db).items():
+                con = sqlite3.connect(db)
+                for name, counter, value in con.execute(
+                        "select kernel_name, counter_name, value from counters_collection"):
                     if counter != ctr:
                         continue
                     key = ("copy" if "copyBuffer" in name else "corr" if "k_heun_corr" in name
                            else "pred" if "k_heun_pred" in name else None)
                     if key:
-                        vals[key].append(mean)
+                        vals[key].append(value)
             if not all(vals.get(k) for k in ("copy", "corr", "pred")):
                 return None, f"{ctr}: kernels missing from the PMC results ({sorted(vals)})"
+            # the 1 GiB copies are the largest copy dispatches (uploads are copies too)
             scale = (1 << 30) / (max(vals["copy"]) * 1024.0)
             out[ctr] = {"scale": scale, "corr_KiB": float(np.mean(vals["corr"])), "pred_KiB": float(np.mean(vals["pred"]))}
     f, w = out["FETCH_SIZE"], out["WRITE_SIZE"]

@@ -544,6 +544,159 @@ int llg_csr_write(const char* path, uint32_t n, const uint32_t* xadj, const uint
     return 0;
 }
 
+// exch::read4spin + exch::setup4SpinCSR (src/exch_read_4_spin.cpp:29-258).
+int llg_four_spin_quartets(const llg_cfg* exch, uint32_t nms, const double* mu_species, uint32_t n,
+                           const uint32_t* mesh, const uint32_t* sublattice, const uint32_t grid[3],
+                           const uint32_t nk[3], const uint32_t pbc[3], int rounding, uint32_t* xadj, uint32_t* jkl,
+                           uint64_t cap, uint64_t* nq, double* jq_species) {
+    if (!exch || !nq || !mesh || !sublattice || !grid || !nk || !pbc || !mu_species || nms == 0 || (jkl && !xadj))
+        IO_FAIL(LLG_E_INVALID, "null argument");
+    if (rounding != LLG_FOUR_ROUND_NEAREST && rounding != LLG_FOUR_ROUND_REFERENCE)
+        IO_FAIL(LLG_E_INVALID, "unknown rounding %d", rounding);
+    double maxq = 0;
+    if (llg_cfg_get_number(exch, "exchange.MaxQuartetPermutations", 0, &maxq) || maxq < 0)
+        IO_FAIL(LLG_E_INVALID, "could not read the setting exchange.MaxQuartetPermutations (int)");
+    double scale[3] = {1.0, 1.0, 1.0};
+    if (llg_cfg_type(exch, "exchange.Scale") != LLG_CFG_MISSING)
+        for (uint32_t c = 0; c < 3; ++c)
+            if (llg_cfg_get_number(exch, "exchange.Scale", c, &scale[c]))
+                IO_FAIL(LLG_E_INVALID, "could not read exchange.Scale[%u]", c);
+    // per species: quartet count, interface species, mesh offsets [q][m][c]
+    std::vector<uint32_t> nperm(nms);
+    std::vector<int64_t> iface(nms);
+    std::vector<std::vector<int32_t>> off(nms);
+    char key[64], sub[128];
+    for (uint32_t s = 0; s < nms; ++s) {
+        std::snprintf(key, sizeof key, "FourSpinExchange_%u", s);
+        if (llg_cfg_type(exch, key) != LLG_CFG_GROUP)
+            IO_FAIL(LLG_E_INVALID, "setting %s does not exist", key);
+        double v = 0;
+        std::snprintf(sub, sizeof sub, "%s.NumPermutations", key);
+        if (llg_cfg_get_number(exch, sub, 0, &v) || v < 0)
+            IO_FAIL(LLG_E_INVALID, "could not read the number of quartets of species %u", s);
+        if (v > maxq) IO_FAIL(LLG_E_INVALID, "species %u: %g quartets, more than MaxQuartetPermutations %g", s, v, maxq);
+        nperm[s] = (uint32_t)v;
+        std::snprintf(sub, sizeof sub, "%s.JQ", key);
+        if (llg_cfg_get_number(exch, sub, 0, &v)) IO_FAIL(LLG_E_INVALID, "could not read JQ of species %u", s);
+        if (jq_species) jq_species[s] = v / (mu_species[s] * 9.27e-24);
+        std::snprintf(sub, sizeof sub, "%s.InterfaceSpecies", key);
+        if (llg_cfg_get_number(exch, sub, 0, &v)) IO_FAIL(LLG_E_INVALID, "could not read InterfaceSpecies of species %u", s);
+        iface[s] = (int64_t)v;
+        off[s].resize((size_t)nperm[s] * 9);
+        for (uint32_t q = 0; q < nperm[s]; ++q)
+            for (uint32_t m = 0; m < 3; ++m) {
+                std::snprintf(sub, sizeof sub, "%s.Permute%u_%u", key, q + 1, m + 1);
+                for (uint32_t c = 0; c < 3; ++c) {
+                    double o = 0;
+                    if (llg_cfg_get_number(exch, sub, c, &o)) IO_FAIL(LLG_E_INVALID, "setting %s not found", sub);
+                    const double x = o * (double)nk[c] * scale[c];
+                    int32_t r;
+                    if (rounding == LLG_FOUR_ROUND_REFERENCE)
+                        r = (int32_t)(x + 0.5);  // truncation toward zero, as the reference
+                    else
+                        r = x < 0 ? -(int32_t)(-x + 0.5) : (int32_t)(x + 0.5);
+                    off[s][((size_t)q * 3 + m) * 3 + c] = r;
+                }
+            }
+    }
+    // mesh point -> spin
+    const uint64_t gsize = (uint64_t)grid[0] * grid[1] * grid[2];
+    if (gsize == 0 || gsize > (uint64_t)1 << 34) IO_FAIL(LLG_E_INVALID, "bad mesh extent");
+    std::vector<int64_t> at(gsize, -1);
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c)
+            if (mesh[3 * (size_t)i + c] >= grid[c]) IO_FAIL(LLG_E_INVALID, "spin %u lies outside the mesh", i);
+        if (sublattice[i] >= nms) IO_FAIL(LLG_E_INVALID, "spin %u: species %u of %u", i, sublattice[i], nms);
+        const uint64_t g = ((uint64_t)mesh[3 * (size_t)i] * grid[1] + mesh[3 * (size_t)i + 1]) * grid[2] +
+                           mesh[3 * (size_t)i + 2];
+        if (at[g] >= 0) IO_FAIL(LLG_E_INVALID, "spins %lld and %u share a mesh point", (long long)at[g], i);
+        at[g] = i;
+    }
+    uint64_t cnt = 0;
+    if (xadj) xadj[0] = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t sl = sublattice[i];
+        for (uint32_t q = 0; q < nperm[sl]; ++q) {
+            int64_t who[3];
+            bool ok = true;
+            for (uint32_t m = 0; m < 3 && ok; ++m) {
+                int64_t p[3];
+                for (int c = 0; c < 3; ++c) {
+                    p[c] = (int64_t)mesh[3 * (size_t)i + c] + off[sl][((size_t)q * 3 + m) * 3 + c];
+                    const int64_t G = grid[c];
+                    if (p[c] < 0 && pbc[c]) p[c] += G;
+                    else if (p[c] >= G && pbc[c]) p[c] -= G;
+                    ok = ok && p[c] >= 0 && p[c] < G;
+                }
+                if (!ok) break;
+                const int64_t j = at[((uint64_t)p[0] * grid[1] + (uint64_t)p[1]) * grid[2] + (uint64_t)p[2]];
+                ok = j >= 0 && ((int64_t)sublattice[j] == (int64_t)sl || (int64_t)sublattice[j] == iface[sl]);
+                who[m] = j;
+            }
+            if (!ok) continue;
+            if (jkl) {
+                if (cnt >= cap) IO_FAIL(LLG_E_INVALID, "more than %llu quartets", (unsigned long long)cap);
+                for (int m = 0; m < 3; ++m) jkl[3 * cnt + (uint64_t)m] = (uint32_t)who[m];
+            }
+            ++cnt;
+        }
+        if (xadj) {
+            if (cnt > 0xFFFFFFFFull) IO_FAIL(LLG_E_INVALID, "more than 2^32 quartets");
+            xadj[i + 1] = (uint32_t)cnt;
+        }
+    }
+    *nq = cnt;
+    return 0;
+}
+
+// 4spin_csr_exch_mat.dat (src/exch_read_4_spin.cpp:260-381).
+int llg_four_csr_read(const char* path, uint32_t n, uint32_t* xadj, uint32_t* jkl, uint64_t* nq) {
+    if (!path || !nq) IO_FAIL(LLG_E_INVALID, "null argument");
+    std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "rb"));
+    if (!f) IO_FAIL(LLG_E_INVALID, "could not open four-spin CSR matrix %s", path);
+    Tokens tk(f.get());
+    std::string t;
+    uint64_t lns = 0, adjsize = 0;
+    if (!tk.next(t) || !parse_u64(t, &lns) || !tk.next(t) || !parse_u64(t, &adjsize))
+        IO_FAIL(LLG_E_INVALID, "%s: could not read the header", path);
+    if (lns != n) IO_FAIL(LLG_E_INVALID, "%s: %llu spins in the file, %u in the system", path, (unsigned long long)lns, n);
+    if (adjsize < 1) IO_FAIL(LLG_E_INVALID, "%s: the quartet list is empty", path);
+    *nq = adjsize;
+    if (!jkl) return 0;
+    if (!xadj) IO_FAIL(LLG_E_INVALID, "null argument");
+    for (uint32_t i = 0; i <= n; ++i) {
+        uint64_t v;
+        if (!tk.next(t) || !parse_u64(t, &v) || v > adjsize) IO_FAIL(LLG_E_INVALID, "%s: bad row pointer %u", path, i);
+        xadj[i] = (uint32_t)v;
+        if (i > 0 && xadj[i] < xadj[i - 1]) IO_FAIL(LLG_E_INVALID, "%s: row pointers decrease at row %u", path, i);
+    }
+    if (xadj[0] != 0 || xadj[n] != adjsize)
+        IO_FAIL(LLG_E_INVALID, "%s: row pointers do not span the %llu quartets", path, (unsigned long long)adjsize);
+    for (uint64_t q = 0; q < 3 * adjsize; ++q) {
+        uint64_t j;
+        if (!tk.next(t) || !parse_u64(t, &j) || j >= n)
+            IO_FAIL(LLG_E_INVALID, "%s: bad spin index at quartet %llu", path, (unsigned long long)(q / 3));
+        jkl[q] = (uint32_t)j;
+    }
+    return 0;
+}
+
+int llg_four_csr_write(const char* path, uint32_t n, const uint32_t* xadj, const uint32_t* jkl, uint64_t nq) {
+    if (!path || !xadj || (nq && !jkl)) IO_FAIL(LLG_E_INVALID, "null argument");
+    if (xadj[0] != 0 || xadj[n] != nq)
+        IO_FAIL(LLG_E_INVALID, "row pointers do not span the %llu quartets", (unsigned long long)nq);
+    std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "w"));
+    if (!f) IO_FAIL(LLG_E_INVALID, "could not open %s for writing", path);
+    FILE* o = f.get();
+    std::fprintf(o, "%u\n%llu\n", n, (unsigned long long)nq);
+    for (uint32_t i = 0; i < n; ++i) std::fprintf(o, "%u\t", xadj[i]);
+    std::fprintf(o, "%u\n", xadj[n]);
+    for (uint64_t q = 0; q < nq; ++q) std::fprintf(o, "%u\t%u\t%u\t", jkl[3 * q], jkl[3 * q + 1], jkl[3 * q + 2]);
+    std::fprintf(o, "\n");
+    if (std::ferror(o)) IO_FAIL(LLG_E_INVALID, "write error on %s", path);
+    return 0;
+}
+
 int llg_read_spins_text(const char* path, uint32_t n, double* sx, double* sy, double* sz) {
     if (!path || !sx || !sy || !sz) IO_FAIL(LLG_E_INVALID, "null argument");
     std::unique_ptr<FILE, FileCloser> f(std::fopen(path, "rb"));

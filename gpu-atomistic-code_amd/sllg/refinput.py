@@ -8,6 +8,8 @@ configuration file onto the descriptor:
   system.Periodic_Boundaries, system.Include_dipole   (src/geom_glob.cpp, src/config.cpp)
   exchange.Read_exch_method = "unitcell", exchange.Exchfile, or exchange.ReadExchangeMatrix
   with exchange.ExchangeMatrixFilename                 (src/exch_routines.cpp:38-114)
+  the exchange file's FourSpin quartets (FourSpinExchange_<s>), or 4spin_csr_exch_mat.dat
+  with exchange.ReadExchangeMatrix4Spin                (src/exch_read_4_spin.cpp:29-381)
   llg.dt, llg.intscheme, llg.applied, llg.update, llg.staggered_field.sapplied<k>,
   llg.SpinConfigMethod ("random", "restart_text", "grid", or the unit cell's initial
   spins)                                               (src/llg.cpp:237-500)
@@ -64,6 +66,10 @@ def _io():
         for nm in ("llg_read_grid", "llg_write_grid"):
             getattr(L, nm).argtypes = [C.c_char_p, C.c_uint32, up, dp, dp, dp]
         L.llg_write_vtu.argtypes = [C.c_char_p, C.c_uint32, dp, dp, dp, dp]
+        L.llg_four_spin_quartets.argtypes = [P, C.c_uint32, dp, C.c_uint32, up, up, up, up, up, C.c_int, up, up,
+                                             C.c_uint64, C.POINTER(C.c_uint64), dp]
+        L.llg_four_csr_read.argtypes = [C.c_char_p, C.c_uint32, up, up, C.POINTER(C.c_uint64)]
+        L.llg_four_csr_write.argtypes = [C.c_char_p, C.c_uint32, up, up, C.c_uint64]
         _bound = True
     return L
 
@@ -177,6 +183,51 @@ def csr_write(path: str, xadj, adjncy, vals3):
                            _p(v, C.c_double), len(adj)), L)
 
 
+FOUR_ROUND = {"nearest": 0, "reference": 1}
+
+
+def four_spin_quartets(ex: Config, nms: int, mu_species, mesh_xyz, sublattice, grid, nk, pbc, rounding="nearest"):
+    """Four-spin quartets of every spin from the exchange file's FourSpinExchange_<s> groups
+    (llg_four_spin_quartets; src/exch_read_4_spin.cpp:29-258).  rounding "nearest" rounds the
+    quartet offsets to mesh points half away from zero; "reference" repeats the reference's
+    static_cast<int>(x + 0.5), which moves negative offsets one mesh point up.  Returns
+    (xadj [n+1], jkl [nq, 3], jq per species in tesla)."""
+    L = _io()
+    u32 = lambda a: np.ascontiguousarray(a, np.uint32)
+    mesh_xyz, sub = u32(mesh_xyz).reshape(-1, 3), u32(sublattice)
+    n = len(sub)
+    g, k, p = u32(grid), u32(nk), u32(pbc)
+    mu = np.ascontiguousarray(mu_species, np.float64)
+    nq = C.c_uint64()
+    args = (ex.h, nms, _p(mu, C.c_double), n, _p(mesh_xyz, C.c_uint32), _p(sub, C.c_uint32), _p(g, C.c_uint32),
+            _p(k, C.c_uint32), _p(p, C.c_uint32), FOUR_ROUND[rounding])
+    _check(L.llg_four_spin_quartets(*args, None, None, 0, C.byref(nq), None), L)
+    xadj = np.zeros(n + 1, np.uint32)
+    jkl = np.zeros((nq.value, 3), np.uint32)
+    jq = np.zeros(nms)
+    _check(L.llg_four_spin_quartets(*args, _p(xadj, C.c_uint32), _p(jkl, C.c_uint32), nq.value, C.byref(nq),
+                                    _p(jq, C.c_double)), L)
+    return xadj, jkl, jq
+
+
+def four_csr_read(path: str, n: int):
+    """4spin_csr_exch_mat.dat -> (xadj [n+1], jkl [nq, 3])."""
+    L = _io()
+    nq = C.c_uint64()
+    _check(L.llg_four_csr_read(path.encode(), n, None, None, C.byref(nq)), L)
+    xadj = np.zeros(n + 1, np.uint32)
+    jkl = np.zeros((nq.value, 3), np.uint32)
+    _check(L.llg_four_csr_read(path.encode(), n, _p(xadj, C.c_uint32), _p(jkl, C.c_uint32), C.byref(nq)), L)
+    return xadj, jkl
+
+
+def four_csr_write(path: str, xadj, jkl):
+    L = _io()
+    xadj = np.ascontiguousarray(xadj, np.uint32)
+    jkl = np.ascontiguousarray(jkl, np.uint32).reshape(-1, 3)
+    _check(L.llg_four_csr_write(path.encode(), len(xadj) - 1, _p(xadj, C.c_uint32), _p(jkl, C.c_uint32), len(jkl)), L)
+
+
 def read_spins_text(path: str, n: int):
     """restart_text spin file -> (3, n), normalised."""
     L = _io()
@@ -244,9 +295,13 @@ def _ncomp_of(ints, n):
 
 
 def load_reference_input(cff_path: str, precision: str = "fp32", scheme: Optional[str] = None,
-                         noise: str = "counter") -> ReferenceInput:
+                         noise: str = "counter", four_rounding: str = "nearest") -> ReferenceInput:
     """Build the System a reference configuration file describes (paths inside it are
-    relative to the configuration file's directory, as when the reference runs there)."""
+    relative to the configuration file's directory, as when the reference runs there).
+    With exchange.FourSpin set in the exchange file, the four-spin quartets are built from
+    its FourSpinExchange_<s> groups (rounding of their offsets: four_spin_quartets), or read
+    from 4spin_csr_exch_mat.dat when the configuration sets exchange.ReadExchangeMatrix4Spin;
+    the term takes species 0's JQ, as the reference's GPU kernel does."""
     base = os.path.dirname(os.path.abspath(cff_path))
     cfg = Config(cff_path)
 
@@ -269,6 +324,8 @@ def load_reference_input(cff_path: str, precision: str = "fp32", scheme: Optiona
     n = int(np.prod(dims)) * na
     mu_atom = [a.mu for a in atoms]
 
+    exfile = cfg.get("exchange.Exchfile")
+    ex = Config(rel(exfile)) if exfile is not None else None
     if cfg.get("exchange.ReadExchangeMatrix", False):
         xadj, adj, vals = csr_read(rel(cfg.get("exchange.ExchangeMatrixFilename", "csr_exch_mat.dat")), n)
         ncomp = 3 if np.any(vals[0] != vals[1]) or np.any(vals[1] != vals[2]) else 1
@@ -280,7 +337,8 @@ def load_reference_input(cff_path: str, precision: str = "fp32", scheme: Optiona
                                       f"exchange.ReadExchangeMatrix are read")
         if cfg.get("exchange.TruncateExchange", False):
             raise NotImplementedError("exchange.TruncateExchange = TRUE is not read")
-        ex = Config(rel(cfg.get("exchange.Exchfile")))
+        if ex is None:
+            raise ValueError("exchange.Exchfile is not set")
         ints = read_exch_unitcell(ex, na)
         ncomp = _ncomp_of(ints, len(ints))
         xadj, adj, vals = build_csr(uc, dims, ints, mu_atom, ncomp=ncomp, pbc=pbc)
@@ -309,6 +367,17 @@ def load_reference_input(cff_path: str, precision: str = "fp32", scheme: Optiona
     if cfg.get("system.Include_dipole", False):
         from . import mesh as _mesh
         sysd.dip = dict(enabled=1, mesh=_mesh(uc, dims), dim=dims, nk=nk, abc=tuple(cfg.get("system.abc")))
+    if ex is not None and ex.get("exchange.FourSpin", False):
+        from . import mesh as _mesh
+        sub = np.array([atoms[i % na].sublattice for i in range(n)], np.uint32)
+        mu_s = np.zeros(nms)
+        for a in atoms:
+            mu_s[a.sublattice] = a.mu
+        grid = tuple(int(d) * int(k) for d, k in zip(dims, nk))
+        fx, fj, jq = four_spin_quartets(ex, nms, mu_s, _mesh(uc, dims), sub, grid, nk, pbc, four_rounding)
+        if cfg.get("exchange.ReadExchangeMatrix4Spin", False):
+            fx, fj = four_csr_read(rel("4spin_csr_exch_mat.dat"), n)
+        sysd.four = dict(xadj=fx, jkl=fj, jq=float(jq[0]), cpu_form=0)
 
     scm = cfg.get("llg.SpinConfigMethod")
     if scm == "random":

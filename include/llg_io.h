@@ -11,6 +11,10 @@
  *                            energy-unit conversion (src/exch_determine_exchange_matrix.cpp:27-144)
  *   llg_csr_read / _write <- csr_exch_mat.dat reading (src/exch.cpp:63-132) and writing
  *                            (src/exch_unitcell.cpp:35-70)
+ *   llg_four_spin_quartets<- exch::read4spin + exch::setup4SpinCSR, the four-spin quartets of
+ *                            the exchange file (src/exch_read_4_spin.cpp:29-258)
+ *   llg_four_csr_read /   <- 4spin_csr_exch_mat.dat reading and writing
+ *   _write                   (src/exch_read_4_spin.cpp:260-381)
  *   llg_*_spins_text,     <- restart_text / grid spin files and the ParaView spin map
  *   llg_*_grid,              (src/spins.cpp:342-468, src/simple.cpp:114-141, src/util.cpp:265-370)
  *   llg_write_vtu
@@ -98,6 +102,38 @@ int llg_read_exch_unitcell(const llg_cfg* exch, uint32_t n_atoms, llg_interactio
 int llg_csr_read(const char* path, uint32_t n, uint32_t* xadj, uint32_t* adjncy, double* vals3, uint64_t* nnz);
 int llg_csr_write(const char* path, uint32_t n, const uint32_t* xadj, const uint32_t* adjncy, const double* vals3,
                   uint64_t nnz);
+
+/* ---------------------------------------------------------------- four-spin quartets */
+
+/* Rounding of the quartet offsets to mesh points: nearest (half away from zero), or the
+ * reference's static_cast<int>(x + 0.5) (src/exch_read_4_spin.cpp:114), which truncates
+ * toward zero, so a negative offset comes out one mesh point too large (-1 -> 0, -2 -> -1). */
+enum { LLG_FOUR_ROUND_NEAREST = 0, LLG_FOUR_ROUND_REFERENCE = 1 };
+
+/* Four-spin quartets of every spin from the exchange file (exch::read4spin and
+ * exch::setup4SpinCSR, src/exch_read_4_spin.cpp:29-258).  The file holds, per species
+ * s < nms, a group FourSpinExchange_<s> with NumPermutations (at most
+ * exchange.MaxQuartetPermutations), JQ (joules), InterfaceSpecies and Permute<q>_<m>
+ * (q = 1..NumPermutations, m = 1, 2, 3: the offsets of spins j, k, l from spin i in unit
+ * cells).  An offset becomes o * nk[c] * Scale[c] mesh points (exchange.Scale; 1 when the
+ * file has none: the reference reads it only for its "permute" and "mapint" methods),
+ * rounded as `rounding` says.  Spin i (mesh coordinates mesh[i][0..2] = lu(i, 0..2),
+ * species sublattice[i]) gets quartet q of its species when, after one periodic wrap along
+ * each axis with pbc set (grid[c] mesh points per axis), all three mesh points lie on the
+ * grid and hold spins whose species is s or InterfaceSpecies; quartets are listed per spin
+ * in permutation order.  xadj [n+1] and jkl [cap][3] receive the quartet CSR (jkl == NULL:
+ * only *nq is returned); jq_species[nms] (may be NULL) receives JQ / (mu_s mu_B) in tesla
+ * (mu_species in Bohr magnetons; the GPU term takes species 0's value, as the reference's
+ * kernel does).  Two spins on one mesh point are an error. */
+int llg_four_spin_quartets(const llg_cfg* exch, uint32_t nms, const double* mu_species, uint32_t n,
+                           const uint32_t* mesh, const uint32_t* sublattice, const uint32_t grid[3],
+                           const uint32_t nk[3], const uint32_t pbc[3], int rounding, uint32_t* xadj, uint32_t* jkl,
+                           uint64_t cap, uint64_t* nq, double* jq_species);
+
+/* 4spin_csr_exch_mat.dat: "n\nnq\n" xadj[0..n] then per quartet "j k l".  Fails unless the
+ * file's n equals n.  Two-pass: jkl == NULL returns only *nq. */
+int llg_four_csr_read(const char* path, uint32_t n, uint32_t* xadj, uint32_t* jkl, uint64_t* nq);
+int llg_four_csr_write(const char* path, uint32_t n, const uint32_t* xadj, const uint32_t* jkl, uint64_t nq);
 
 /* ---------------------------------------------------------------- spin configuration files */
 
