@@ -159,8 +159,8 @@ def traffic_probe(args):
     g.set_spins(s0)
     g.set_temperature(args.T)
     g.set_launch(*args.launch)
-    g.step(0, 3)
-    g.step(3, 10)
+    g.step(0, 50)  # the clocks settle before the profiled steps
+    g.step(50, 100)
     g.sync()
     g.close()
     nbytes = 1 << 30

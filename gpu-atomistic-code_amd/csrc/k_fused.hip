@@ -336,6 +336,7 @@ struct FusedArgs {
     uint32_t il_na;  // > 1: sublattice-interleaved tile order (il_na sublattices of il_cells rows)
     uint32_t il_cells;
     int tile_sub;    // every tile lies in one sublattice block (perm.ncell % blockDim.x == 0)
+    int na_sh, ncell_sh;  // log2 of perm.na and perm.ncell when powers of two (shifts), else -1
     float* th_out;   // predictor: store the thermal field it drew (or null)
     const float* th_in;  // corrector: read it instead of drawing again (or null)
     float rdt, half_rdt, sqrtT;
@@ -367,11 +368,15 @@ __device__ __forceinline__ uint32_t tile_index(int xcd) {
 // L % na, cell tile L / na, so an XCD's range covers one cell range of every sublattice and
 // the neighbours its rows gather (all sublattices, nearby cells) are fetched into its L2
 // once, not once per sublattice.  Requires il_cells % blockDim.x == 0 (checked on the host).
+template <int TPB>
 __device__ __forceinline__ uint32_t row_tile(const FusedArgs& a) {
     uint32_t L = tile_index(a.xcd);
     if (a.il_na > 1) {
-        const uint32_t tps = a.il_cells / blockDim.x;
-        L = (L % a.il_na) * tps + L / a.il_na;
+        const uint32_t tps = a.il_cells / TPB;
+        if (a.na_sh >= 0)
+            L = (L & (a.il_na - 1u)) * tps + (L >> a.na_sh);
+        else
+            L = (L % a.il_na) * tps + L / a.il_na;
     }
     return L;
 }
@@ -380,13 +385,14 @@ __device__ __forceinline__ uint32_t row_tile(const FusedArgs& a) {
 // lies inside one sublattice block, the block's sublattice and first cell are uniform
 // (scalar arithmetic once per block) and the lane adds its offset; otherwise the lane
 // inverts the permutation itself.
-__device__ __forceinline__ void row_of_thread(const FusedArgs& a, uint32_t tpb, uint32_t& i, uint32_t& iref) {
-    const uint32_t d0 = row_tile(a) * tpb;
+template <int TPB>
+__device__ __forceinline__ void row_of_thread(const FusedArgs& a, uint32_t& i, uint32_t& iref) {
+    const uint32_t d0 = row_tile<TPB>(a) * TPB;
     i = d0 + threadIdx.x;
     if (a.perm.na <= 1) {
         iref = i;
     } else if (a.tile_sub) {
-        const uint32_t t = d0 / a.perm.ncell;
+        const uint32_t t = a.ncell_sh >= 0 ? d0 >> a.ncell_sh : d0 / a.perm.ncell;
         iref = (d0 - t * a.perm.ncell + threadIdx.x) * a.perm.na + t;
     } else {
         iref = a.perm.to_ref(i);
@@ -459,7 +465,10 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
         h[1] += a.stg[n + i];
         h[2] += a.stg[2 * (size_t)n + i];
     }
-    const Species<float> sp = a.multi ? a.sp[a.spec[i]] : a.sp0;
+    // one species: its constants come from the kernel arguments (scalar loads), not through
+    // a pointer that a per-spin species index would need
+    Species<float> sp = a.sp0;
+    if (a.multi) sp = a.sp[a.spec[i]];
     if (a.noise == 1) {
         float th[3];
         if (a.th_in) {
@@ -520,7 +529,7 @@ inline size_t line_lds_bytes(const Exch<float>& e) {
 }
 
 // With G3 the kernels gather neighbours (and read the row's own spin) from the AoS3 copy
-// (S3 / Y3, [3 i + c]) and keep it current: the predictor writes Y3 (and Y only when Y is
+// (S3 / Y3, [3 i + c]), keep A = S + rdt/2 f(S) in the same layout, and keep S3 current: the predictor writes Y3 (and Y only when Y is
 // non-null: the four-spin term or a dipolar refresh at the predicted spins reads it), the
 // corrector S3 (and S only when S is non-null: the four-spin term reads it; every other
 // reader of the spins first rebuilds S from S3, sync_spins).
@@ -531,7 +540,7 @@ __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __r
                                                     float* __restrict__ Y3, float* __restrict__ A) {
     SLLG_STAGE_DICT(FMT, NC, a)
     uint32_t i, iref;
-    row_of_thread(a, TPB, i, iref);
+    row_of_thread<TPB>(a, i, iref);
     if (!row_live<FMT>(a, i)) return;
     const uint32_t n = a.n;
     float s[3], f[3];
@@ -540,8 +549,13 @@ __global__ __launch_bounds__(TPB) void k_heun_pred(FusedArgs a, const float* __r
 #pragma unroll
     for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
     const float r = rsqrtf(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+    if constexpr (G3) {  // A in the AoS3 layout too: one load in the corrector
+        reinterpret_cast<Spin3*>(A)[i] =
+            Spin3{s[0] + a.half_rdt * f[0], s[1] + a.half_rdt * f[1], s[2] + a.half_rdt * f[2]};
+    } else {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
+        for (int c = 0; c < 3; ++c) A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
+    }
     if (!G3 || Y) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) Y[(size_t)c * n + i] = y[c] * r;
@@ -555,12 +569,19 @@ __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __r
                                                     float* __restrict__ S, float* __restrict__ S3) {
     SLLG_STAGE_DICT(FMT, NC, a)
     uint32_t i, iref;
-    row_of_thread(a, TPB, i, iref);
+    row_of_thread<TPB>(a, i, iref);
     if (!row_live<FMT>(a, i)) return;
     const uint32_t n = a.n;
     float av[3];  // loaded before the field so its latency overlaps the neighbour gathers
+    if constexpr (G3) {
+        const Spin3 v = reinterpret_cast<const Spin3*>(A)[i];
+        av[0] = v.x;
+        av[1] = v.y;
+        av[2] = v.z;
+    } else {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) av[c] = A[(size_t)c * n + i];
+        for (int c = 0; c < 3; ++c) av[c] = A[(size_t)c * n + i];
+    }
     float s[3], f[3];
     fused_rhs<FMT, NC, UNR, G3, LFB>(a, i, iref, Y, G3 ? Y3 : Y, sdict, s, f, a.t);
     float p[3];
@@ -657,7 +678,7 @@ template <int FMT, int NC, int M>
 __global__ __launch_bounds__(kTPB) void k_rk4(FusedArgs a, const float* __restrict__ Yin, float* __restrict__ Yout,
                                               float* __restrict__ ACC, float* __restrict__ S) {
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = row_tile(a) * kTPB + threadIdx.x;
+    const uint32_t i = row_tile<kTPB>(a) * kTPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
@@ -699,7 +720,7 @@ __global__ __launch_bounds__(kTPB) void k_rk5(FusedArgs a, const float* __restri
                                               float* __restrict__ S, float* __restrict__ K0, float* __restrict__ K1,
                                               float* __restrict__ K2, float* __restrict__ K3, float* __restrict__ K4) {
     SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = row_tile(a) * kTPB + threadIdx.x;
+    const uint32_t i = row_tile<kTPB>(a) * kTPB + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t n = a.n;
     float s[3], f[3];
@@ -781,6 +802,9 @@ static FusedArgs fused_args(const llg_ctx* c, const StepSched& s) {
         a.il_cells = c->perm.ncell;
     }
     a.tile_sub = (c->perm.na > 1 && c->perm.ncell % 1024u == 0) ? 1 : 0;
+    auto log2_exact = [](uint32_t v) -> int { return (v && !(v & (v - 1))) ? __builtin_ctz(v) : -1; };
+    a.na_sh = log2_exact(c->perm.na);
+    a.ncell_sh = log2_exact(c->perm.ncell);
     a.rdt = (float)c->rdt;
     a.half_rdt = (float)(0.5 * c->rdt);
     a.sqrtT = (float)std::sqrt(c->T);
