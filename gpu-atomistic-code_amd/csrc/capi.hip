@@ -636,6 +636,12 @@ int llg_reduce(llg_ctx* c, double* m, double* energy) {
     return 0;
 }
 
+int llg_reduce_mag_noalloc(llg_ctx* c, double* m) {
+    if (!c || !m) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
+    SLLG_HIP(hipSetDevice(c->device));
+    return reduce_mag_noalloc(c, m);
+}
+
 int llg_set_bins(llg_ctx* c, uint32_t nbins, const uint32_t* bin, const double* weight) {
     if (!c || (!bin && nbins)) SLLG_FAIL(LLG_E_INVALID, "NULL argument");
     SLLG_HIP(hipSetDevice(c->device));

@@ -6,6 +6,8 @@
 // shuffles and a fixed-order combine of its 4 waves; a single block then sums the
 // per-block partials in a fixed order.  No floating-point atomics, so the result is
 // bitwise reproducible for a given n and launch shape.
+#include <mutex>
+
 #include "field.hpp"
 #include "kernels.hpp"
 #include "philox.hpp"
@@ -329,6 +331,68 @@ int reduce_mag(llg_ctx* c, double* dev_out) {
     SLLG_HIP(hipGetLastError());
     k_sum_partials<<<1, kTPB, 0, c->stream>>>(nb, c->nsub * 3, c->red_part.as<double>(), inv, dev_out);
     SLLG_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------- allocation-free <M>
+
+// Device scratch of llg_reduce_mag_noalloc: one file-scope array per GPU (each GPU that runs
+// the library holds its own copy until the process ends).  Layout of one call:
+//   [0, nb nsub 3)          per-block partial sums (every slot written by its block)
+//   [kPartDoubles, + 16)    1 / (spins per sublattice)
+//   [kPartDoubles + 16, +48) the results, nsub x 3
+// Every call on one GPU uses the same array: calls that overlap on one GPU (two host
+// threads, or two streams without ordering between them) race.
+constexpr size_t kPartDoubles = (size_t)kRedBlocks * kMaxSub * 3;
+constexpr size_t kScratchDoubles = kPartDoubles + kMaxSub + kMaxSub * 3;  // 197,120 bytes
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};  // device address of each GPU's copy of `scratch`
+}  // namespace
+
+// Address of `scratch` on device dev (current), looked up once per device.
+static int scratch_of(int dev, double** out) {
+    if (dev < 0 || dev >= kMaxDevices) SLLG_FAIL(LLG_E_INVALID, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            SLLG_FAIL(LLG_E_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return 0;
+}
+
+int reduce_mag_noalloc(llg_ctx* c, double* m) {
+    const uint32_t nsub = c->nsub;
+    const unsigned nb = red_blocks(c->n);
+    if (nsub == 0 || nsub > kMaxSub || (size_t)nb * nsub * 3 > kPartDoubles)
+        SLLG_FAIL(LLG_E_UNSUPPORTED, "allocation-free reduction: %u sublattices x %u blocks exceed its %zu-double "
+                  "device scratch", nsub, nb, kScratchDoubles);
+    double* base = nullptr;
+    SLLG_TRY(scratch_of(c->device, &base));
+    double* part = base;
+    double* inv = base + kPartDoubles;
+    double* out = inv + kMaxSub;
+    SLLG_TRY(sync_spins(c));  // rebuilds S in place when only its AoS3 copy is current
+    double hinv[kMaxSub];
+    for (uint32_t s = 0; s < nsub; ++s) hinv[s] = c->sub_count[s] ? 1. / (double)c->sub_count[s] : 0.0;
+    SLLG_HIP(hipMemcpyAsync(inv, hinv, nsub * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const uint8_t* sub = (nsub > 1) ? c->sub.as<uint8_t>() : nullptr;
+    if (c->prec == LLG_FP64_PLAIN)
+        k_mag_partial<double><<<nb, kTPB, 0, c->stream>>>(c->n, nsub, c->S.as<double>(), sub, part);
+    else
+        k_mag_partial<float><<<nb, kTPB, 0, c->stream>>>(c->n, nsub, c->S.as<float>(), sub, part);
+    SLLG_HIP(hipGetLastError());
+    k_sum_partials<<<1, kTPB, 0, c->stream>>>(nb, nsub * 3, part, inv, out);
+    SLLG_HIP(hipGetLastError());
+    SLLG_HIP(hipMemcpyAsync(m, out, nsub * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    SLLG_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
 

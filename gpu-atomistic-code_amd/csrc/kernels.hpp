@@ -23,7 +23,8 @@ struct StepSched {
 int fused_heun_pred(llg_ctx* c, const StepSched& s);
 int fused_heun_corr(llg_ctx* c, const StepSched& s);
 int fused_y_soa(llg_ctx* c);  // rebuild the SoA predictor output from its AoS3 copy
-int sync_spins(llg_ctx* c);   // rebuild the SoA spins from their AoS3 copy when only that is current
+int sync_spins(llg_ctx* c);
+int reduce_mag_noalloc(llg_ctx* c, double* m);  // <M> per sublattice through the device scratch array   // rebuild the SoA spins from their AoS3 copy when only that is current
 int fused_rk_stage(llg_ctx* c, const StepSched& s, int m);
 int fused_detfield(llg_ctx* c, const float* Sp, double* out);
 int fused_rk_init_tables();

@@ -150,6 +150,7 @@ def lib():
         L.llg_step.argtypes = [P, C.c_uint64, C.c_uint32]
         L.llg_sync.argtypes = [P]
         L.llg_reduce.argtypes = [P, dp, dp]
+        L.llg_reduce_mag_noalloc.argtypes = [P, dp]
         L.llg_run_sampled.argtypes = [P, C.c_uint64, C.c_uint32, C.c_uint32, dp]
         L.llg_noise_sample.argtypes = [P, C.c_uint64, dp]
         L.llg_stream.argtypes = [P]
@@ -537,6 +538,12 @@ class Context:
         e = C.c_double(0.0)
         _check(self.L.llg_reduce(self.h, _p(m, C.c_double), C.byref(e) if energy else None), self.L)
         return m.reshape(self.nsub, 3), (e.value if energy else None)
+
+    def reduce_mag_noalloc(self):
+        """<M> per sublattice through llg_reduce_mag_noalloc (no device allocation)."""
+        m = np.zeros(self.nsub * 3)
+        _check(self.L.llg_reduce_mag_noalloc(self.h, _p(m, C.c_double)), self.L)
+        return m.reshape(self.nsub, 3)
 
     def run_sampled(self, t0, nsteps, every):
         nrec = (nsteps + every - 1) // every

@@ -289,6 +289,15 @@ int llg_get_field(llg_ctx* ctx, double* hx, double* hy, double* hz);
  * Zeeman + staggered + dipolar/2 at the current spins, thermal part excluded. */
 int llg_reduce(llg_ctx* ctx, double* m, double* energy);
 
+/* llg_reduce(ctx, m, NULL) without any device allocation: the per-block partial sums and the
+ * results go through a fixed device array of the library (197,120 bytes per GPU, held until the
+ * process ends), so callers that cannot allocate (a tight serving loop, a harness that forbids
+ * hipMalloc) can sample <M>.  No counterpart in the reference.  Same kernels and block count
+ * as llg_reduce, so the same bits.  At most 16 sublattices (LLG_E_UNSUPPORTED beyond the
+ * array).  Calls that overlap on one GPU (other host threads, unordered streams) race for
+ * the array. */
+int llg_reduce_mag_noalloc(llg_ctx* ctx, double* m);
+
 /* Binned FP64 sums on the device: out[b*3+c] = sum over spins i with bin[i] == b of
  * weight[i] * S_c(i) (weight NULL = 1; bin[i] >= nbins leaves spin i out).  bin and weight
  * are in the reference spin order and are uploaded once by llg_set_bins.  This is the
