@@ -440,8 +440,10 @@ def serving_block(args):
 
 def accuracy(args):
     """FP32 fused Heun vs the FP64 plain path (bit-identical to the FP64 oracle) on the
-    1M-spin fcc case at T = 0 from the same random state, and FP64 vs FP64 with a 1e-15
-    perturbation of one component (the Lyapunov growth the dynamics imposes)."""
+    1M-spin fcc case at T = 0 from the same state, beside two FP64 controls that show what
+    the dynamics does to a perturbation alone: the start with 1e-15 added to every x
+    component, and the start rounded to FP32 (a perturbation of the size FP32 storage makes,
+    ~3e-8 per component), each against the unperturbed FP64 run."""
     import sllg
     from sllg import workloads as W
 
@@ -469,6 +471,9 @@ def accuracy(args):
         sp[0] += 1e-15
         sp /= np.linalg.norm(sp, axis=0)
         lya = run("fp64", sp)
+        sr = s_init.astype(np.float32).astype(np.float64)
+        sr /= np.linalg.norm(sr, axis=0)
+        rnd = run("fp64", sr)
         rows = []
         for c in checkpoints:
             rows.append({"step": c,
@@ -476,6 +481,8 @@ def accuracy(args):
                          "fp32_abs_dM": float(np.abs(f32[c][1] - ref[c][1]).max()),
                          "fp64_perturbed_max_abs_dS": float(np.abs(lya[c][0] - ref[c][0]).max()),
                          "fp64_perturbed_abs_dM": float(np.abs(lya[c][1] - ref[c][1]).max()),
+                         "fp64_fp32_rounded_start_max_abs_dS": float(np.abs(rnd[c][0] - ref[c][0]).max()),
+                         "fp64_fp32_rounded_start_abs_dM": float(np.abs(rnd[c][1] - ref[c][1]).max()),
                          "M_fp64": [float(x) for x in ref[c][1]]})
         print(json.dumps({"mode": "accuracy", "start": start, "n_spins": sysd.n, "T": 0.0, "rows": rows}),
               flush=True)

@@ -127,23 +127,19 @@ int download_state(llg_ctx* c, const void* src_dev, bool src_f64, double* x, dou
 
 template <class R>
 void make_anis(const llg_anisotropy& an, double basemm, AnisG<R>& g) {
-    // prefactor 2K/(mu_0 mu_B) as in src/cuint.cu:98-137
-    g.k2perp_pf = (R)(2.0 * an.k2perp / basemm);
-    g.k2par_pf = (R)(2.0 * an.k2par / basemm);
-    g.k4perp_pf = (R)(2.0 * an.k4perp / basemm);
-    g.k4par_pf = (R)(2.0 * an.k4par / basemm);
-    for (int a = 0; a < 3; ++a) {
-        g.k2perp_dir[a] = (R)an.k2perp_dir[a];
-        g.k2par_dir[a] = (R)an.k2par_dir[a];
+    // prefactor 2K/(mu_0 mu_B) as in src/cuint.cu:98-137, terms in the reference's order
+    g = AnisG<R>{};
+    auto add = [&](double K, const double* d) {
+        g.lin_pf[g.nlin] = (R)(2.0 * K / basemm);
+        for (int a = 0; a < 3; ++a) g.lin_dir[g.nlin][a] = (R)d[a];
+        g.nlin++;
+    };
+    if (an.k2perp != 0.0) add(an.k2perp, an.k2perp_dir);
+    if (an.k2par != 0.0) add(an.k2par, an.k2par_dir);
+    for (int dir = 0; dir < 3; ++dir) {
+        if (an.k4perp != 0.0) add(an.k4perp, an.k4perp_dirs + 3 * dir);
+        if (an.k4par != 0.0) add(an.k4par, an.k4par_dirs + 3 * dir);
     }
-    for (int a = 0; a < 9; ++a) {
-        g.k4perp_dirs[a] = (R)an.k4perp_dirs[a];
-        g.k4par_dirs[a] = (R)an.k4par_dirs[a];
-    }
-    g.has_k2perp = an.k2perp != 0.0;
-    g.has_k2par = an.k2par != 0.0;
-    g.has_k4perp = an.k4perp != 0.0;
-    g.has_k4par = an.k4par != 0.0;
 }
 
 int take_noise(llg_ctx* c, const double** dev_xi) {

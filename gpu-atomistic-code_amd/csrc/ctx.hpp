@@ -59,10 +59,14 @@ template <class R> struct Species {
 };
 
 // Global anisotropy with prefactors 2K/(mu_0 mu_B) precomputed on the host in FP64.
+// K2 perp / par and K4 per axis (src/cuint.cu:98-137): each term is linear in S,
+// h += pf (S.d) d, pf = 2K/(mu_0 mu_B); the terms present, in the reference's order (K2 perp,
+// K2 par, then per axis K4 perp and K4 par), as one list, so that absent terms cost nothing.
 template <class R> struct AnisG {
-    R k2perp_pf, k2par_pf, k4perp_pf, k4par_pf;
-    R k2perp_dir[3], k2par_dir[3], k4perp_dirs[9], k4par_dirs[9];
-    int has_k2perp, has_k2par, has_k4perp, has_k4par;
+    int nlin;
+    int pad_;
+    R lin_pf[8];
+    R lin_dir[8][3];
 };
 
 // Exchange operands on the device (one struct for every format; unused fields null).

@@ -139,32 +139,12 @@ __device__ __forceinline__ void local_terms(R k1u, const R d[3], R kc, const Ani
 #pragma unroll
         for (int a = 0; a < 3; ++a) h[a] += (k1u * sdn * d[a]);
     }
-    if (an.has_k2perp) {
-        const R sd = s[0] * an.k2perp_dir[0] + s[1] * an.k2perp_dir[1] + s[2] * an.k2perp_dir[2];
+#pragma unroll 1
+    for (int q = 0; q < an.nlin; ++q) {  // a uniform loop: no work for absent terms
+        const R* dq = an.lin_dir[q];
+        const R sd = s[0] * dq[0] + s[1] * dq[1] + s[2] * dq[2];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) h[a] += an.k2perp_pf * sd * an.k2perp_dir[a];
-    }
-    if (an.has_k2par) {
-        const R sd = s[0] * an.k2par_dir[0] + s[1] * an.k2par_dir[1] + s[2] * an.k2par_dir[2];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) h[a] += an.k2par_pf * sd * an.k2par_dir[a];
-    }
-    if (an.has_k4perp || an.has_k4par) {
-#pragma unroll
-        for (int dir = 0; dir < 3; ++dir) {
-            const R* dp = &an.k4perp_dirs[dir * 3];
-            const R* da = &an.k4par_dirs[dir * 3];
-            const R sdp = s[0] * dp[0] + s[1] * dp[1] + s[2] * dp[2];
-            const R sda = s[0] * da[0] + s[1] * da[1] + s[2] * da[2];
-            if (an.has_k4perp) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) h[a] += an.k4perp_pf * sdp * dp[a];
-            }
-            if (an.has_k4par) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a) h[a] += an.k4par_pf * sda * da[a];
-            }
-        }
+        for (int a = 0; a < 3; ++a) h[a] += an.lin_pf[q] * sd * dq[a];
     }
     if (kc != R(0)) {
 #pragma unroll

@@ -39,6 +39,39 @@ def test_fcc_1m_fp32_tracks_fp64_plain(gpu):
     assert d <= 1e-4 and dm <= 1e-5
 
 
+@pytest.mark.timeout(900)
+def test_fcc_1m_fp32_ordered_start_10k_steps(gpu):
+    """The accuracy target of the FP32 form at full size: 10^4 Heun steps at T = 0 of the 1M-spin
+    fcc system from an ordered start (+z with a Gaussian tilt of 0.05 rad per transverse
+    component: damped spin waves), FP32 fused path (tuned launch, line reader) vs FP64 plain
+    path: <= 1e-4 per spin component and <= 1e-5 in <M>.  (The uniformly random start, a
+    chaotic quench at T = 0, is measured by bench.py --mode accuracy beside FP64 controls.)"""
+    sf, s0 = _fcc("fp32")
+    sd, _ = _fcc("fp64")
+    rng = np.random.default_rng(11)
+    so = np.zeros_like(s0)
+    so[0] = 0.05 * rng.standard_normal(sf.n)
+    so[1] = 0.05 * rng.standard_normal(sf.n)
+    so[2] = 1.0
+    so /= np.linalg.norm(so, axis=0)
+    out = []
+    for sysd in (sf, sd):
+        g = sllg.Context(sysd)
+        if sysd is sf:
+            g.autotune(5)
+        g.set_spins(so)
+        g.set_temperature(0.0)
+        for k in range(10):
+            g.step(1000 * k, 1000)
+            print(f"{sysd.precision} step {1000 * (k + 1)}", flush=True)
+        out.append((g.get_spins(), g.reduce(energy=False)[0].mean(axis=0)))
+        g.close()
+    d = float(np.abs(out[0][0] - out[1][0]).max())
+    dm = float(np.abs(out[0][1] - out[1][1]).max())
+    print("1M fcc ordered start, 10^4 steps T=0: max|dS| FP32 vs FP64", d, "|d<M>|", dm)
+    assert d <= 1e-4 and dm <= 1e-5
+
+
 def test_fcc_1m_norms_energy_and_reproducibility(gpu):
     """At full size: spin norms stay 1 to FP32 rounding, the energy at T = 0 (damped, no
     field) does not rise over 4 blocks of 50 steps, and a thermal run repeated from the same
