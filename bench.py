@@ -412,16 +412,19 @@ def serving_block(args):
     specs = config5_specs(args.seed, args.requests, args.rate, args.lmin, args.lmax, args.smin, args.smax)
     _serve_warmup()
     _, st, _ = SB.serve(SB.make_requests(specs), budget_bytes=int(args.budget_gb * 2**30), graph=True,
-                        progress=_progress("serving stream"), progress_every=256)
+                        progress=_progress("serving stream"), progress_every=256, max_live_spins=args.max_live_spins)
     mean_su = float(np.mean([s["L"] ** 3 * s["steps"] for s in specs]))
     out = {"workload": f"{args.requests} Poisson requests at {args.rate:g}/s, simple-cubic M(T) and field-ramp "
                        f"solves, L log-uniform in [{args.lmin}, {args.lmax}], {args.smin}-{args.smax} Heun steps, "
                        f"T uniform in [0, 1.5 Tc], NN J=3e-21 J, K=1e-23 J, FP32, seed {args.seed}",
-           "batched_fp32": _stats(st), "mean_spin_updates_per_request": mean_su}
+           "batched_fp32": _stats(st), "mean_spin_updates_per_request": mean_su,
+           "admission": f"FIFO by arrival within an HBM budget of {args.budget_gb:g} GiB and "
+                        f"{args.max_live_spins} live spins (0: no spin cap)"}
     # sub-capacity: the same kind of stream at an arrival rate the device keeps up with
     sub = config5_specs(args.seed + 1, args.sub_requests, args.sub_rate, args.lmin, args.lmax, args.smin, args.smax)
     _, ss, _ = SB.serve(SB.make_requests(sub), budget_bytes=int(args.budget_gb * 2**30), graph=True,
-                        progress=_progress("sub-capacity stream"), progress_every=64)
+                        progress=_progress("sub-capacity stream"), progress_every=64,
+                        max_live_spins=args.max_live_spins)
     out["sub_capacity"] = {"requests": args.sub_requests, "rate_per_s": args.sub_rate, "batched_fp32": _stats(ss)}
     # sequential FP64 plain solves of a seeded sample of the stream, back to back
     if args.seq_every > 0:
@@ -586,7 +589,7 @@ def serve_mode(args):
     reqs = SB.make_requests(specs)
     _serve_warmup()
     _, st, _ = SB.serve(reqs, budget_bytes=int(args.budget_gb * 2**30), graph=True,
-                        progress=_progress("stream"), progress_every=64)
+                        progress=_progress("stream"), progress_every=64, max_live_spins=args.max_live_spins)
     out = {"mode": "serve", "requests": args.requests, "rate_per_s": args.rate, "batched_fp32": _stats(st)}
     if args.sequential:
         _, sq, _ = SB.serve(reqs, sequential=True, precision=sllg.FP64_PLAIN, progress=_progress("sequential"),
@@ -672,6 +675,8 @@ def main():
     ap.add_argument("--smin", type=int, default=2000)
     ap.add_argument("--smax", type=int, default=20000)
     ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--max-live-spins", type=int, default=1 << 24,
+                    help="serving: FIFO admission holds the live requests' spins below this (0: no cap)")
     ap.add_argument("--sequential", action="store_true", help="serve: also run the sequential FP64 plain comparator")
     ap.add_argument("--no-serve", action="store_true", help="headline: skip the serving measurement")
     ap.add_argument("--seed", type=int, default=7)

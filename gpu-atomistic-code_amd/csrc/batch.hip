@@ -834,11 +834,18 @@ extern "C" int llg_serve_run(const llg_serve_config* cfg, const llg_request* req
             // phase with the others, so launch chunks keep the full sampling interval instead of
             // shrinking to the gaps between the requests' sampling steps
             bool in_phase = true;
-            for (const auto& lv : live) in_phase = in_phase && lv.done % reqs[lv.req].update == 0;
+            uint64_t live_spins = 0;
+            for (const auto& lv : live) {
+                in_phase = in_phase && lv.done % reqs[lv.req].update == 0;
+                live_spins += (uint64_t)reqs[lv.req].L * reqs[lv.req].L * reqs[lv.req].L;
+            }
             while (in_phase && next < nreq && reqs[order[next]].arrival_s <= now) {
                 if (cfg->max_members && live.size() >= cfg->max_members) break;
                 const uint32_t i = order[next];
                 const llg_request& r = reqs[i];
+                const uint64_t nspins = (uint64_t)r.L * r.L * r.L;
+                if (cfg->max_live_spins && !live.empty() && live_spins + nspins > cfg->max_live_spins) break;
+                live_spins += nspins;
                 ScLattice* lat = nullptr;
                 SLLG_TRY(lattice_of(r, &lat));
                 llg_species sp;

@@ -37,7 +37,7 @@ class ServeConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("flags", C.c_uint32), ("hbm_budget_bytes", C.c_uint64),
                 ("max_members", C.c_uint32), ("chunk", C.c_uint32), ("sequential", C.c_int), ("precision", C.c_int),
                 ("progress", ProgressFn), ("progress_user", C.c_void_p), ("progress_every", C.c_uint32),
-                ("pad_", C.c_uint32)]
+                ("max_live_spins", C.c_uint32)]
 
 
 class ServeStats(C.Structure):
@@ -199,7 +199,7 @@ def make_requests(specs):
 
 
 def serve(requests, device=0, budget_bytes=0, max_members=0, chunk=0, sequential=False, precision=1, graph=False,
-          want_series=False, progress=None, progress_every=0):
+          want_series=False, progress=None, progress_every=0, max_live_spins=0):
     """Run llg_serve_run; returns (results list of dicts, stats dict, series list or None).
 
     With want_series, series[i] is request i's <M> samples (ceil(steps/update) x 3) and, for
@@ -210,7 +210,8 @@ def serve(requests, device=0, budget_bytes=0, max_members=0, chunk=0, sequential
     n = len(requests)
     hook = ProgressFn(lambda done, total, wall, user: progress(done, total, wall)) if progress else ProgressFn()
     cfg = ServeConfig(device, BATCH_GRAPH if graph else 0, int(budget_bytes), max_members, chunk,
-                      1 if sequential else 0, precision, hook, None, progress_every if progress else 0, 0)
+                      1 if sequential else 0, precision, hook, None, progress_every if progress else 0,
+                      int(max_live_spins))
     res = (Result * n)()
     st = ServeStats()
     series = offs = sq = sq_offs = None

@@ -146,7 +146,8 @@ typedef struct llg_serve_config {
     void (*progress)(uint32_t completed, uint32_t nreq, double wall_s, void* user);
     void* progress_user;
     uint32_t progress_every;
-    uint32_t pad_;
+    uint32_t max_live_spins;  /* 0: unlimited; else a request waits (FIFO) while admitting it would take
+                                 the live requests' spins above this, unless none is live */
 } llg_serve_config;
 
 typedef struct llg_serve_stats {

@@ -381,6 +381,19 @@ def test_nonfinite_batch_member_is_isolated(gpu):
         assert np.array_equal(bad.get_spins(k), good.get_spins(gk))
 
 
+def test_serve_live_spin_cap(gpu):
+    """A cap on the live requests' spins: four 8,000-spin requests arriving together under a
+    16,000-spin cap run two at a time, first come first served, and each gives the <M> series
+    it gives without the cap (bit for bit)."""
+    specs = [dict(L=20, steps=300, update=100, T=100.0, arrival_s=0.0, seed=80 + k) for k in range(4)]
+    res_c, st_c, ser_c = SB.serve(SB.make_requests(specs), graph=True, want_series=True, max_live_spins=16000)
+    res_u, st_u, ser_u = SB.serve(SB.make_requests(specs), graph=True, want_series=True)
+    assert st_c["completed"] == 4 and st_c["max_live"] == 2 and st_u["max_live"] == 4
+    assert res_c[2]["admit_s"] >= min(res_c[0]["done_s"], res_c[1]["done_s"]) - 1e-9
+    for k in range(4):
+        assert np.array_equal(ser_c[k], ser_u[k])
+
+
 def test_serve_waits_for_memory_then_admits(gpu):
     """An HBM budget that holds one request at a time: all requests arrive at once, each waits
     for the previous one to leave, and all complete."""
