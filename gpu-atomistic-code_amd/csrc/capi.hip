@@ -879,7 +879,7 @@ extern "C" int llg_set_launch(llg_ctx* c, int tpb, int xcd, int unroll, int ther
     c->tune_unr = unroll;
     c->tune_thc = thermal_cache ? 1 : 0;
     c->tune_g3 = gather_copy ? 1 : 0;
-    c->tune_line = line_mode ? 1 : 0;
+    c->tune_line = line_mode == 2 ? 2 : (line_mode ? 1 : 0);
     return 0;
 }
 
@@ -926,14 +926,23 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     }
     const TuneChoice caller{c->tune_tpb, c->tune_xcd, c->tune_unr, c->tune_thc, c->tune_g3, c->tune_line};
     const int tpbs[3] = {256, 512, 1024};
-    // line encodings: line reader or the packed entries kept beside it (unless pinned)
-    const int nline = (is_line_fmt(c->ex32.fmt) && !(c->flags & LLG_FLAG_TUNE_FIXED_READER)) ? 2 : 1;
+    // line encodings: tile reader (when the encoding has tile streams), line reader, or the
+    // packed entries kept beside them (unless the caller pinned the reader)
+    std::vector<int> readers;
+    const bool lfmt = is_line_fmt(c->ex32.fmt);
+    if (lfmt && !(c->flags & LLG_FLAG_TUNE_FIXED_READER)) {
+        if (c->ex32.tnw) readers.push_back(2);
+        readers.push_back(1);
+        readers.push_back(0);
+    } else {
+        readers.push_back(c->tune_line);
+    }
     const int unrs[2] = {2, 4};
     // handing the thermal field over only matters at T > 0 with counter noise
     const int nthc = (c->T > 0.0 && c->noise_mode == LLG_NOISE_COUNTER) ? 2 : 1;
     // the AoS3 gather copy applies to the packed and line encodings
-    const int ng3 = (c->ex32.fmt == FMT_SELL_PACK || is_line_fmt(c->ex32.fmt)) ? 2 : 1;
-    const int line0 = c->tune_line;
+    const int ng3 = (c->ex32.fmt == FMT_SELL_PACK || lfmt) ? 2 : 1;
+    const int rd_default = lfmt ? 1 : c->tune_line;
     double best = 1e300;
     TuneChoice b{256, 1, 2, c->tune_thc, c->tune_g3, c->tune_line};
     std::vector<std::pair<double, TuneChoice>> timed;  // every candidate's first timing
@@ -943,18 +952,23 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
     const int noise = c->noise_mode;
     c->noise_mode = LLG_NOISE_COUNTER;  // no host queue traffic while timing
     int rc = 0;
-    for (int li = 0; li < nline && !rc; ++li)
-    for (int gi = 0; gi < ng3 && !rc; ++gi)
+    for (size_t li = 0; li < readers.size() && !rc; ++li) {
+    const int rd = readers[li];
+    // the tile reader has one block size (its workgroup's slices) and stages from the AoS3
+    // copy; the gather depth applies to the packed reader only
+    const bool packed = lfmt ? rd == 0 : c->ex32.fmt == FMT_SELL_PACK;
+    for (int gi = 0; gi < (rd == 2 ? 1 : ng3) && !rc; ++gi)
     for (int hi = 0; hi < nthc && !rc; ++hi)
-    for (int ui = 0; ui < ((nline == 2 ? li == 1 : c->ex32.fmt == FMT_SELL_PACK) ? 2 : 1) && !rc; ++ui)
-        for (int ti = 0; ti < 3 && !rc; ++ti)
+    for (int ui = 0; ui < (packed ? 2 : 1) && !rc; ++ui)
+        for (int ti = 0; ti < (rd == 2 ? 1 : 3) && !rc; ++ti)
             for (int xcd = 1; xcd >= 0 && !rc; --xcd) {
-                c->tune_tpb = tpbs[ti];
+                const int tpb = rd == 2 ? (int)(64 * c->ex32.tnw) : tpbs[ti];
+                c->tune_tpb = tpb;
                 c->tune_xcd = xcd;
                 c->tune_unr = unrs[ui];
                 if (nthc == 2) c->tune_thc = 1 - hi;
-                if (ng3 == 2) c->tune_g3 = 1 - gi;
-                c->tune_line = nline == 2 ? 1 - li : line0;
+                if (ng3 == 2) c->tune_g3 = rd == 2 ? 1 : 1 - gi;
+                c->tune_line = rd;
                 rc = do_steps(c, 0, 2);
                 if (rc) break;
                 (void)hipEventRecord(e0, c->stream);
@@ -964,15 +978,16 @@ extern "C" int llg_autotune(llg_ctx* c, uint32_t nsteps, llg_tune_result* out) {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, e0, e1);
                 const double per = ms / nsteps;
-                if (tpbs[ti] == 256 && xcd == 1 && unrs[ui] == 2 && hi == 0 && gi == 0 && li == 0)
+                if (tpb == 256 && xcd == 1 && unrs[ui] == 2 && hi == 0 && gi == 0 && rd == rd_default)
                     out->ms_per_step_default = per;
                 out->candidates++;
-                timed.push_back({per, TuneChoice{tpbs[ti], xcd, unrs[ui], c->tune_thc, c->tune_g3, c->tune_line}});
+                timed.push_back({per, TuneChoice{tpb, xcd, unrs[ui], c->tune_thc, c->tune_g3, c->tune_line}});
                 if (per < best) {
                     best = per;
                     b = timed.back().second;
                 }
             }
+    }
     // The minimum of many short noisy timings favours a lucky candidate: the fastest few are
     // timed again, in turn, over four times as many steps, and the best of those is kept.
     if (!rc && timed.size() > 1) {

@@ -33,6 +33,13 @@ constexpr uint32_t kLineFallback = 0xFFFFFFFFu;  // line-mode slice read from th
 constexpr int line_group(int nc) { return nc == 1 ? 3 : (nc == 3 ? 2 : 1); }
 constexpr int line_words(int nc) { return (1 + 2 * line_group(nc) + 3 * line_group(nc) * nc + 3) / 4 * 4; }
 
+// LDS-staged line reader ("tile" reader, beside a 64-cell line encoding without packed
+// slices): a workgroup of tnw slices stages the union of their neighbour lines in LDS as
+// [line][component][64] floats, 768 bytes per line, and reads its windows from there.
+constexpr uint32_t kTileLineBytes = 3 * 64 * 4;
+constexpr uint32_t kTileLdsMax = 64 * 1024;  // dynamic LDS of one workgroup
+constexpr int tile_head(int nw) { return (2 + 3 * nw + 7) / 8 * 8; }  // start of the staged-line list
+
 enum ExchFmt : int {
     FMT_NONE = 0,
     FMT_DIA = 1,
@@ -100,6 +107,13 @@ template <class V> struct Exch {
     const uint2* lslice = nullptr;
     const uint32_t* lent = nullptr;       // group records
     uint32_t lfall = 0;                   // slices that use the packed entries
+    // tile reader: per workgroup the first word of its stream in tent (streams of equal
+    // content stored once); tnw == 0: no tile streams
+    const uint32_t* tgrp = nullptr;
+    const uint32_t* tent = nullptr;
+    uint32_t tnw = 0;                     // slices (waves) per workgroup
+    uint32_t tumax = 0;                   // most lines any workgroup stages
+    uint32_t tlps = 0;                    // slices per sublattice block
     // four-spin quartets of each row (same row order as the spins of this path)
     const uint32_t* fs_xadj = nullptr;    // [n+1]; null = term off
     const uint4* fs_jkl = nullptr;        // (j, k, l, 0)
@@ -163,14 +177,15 @@ struct KernelTimer;  // capi.hip
 struct ExchStore {
     int device = 0, prec = 0;
     uint32_t n = 0;
-    DevBuf a, b, c, d, e, dict64, fs_x, fs_q;
+    DevBuf a, b, c, d, e, f, g, dict64, fs_x, fs_q;
     Exch<float> ex32;
     Exch<double> ex64;
     Perm perm;
     std::string encoding;
     HostExch h;  // scalars kept; the large host vectors are released after upload
     uint64_t bytes() const {
-        return a.bytes + b.bytes + c.bytes + d.bytes + e.bytes + dict64.bytes + fs_x.bytes + fs_q.bytes;
+        return a.bytes + b.bytes + c.bytes + d.bytes + e.bytes + f.bytes + g.bytes + dict64.bytes + fs_x.bytes +
+               fs_q.bytes;
     }
 };
 
@@ -204,7 +219,7 @@ struct llg_ctx {
     int tune_unr = 2;    // packed-exchange chunks loaded ahead per wave (llg_autotune)
     int tune_thc = 1;    // predictor hands its thermal field to the corrector (llg_autotune)
     int tune_g3 = 1;     // gather neighbours from the AoS3 copy of the spins (llg_autotune)
-    int tune_line = 1;   // use the line reader when the couplings have a line encoding (llg_autotune)
+    int tune_line = 1;   // line encodings: 0 packed entries, 1 line reader, 2 tile reader (llg_autotune)
     bool g3_valid = false;   // S3 holds the current S
     bool s_soa_valid = true;  // S (SoA) holds the current spins (else only S3 does: see sync_spins)
     bool y_soa_valid = true;  // Y (SoA) holds the last predictor output (else only Y3 does)

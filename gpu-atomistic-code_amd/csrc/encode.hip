@@ -178,6 +178,115 @@ struct TupleHash {
     }
 };
 
+// Tile streams of the LDS-staged line reader, made from the line records of a 64-cell line
+// encoding in which every slice is a line.  A workgroup takes nw slices: nw / na
+// consecutive lines of each of the na sublattice blocks (slice q = w nw + k of workgroup w
+// is line q / na of sublattice q % na), so the lines it reads overlap as much as the
+// sublattice-major order allows.  Its stream:
+//   [0]                 lines staged (nU), [1] nw
+//   [2 + k]             slice k's records: first word, relative to the stream
+//   [2 + nw + k]        slice k's group count
+//   [2 + 2 nw + k]      LDS byte offset of slice k's own line
+//   [H, H + nU)         first rows of the staged lines relative to the workgroup's first row
+//                       (line q0 / na of sublattice 0, q0 = w nw), int32; H = tile_head(nw)
+//   then slice k's line records, every window start (current and next group) replaced by
+//   the LDS byte offset of its line, kTileLineBytes per line in staging order.
+// Streams of equal content are stored once (interior workgroups of a periodic lattice
+// repeat); W zero words follow the last stream, so a reader may fetch one record past a
+// slice's last.  Chooses nw = 8, else 16: the first that divides the slices evenly
+// and stages at most kTileLdsMax bytes.  On failure the encoding keeps no tile streams.
+void build_tiles(llg_ctx* c, const std::vector<uint32_t>& words, const std::vector<uint2>& table, int G, int W) {
+    ExchStore& X = *c->xs;
+    Exch<float>& x = X.ex32;
+    const uint32_t n = c->n, nsl = n / kSlice, na = X.perm.na;
+    if (na > 1 && X.perm.ncell % kSlice) return;
+    const uint32_t lps = na > 1 ? X.perm.ncell / kSlice : nsl;
+    for (uint32_t nw : {8u, 16u}) {
+        if (nw % na || nsl % nw || lps % (nw / na)) continue;
+        const uint32_t kk = nw / na, nwg = nsl / nw, H = (uint32_t)tile_head((int)nw);
+        std::vector<uint32_t> tw, tg(nwg), st;
+        std::unordered_map<uint64_t, std::vector<uint32_t>> seen;  // stream hash -> workgroups
+        std::vector<uint32_t> offs(nwg);
+        uint32_t umax = 0;
+        bool ok = true;
+        std::vector<int64_t> rows;  // staged lines' first rows, staging order
+        std::unordered_map<int64_t, uint32_t> slot;
+        for (uint32_t w = 0; w < nwg && ok; ++w) {
+            const int64_t B = (int64_t)(w * kk) * kSlice;
+            rows.clear();
+            slot.clear();
+            auto slot_of = [&](int64_t r) -> uint32_t {
+                auto it = slot.find(r);
+                if (it != slot.end()) return it->second;
+                const uint32_t u = (uint32_t)rows.size();
+                slot.emplace(r, u);
+                rows.push_back(r);
+                return u;
+            };
+            st.assign(H, 0u);
+            std::vector<uint32_t> recs;
+            for (uint32_t k = 0; k < nw; ++k) {
+                const uint32_t q = w * nw + k, s = (q % na) * lps + q / na;
+                const int64_t base = (int64_t)s * kSlice;
+                const uint2 t = table[s];
+                st[2 + nw + k] = t.y;
+                st[2 + 2 * nw + k] = slot_of(base) * kTileLineBytes;
+                st[2 + k] = (uint32_t)recs.size();  // made absolute below
+                for (uint32_t g = 0; g < t.y; ++g) {
+                    const uint32_t* r = words.data() + t.x + (size_t)g * W;
+                    const size_t o = recs.size();
+                    recs.insert(recs.end(), r, r + W);
+                    for (int u = 0; u < G; ++u) {
+                        recs[o + 1 + u] = slot_of(base + (int32_t)r[1 + u]) * kTileLineBytes;
+                        recs[o + W - G + u] = slot_of(base + (int32_t)r[W - G + u]) * kTileLineBytes;
+                    }
+                }
+            }
+            const uint32_t nU = (uint32_t)rows.size();
+            if ((uint64_t)nU * kTileLineBytes > kTileLdsMax) {
+                ok = false;
+                break;
+            }
+            umax = std::max(umax, nU);
+            st[0] = nU;
+            st[1] = nw;
+            for (int64_t r : rows) st.push_back((uint32_t)(int32_t)(r - B));
+            while (st.size() % 4) st.push_back(0u);
+            const uint32_t r0 = (uint32_t)st.size();
+            for (uint32_t k = 0; k < nw; ++k) st[2 + k] += r0;
+            st.insert(st.end(), recs.begin(), recs.end());
+            uint64_t hsh = 1469598103934665603ull;
+            for (uint32_t v : st) hsh = (hsh ^ v) * 1099511628211ull;
+            bool found = false;
+            auto& cand = seen[hsh];
+            for (uint32_t o : cand)
+                if (offs[o] + st.size() <= tw.size() && tw[offs[o]] == st[0] &&
+                    std::memcmp(tw.data() + offs[o], st.data(), st.size() * 4) == 0) {
+                    // equal prefix of equal length: the stored stream has the same header,
+                    // so its length is the same
+                    offs[w] = offs[o];
+                    found = true;
+                    break;
+                }
+            if (!found) {
+                cand.push_back(w);
+                offs[w] = (uint32_t)tw.size();
+                tw.insert(tw.end(), st.begin(), st.end());
+            }
+        }
+        if (!ok) continue;
+        tw.insert(tw.end(), (size_t)W, 0u);
+        if (upload(X.f, tw.data(), tw.size(), c->stream)) return;
+        if (upload(X.g, offs.data(), offs.size(), c->stream)) return;
+        x.tent = X.f.as<uint32_t>();
+        x.tgrp = X.g.as<uint32_t>();
+        x.tnw = nw;
+        x.tumax = umax;
+        x.tlps = lps;
+        return;
+    }
+}
+
 // Line mode of the FP32 path.  When the lattice's fastest cell index runs along lines of
 // len = 64 or 16 cells, a 64-row slice is one line (len 64) or four consecutive lines
 // (len 16), and neighbour j of lane l is at row
@@ -372,6 +481,8 @@ bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vec
     if (upload(X.d, words.data(), words.size(), c->stream)) return false;
     if (upload(X.e, table.data(), table.size(), c->stream)) return false;
     Exch<float>& x = X.ex32;
+    x.tnw = x.tumax = x.tlps = 0;
+    if (len == 64 && nfall == 0) build_tiles(c, words, table, G, W);
     x.fmt = len == 64 ? (open ? FMT_SELL_LINE_OPEN : FMT_SELL_LINE) : (open ? FMT_SELL_LINE16_OPEN : FMT_SELL_LINE16);
     x.lslice = X.e.as<uint2>();
     x.lent = X.d.as<uint32_t>();
@@ -585,6 +696,10 @@ int build_sell(llg_ctx* c) {
                     char buf[128];
                     std::snprintf(buf, sizeof buf, "line%u%s-t%u-slices%uof%u-lines%u-packed%u", len, open ? "o" : "",
                                   P.na, lstored, nsl, lines_row, nfall);
+                    if (x.tnw) {  // tile streams: slices per workgroup, most lines staged
+                        const size_t l = std::strlen(buf);
+                        std::snprintf(buf + l, sizeof buf - l, "-tile%ux%u", x.tnw, x.tumax);
+                    }
                     X.encoding = buf;
                     done = true;
                 }

@@ -410,6 +410,92 @@ __device__ __forceinline__ bool row_live(const FusedArgs& a, uint32_t i) {
         return i < a.n;
 }
 
+// Species constants of row i: from the kernel arguments (scalar loads) for one species, not
+// through a pointer that a per-spin species index would need.
+__device__ __forceinline__ Species<float> species_of(const FusedArgs& a, uint32_t i) {
+    Species<float> sp = a.sp0;
+    if (a.multi) sp = a.sp[a.spec[i]];
+    return sp;
+}
+
+// Thermal field of row i at step t (a.noise != 0): counter noise drawn here (and stored for
+// the corrector when a.th_out is set) or taken from the predictor (a.th_in), or the injected
+// noise of the step.
+__device__ __forceinline__ void thermal_field(const FusedArgs& a, uint32_t i, uint32_t iref, const Species<float>& sp,
+                                              uint64_t t, float th[3]) {
+    const uint32_t n = a.n;
+    if (a.noise == 1) {
+        if (a.th_in) {
+            th[0] = a.th_in[i];
+            th[1] = a.th_in[n + i];
+            th[2] = a.th_in[2 * (size_t)n + i];
+        } else {
+            float z[3];
+            normals3_f32(a.seed, a.stream, iref, t, z);
+            const float TP = a.sqrtT * sp.sigma;
+            th[0] = z[0] * TP;
+            th[1] = z[1] * TP;
+            th[2] = z[2] * TP;
+            if (a.th_out) {
+                a.th_out[i] = th[0];
+                a.th_out[n + i] = th[1];
+                a.th_out[2 * (size_t)n + i] = th[2];
+            }
+        }
+    } else {
+        const float TP = a.sqrtT * sp.sigma;
+        th[0] = (float)a.xi[i] * TP;
+        th[1] = (float)a.xi[n + i] * TP;
+        th[2] = (float)a.xi[2 * (size_t)n + i] * TP;
+    }
+}
+
+// Everything of f(S) at row i after the pair exchange h: four-spin term, dipolar field hd
+// (already loaded when EDEM), staggered field, thermal field, applied field, anisotropy,
+// and the LLG right-hand side.  The thermal field is drawn here, after the exchange: a
+// global store (the predictor's hand-off of its draw) before the exchange would make the
+// compiler read the read-only coupling streams with vector instead of scalar loads.
+template <bool EDEM>
+__device__ __forceinline__ void fused_tail(const FusedArgs& a, uint32_t i, uint32_t iref, const float* __restrict__ src,
+                                           const float s[3], float h[3], float hd[3], float f[3], uint64_t t) {
+    const uint32_t n = a.n;
+    if (a.ex.fs_xadj) {
+        float h4[3];
+        fourspin_row<float, float, float>(a.ex, i, src, h4);
+        h[0] += h4[0];
+        h[1] += h4[1];
+        h[2] += h4[2];
+    }
+    if (a.dem) {
+        if constexpr (!EDEM) {
+            hd[0] = a.dem[i];
+            hd[1] = a.dem[n + i];
+            hd[2] = a.dem[2 * (size_t)n + i];
+        }
+        h[0] += hd[0];
+        h[1] += hd[1];
+        h[2] += hd[2];
+    }
+    if (a.stg) {
+        h[0] += a.stg[i];
+        h[1] += a.stg[n + i];
+        h[2] += a.stg[2 * (size_t)n + i];
+    }
+    const Species<float> sp = species_of(a, i);
+    if (a.noise) {
+        float th[3];
+        thermal_field(a, i, iref, sp, t, th);
+        h[0] += th[0];
+        h[1] += th[1];
+        h[2] += th[2];
+    }
+    h[0] += a.B[0];
+    h[1] += a.B[1];
+    h[2] += a.B[2];
+    local_terms<float>(sp.k1u, sp.k1dir, sp.kc, a.an, s, h);
+    llg_rhs<float>(sp.llgpf, sp.lambda, s, h, f);
+}
+
 // t: the step the thermal field is drawn for.  src: the spins (SoA) the field is evaluated at; gsrc: where neighbours are gathered from
 // (src itself, or with G3 its AoS3 copy, which then also gives the row's own spin; src is
 // then read only by the four-spin term).  EDEM: the dipolar field is loaded with the row's
@@ -443,65 +529,7 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
         exch_row_line<NC, G3, line_len(FMT), line_open(FMT), LFB>(a.ex, i, gsrc, sdict, h);
     else
         exch_any<FMT, NC, float, float, float>(a.ex, i, src, h);
-    if (a.ex.fs_xadj) {
-        float h4[3];
-        fourspin_row<float, float, float>(a.ex, i, src, h4);
-        h[0] += h4[0];
-        h[1] += h4[1];
-        h[2] += h4[2];
-    }
-    if (a.dem) {
-        if constexpr (!EDEM) {
-            hd[0] = a.dem[i];
-            hd[1] = a.dem[n + i];
-            hd[2] = a.dem[2 * (size_t)n + i];
-        }
-        h[0] += hd[0];
-        h[1] += hd[1];
-        h[2] += hd[2];
-    }
-    if (a.stg) {
-        h[0] += a.stg[i];
-        h[1] += a.stg[n + i];
-        h[2] += a.stg[2 * (size_t)n + i];
-    }
-    // one species: its constants come from the kernel arguments (scalar loads), not through
-    // a pointer that a per-spin species index would need
-    Species<float> sp = a.sp0;
-    if (a.multi) sp = a.sp[a.spec[i]];
-    if (a.noise == 1) {
-        float th[3];
-        if (a.th_in) {
-            th[0] = a.th_in[i];
-            th[1] = a.th_in[n + i];
-            th[2] = a.th_in[2 * (size_t)n + i];
-        } else {
-            float z[3];
-            normals3_f32(a.seed, a.stream, iref, t, z);
-            const float TP = a.sqrtT * sp.sigma;
-            th[0] = z[0] * TP;
-            th[1] = z[1] * TP;
-            th[2] = z[2] * TP;
-            if (a.th_out) {
-                a.th_out[i] = th[0];
-                a.th_out[n + i] = th[1];
-                a.th_out[2 * (size_t)n + i] = th[2];
-            }
-        }
-        h[0] += th[0];
-        h[1] += th[1];
-        h[2] += th[2];
-    } else if (a.noise == 2) {
-        const float TP = a.sqrtT * sp.sigma;
-        h[0] += (float)a.xi[i] * TP;
-        h[1] += (float)a.xi[n + i] * TP;
-        h[2] += (float)a.xi[2 * (size_t)n + i] * TP;
-    }
-    h[0] += a.B[0];
-    h[1] += a.B[1];
-    h[2] += a.B[2];
-    local_terms<float>(sp.k1u, sp.k1dir, sp.kc, a.an, s, h);
-    llg_rhs<float>(sp.llgpf, sp.lambda, s, h, f);
+    fused_tail<EDEM>(a, i, iref, src, s, h, hd, f, t);
 }
 
 // Stage the coupling dictionary of the packed encoding in LDS (before any early exit).
@@ -593,6 +621,171 @@ __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __r
         for (int c = 0; c < 3; ++c) S[(size_t)c * n + i] = p[c] * r;
     }
     if constexpr (G3) reinterpret_cast<Spin3*>(S3)[i] = Spin3{p[0] * r, p[1] * r, p[2] * r};
+}
+
+// ---------------------------------------------------------------- tile reader
+// LDS-staged line reader of the Heun pair (64-cell line encodings in which every slice is a
+// line, AoS3 spins).  A workgroup of NW waves takes the NW slices its stream describes
+// (encode.hip, build_tiles), loads every line they read once from the AoS3 copy into LDS
+// as [line][component][64] floats, and each wave then runs its slice's line records with
+// the windows read from LDS: a line shared by the workgroup's slices leaves the cache
+// hierarchy once per workgroup instead of once per slice that reads it.  The records,
+// their grouping and the order of the sums are the line reader's, so both readers give
+// the same bits.
+
+struct TileRow {
+    const uint32_t* ts;  // the workgroup's stream
+    uint32_t k, lane, i, iref;
+    int32_t base;        // first row of the workgroup (sublattice 0, its first line)
+};
+
+template <int NW>
+__device__ __forceinline__ TileRow tile_row(const FusedArgs& a) {
+    TileRow r;
+    const uint32_t w = tile_index(a.xcd);
+    r.ts = a.ex.tent + __builtin_amdgcn_readfirstlane(a.ex.tgrp[w]);
+    r.k = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    r.lane = threadIdx.x & 63u;
+    const uint32_t q = w * NW + r.k, na = a.perm.na;
+    uint32_t sub, line, line0;
+    if (a.na_sh >= 0) {
+        sub = q & (na - 1u);
+        line = q >> a.na_sh;
+        line0 = (w * NW) >> a.na_sh;
+    } else {
+        sub = q % na;
+        line = q / na;
+        line0 = w * NW / na;
+    }
+    r.i = (sub * a.ex.tlps + line) * 64u + r.lane;
+    r.iref = na > 1 ? (line * 64u + r.lane) * na + sub : r.i;
+    r.base = (int32_t)(line0 * 64u);
+    return r;
+}
+
+// Wave k stages lines [k c, (k + 1) c) of the workgroup's list, c = its length over NW
+// rounded up to a multiple of 8: per 8 lines one scalar load of their first rows (the list
+// starts 16-byte aligned; words past its end are the stream's own), then up to 8 line loads
+// in flight, then their LDS stores.
+template <int NW>
+__device__ __forceinline__ void tile_stage(const TileRow& r, const float* __restrict__ G3src, uint32_t n, float* lds) {
+    const SpinBuf sb = spin_buf(G3src, n);
+    const uint32_t nU = r.ts[0];
+    const uint32_t cpw = ((nU + NW - 1) / NW + 7) & ~7u;
+    const uint32_t lo = r.k * cpw, hi = min(nU, lo + cpw);
+    for (uint32_t u0 = lo; u0 < hi; u0 += 8) {
+        const uint4* __restrict__ U4 = reinterpret_cast<const uint4*>(r.ts + tile_head(NW) + u0);
+        const uint4 qa = U4[0], qb = U4[1];
+        const uint32_t st[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+        u32x3 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (u0 + j < hi) {
+                const uint32_t row = __builtin_amdgcn_readfirstlane((uint32_t)(r.base + (int32_t)st[j]));
+                v[j] = __builtin_amdgcn_raw_buffer_load_b96(sb.r, r.lane * 12u, row * 12u, 0);
+            }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (u0 + j < hi) {
+                float* d = lds + (u0 + j) * (kTileLineBytes / 4) + r.lane;
+                d[0] = __uint_as_float(v[j].x);
+                d[64] = __uint_as_float(v[j].y);
+                d[128] = __uint_as_float(v[j].z);
+            }
+    }
+}
+
+// The G windows of one record from LDS (byte offsets of their lines in w[first, first + G)).
+template <int NC>
+__device__ __forceinline__ void tile_windows(const float* lds, uint32_t lane, const uint32_t* ws, u32x3* x) {
+#pragma unroll
+    for (int u = 0; u < line_group(NC); ++u) {
+        const float* p = lds + (ws[u] >> 2) + lane;
+        x[u].x = __float_as_uint(p[0]);
+        x[u].y = __float_as_uint(p[64]);
+        x[u].z = __float_as_uint(p[128]);
+    }
+}
+
+// Pair exchange of this wave's slice from LDS; the next record's scalar load is issued
+// before the current record's arithmetic (the stream has one spare record at its end).
+template <int NC, int NW, bool OPEN>
+__device__ __forceinline__ void exch_row_tile(const TileRow& r, const float* lds, float dot[3]) {
+    using RW = LineRecW<NC>;
+    constexpr int W = RW::W;
+    const uint32_t* __restrict__ rp = r.ts + __builtin_amdgcn_readfirstlane(r.ts[2 + r.k]);
+    const uint32_t ng = __builtin_amdgcn_readfirstlane(r.ts[2 + NW + r.k]);
+    float p[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    RW c0 = load_line_rec<NC>(rp);
+    for (uint32_t g = 0; g < ng; ++g) {
+        // LDS reads and scalar loads share one wait counter: the next record's load is
+        // issued with this record's window reads, so their latencies overlap
+        const RW c1 = load_line_rec<NC>(rp + (size_t)(g + 1) * W);
+        u32x3 x[RW::G];
+        tile_windows<NC>(lds, r.lane, c0.w + 1, x);
+        __builtin_amdgcn_sched_barrier(0);
+        line_group_sum<NC>(c0, x, p);
+        c0 = c1;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dot[c] = p[0][c] + line_next<64, OPEN>(p[1][c]) + line_prev<64, OPEN>(p[2][c]);
+}
+
+// Own spin (from LDS), pair exchange, then the rest of f(S).
+template <int NC, int NW, bool OPEN>
+__device__ __forceinline__ void tile_rhs(const FusedArgs& a, const TileRow& r, const float* lds,
+                                         const float* __restrict__ src, float s[3], float f[3]) {
+    const float* o = lds + (r.ts[2 + 2 * NW + r.k] >> 2) + r.lane;
+    s[0] = o[0];
+    s[1] = o[64];
+    s[2] = o[128];
+    float h[3], hd[3] = {0.f, 0.f, 0.f};
+    exch_row_tile<NC, NW, OPEN>(r, lds, h);
+    fused_tail<false>(a, r.i, r.iref, src, s, h, hd, f, a.t);
+}
+
+template <int NC, int NW, bool OPEN>
+__global__ __launch_bounds__(NW * 64) void k_heun_pred_tile(FusedArgs a, const float* __restrict__ S,
+                                                            const float* __restrict__ S3, float* __restrict__ Y,
+                                                            float* __restrict__ Y3, float* __restrict__ A) {
+    extern __shared__ float tl_lds[];
+    const TileRow r = tile_row<NW>(a);
+    tile_stage<NW>(r, S3, a.n, tl_lds);
+    __syncthreads();
+    const uint32_t n = a.n, i = r.i;
+    float s[3], f[3];
+    tile_rhs<NC, NW, OPEN>(a, r, tl_lds, S, s, f);
+    float y[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
+    const float q = rsqrtf(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+    reinterpret_cast<Spin3*>(A)[i] = Spin3{s[0] + a.half_rdt * f[0], s[1] + a.half_rdt * f[1], s[2] + a.half_rdt * f[2]};
+    if (Y) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Y[(size_t)c * n + i] = y[c] * q;
+    }
+    reinterpret_cast<Spin3*>(Y3)[i] = Spin3{y[0] * q, y[1] * q, y[2] * q};
+}
+
+template <int NC, int NW, bool OPEN>
+__global__ __launch_bounds__(NW * 64) void k_heun_corr_tile(FusedArgs a, const float* __restrict__ Y,
+                                                            const float* __restrict__ Y3, const float* __restrict__ A,
+                                                            float* __restrict__ S, float* __restrict__ S3) {
+    extern __shared__ float tl_lds[];
+    const TileRow r = tile_row<NW>(a);
+    const uint32_t n = a.n, i = r.i;
+    const Spin3 av = reinterpret_cast<const Spin3*>(A)[i];  // overlaps the staging loads
+    tile_stage<NW>(r, Y3, a.n, tl_lds);
+    __syncthreads();
+    float s[3], f[3];
+    tile_rhs<NC, NW, OPEN>(a, r, tl_lds, Y, s, f);
+    const float p[3] = {av.x + a.half_rdt * f[0], av.y + a.half_rdt * f[1], av.z + a.half_rdt * f[2]};
+    const float q = rsqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    if (S) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) S[(size_t)c * n + i] = p[c] * q;
+    }
+    reinterpret_cast<Spin3*>(S3)[i] = Spin3{p[0] * q, p[1] * q, p[2] * q};
 }
 
 // SoA -> AoS3 gather copy (after the spins changed outside the Heun kernels).
@@ -944,10 +1137,50 @@ static bool cache_thermal(const llg_ctx* c) { return c->tune_thc && c->T > 0.0 &
 // Readers of the SoA predicted spins: the four-spin term and a dipolar refresh at Y.
 bool batch_needs_y_soa(const llg_ctx* c) { return c->ex32.fs_xadj || (c->dip && c->dip_every_stage); }
 
+// The tile reader runs when chosen (line_mode 2) on an encoding that has tile streams; it
+// stages from the AoS3 copy, so it implies the gather copy.
+static bool use_tile(const llg_ctx* c) {
+    const int f = c->ex32.fmt;
+    return c->tune_line == 2 && c->ex32.tnw && (f == FMT_SELL_LINE || f == FMT_SELL_LINE_OPEN);
+}
+
 // The AoS3 gather copy is used for the packed and line encodings when the tuner chose it.
 static bool use_g3(const llg_ctx* c) {
+    if (use_tile(c)) return true;
     const int f = exec_fmt(c);
     return c->tune_g3 && (f == FMT_SELL_PACK || is_line_fmt(f));
+}
+
+template <int NC, int NW, bool OPEN>
+static void launch_tile_nw(bool pred, const FusedArgs& a, const HeunBufs& b, hipStream_t st) {
+    const uint32_t nwg = a.n / (64u * NW);
+    const size_t lds = (size_t)a.ex.tumax * kTileLineBytes;
+    if (pred)
+        k_heun_pred_tile<NC, NW, OPEN><<<nwg, NW * 64, lds, st>>>(a, b.S, b.S3, b.Y, b.Y3, b.A);
+    else
+        k_heun_corr_tile<NC, NW, OPEN><<<nwg, NW * 64, lds, st>>>(a, b.Y, b.Y3, b.A, b.S, b.S3);
+}
+template <int NC, bool OPEN>
+static void launch_tile_o(bool pred, const FusedArgs& a, const HeunBufs& b, hipStream_t st) {
+    if (a.ex.tnw == 16)
+        launch_tile_nw<NC, 16, OPEN>(pred, a, b, st);
+    else
+        launch_tile_nw<NC, 8, OPEN>(pred, a, b, st);
+}
+template <int NC>
+static void launch_tile_c(bool pred, const FusedArgs& a, const HeunBufs& b, hipStream_t st) {
+    if (a.ex.fmt == FMT_SELL_LINE_OPEN)
+        launch_tile_o<NC, true>(pred, a, b, st);
+    else
+        launch_tile_o<NC, false>(pred, a, b, st);
+}
+static void launch_tile(bool pred, const FusedArgs& a, const HeunBufs& b, hipStream_t st) {
+    if (a.ex.ncomp == 1)
+        launch_tile_c<1>(pred, a, b, st);
+    else if (a.ex.ncomp == 3)
+        launch_tile_c<3>(pred, a, b, st);
+    else
+        launch_tile_c<9>(pred, a, b, st);
 }
 
 static int heun_bufs(llg_ctx* c, bool g3, HeunBufs& b) {
@@ -1006,6 +1239,11 @@ int fused_heun_pred(llg_ctx* c, const StepSched& s) {
     const bool ysoa = !g3 || c->ex32.fs_xadj || (c->dip && c->dip_every_stage);
     if (!ysoa) b.Y = nullptr;
     c->y_soa_valid = ysoa;
+    if (use_tile(c)) {
+        launch_tile(true, a, b, c->stream);
+        SLLG_HIP(hipGetLastError());
+        return 0;
+    }
     const int tpb = c->tune_tpb, unr = c->tune_unr;
 #define SLLG_L(F, N)                                                                          \
     do {                                                                                      \
@@ -1036,7 +1274,10 @@ int fused_heun_corr(llg_ctx* c, const StepSched& s) {
         else if (tpb == 512) launch_heun_corr<F, N, 512>(a, b, c->n, unr, g3, c->stream);     \
         else launch_heun_corr<F, N, 256>(a, b, c->n, unr, g3, c->stream);                     \
     } while (0)
-    SLLG_DISPATCH_FMT_NC(exec_fmt(c), c->ex32.ncomp, SLLG_L);
+    if (use_tile(c))
+        launch_tile(false, a, b, c->stream);
+    else
+        SLLG_DISPATCH_FMT_NC(exec_fmt(c), c->ex32.ncomp, SLLG_L);
 #undef SLLG_L
     SLLG_HIP(hipGetLastError());
     c->g3_valid = g3;  // S3 is current only if the corrector wrote it

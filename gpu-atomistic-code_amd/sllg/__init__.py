@@ -96,7 +96,7 @@ class Desc(C.Structure):
 class Info(C.Structure):
     _fields_ = [("device_bytes", C.c_uint64), ("bytes_per_spin_update", C.c_double),
                 ("dominant_kernel_bytes_per_spin", C.c_double), ("dominant_kernel", C.c_char * 48),
-                ("exch_encoding", C.c_char * 48)]
+                ("exch_encoding", C.c_char * 64)]
 
 
 class TuneResult(C.Structure):

@@ -358,7 +358,8 @@ typedef struct llg_tune_result {
     int unroll;
     int thermal_cache;  /* 1: the predictor hands its thermal field to the corrector */
     int gather_copy;    /* 1: neighbours are gathered from an AoS3 copy of the spins */
-    int line_mode;      /* 1: line reader (line encodings), 0: the packed entries kept beside it */
+    int line_mode;      /* line encodings: 1 line reader, 2 LDS-staged line reader (when the encoding has
+                           tile streams; else as 1), 0 the packed entries kept beside them */
 } llg_tune_result;
 int llg_autotune(llg_ctx* ctx, uint32_t nsteps, llg_tune_result* out);
 int llg_set_launch(llg_ctx* ctx, int tpb, int xcd, int unroll, int thermal_cache, int gather_copy, int line_mode);
@@ -370,7 +371,7 @@ typedef struct llg_info {
     double bytes_per_spin_update; /* whole step */
     double dominant_kernel_bytes_per_spin;
     char dominant_kernel[48];
-    char exch_encoding[48];
+    char exch_encoding[64];
 } llg_info;
 int llg_get_info(llg_ctx* ctx, llg_info* info);
 

@@ -55,11 +55,14 @@ def tensor_csr_system(L=6, precision="fp64", ncomp=9, seed=5, fmt="csr", scheme=
     return sysd, sllg.random_spins(n, seed, seed + 100)
 
 
-def run_pair(sysd, s0, nsteps, T=0.0, B=(0.0, 0.0, 0.0), stag=None, xi=None, oracle_opts=0):
-    """Run the GPU handle and the oracle from the same state; return both contexts."""
+def run_pair(sysd, s0, nsteps, T=0.0, B=(0.0, 0.0, 0.0), stag=None, xi=None, oracle_opts=0, launch=None):
+    """Run the GPU handle (launch shape `launch` if given) and the oracle from the same
+    state; return both contexts."""
     import pyoracle
 
     g = sllg.Context(sysd)
+    if launch is not None:
+        g.set_launch(*launch)
     o = pyoracle.OracleContext(sysd, oracle_opts)
     for c in (g, o):
         c.set_spins(s0)

@@ -504,6 +504,61 @@ def test_line_mode_gather_copy(gpu):
     assert np.array_equal(res[0][1], res[1][1])
 
 
+@pytest.mark.parametrize("case", ["fcc", "sc", "sc64open", "diag", "tensor", "fcc_multi_dip"])
+def test_tile_reader(gpu, case):
+    """The LDS-staged line reader (line_mode 2: a workgroup stages the lines its slices read,
+    then sums the line records over windows read from LDS) gives the line reader's bits: same
+    records, same order of the sums; with and without the thermal hand-off, across a
+    set_spins between steps, with several species, staggered and dipolar fields.  Its
+    trajectory matches the oracle to FP32 rounding."""
+    stag = False
+    if case == "fcc":
+        sysd, s0 = W.fcc_3shell(precision="fp32", dims=(2, 2, 64))
+    elif case == "sc":
+        sysd, s0, _ = W.sc_ferro(precision="fp32", dims=(4, 4, 64))
+    elif case == "sc64open":
+        sysd, s0, _ = W.sc_ferro(precision="fp32", dims=(4, 4, 64), pbc=(1, 1, 0))
+    elif case == "diag":
+        sysd, s0 = tensor_csr_system(precision="fp32", ncomp=3, dims=(2, 4, 64))
+    elif case == "tensor":
+        sysd, s0 = tensor_csr_system(precision="fp32", ncomp=9, dims=(4, 2, 64))
+    else:
+        sysd, s0 = W.fcc_3shell(precision="fp32", dims=(2, 2, 64))
+        sysd.species = [dict(mu=2.22 if t != 2 else 1.5, lambda_=0.05 + 0.02 * t, k1u=1e-23 * (t % 2),
+                             k1u_dir=(0.0, 0.6, 0.8), sublattice=t) for t in range(4)]
+        sysd.spin_species = (np.arange(sysd.n) % 4).astype(np.uint32)
+        sysd.n_sublattice = 4
+        uc = sllg.unit_cell("fcc")
+        sysd.dip = dict(enabled=1, mesh=sllg.mesh(uc, (2, 2, 64)), dim=(2, 2, 64), nk=(2, 2, 2),
+                        abc=(3.6e-10,) * 3, refresh=5)
+        stag = True
+    rng = np.random.default_rng(3)
+    hst = rng.normal(0.0, 0.3, (3, sysd.n))
+    res = []
+    for line_mode, thc in ((1, 1), (2, 1), (2, 0)):
+        g = sllg.Context(sysd)
+        enc = g.info()["exch_encoding"]
+        assert "-tile" in enc, enc
+        g.set_launch(512, 1, 2, thc, 1, line_mode)
+        g.set_spins(s0)
+        g.set_temperature(150.0)
+        g.set_applied(np.array([0.1, 0.0, 0.4]))
+        if stag:
+            g.set_staggered(hst)
+        g.step(0, 25)
+        mid = g.get_spins()
+        g.set_spins(mid[:, ::-1].copy())  # the AoS3 copy the tile reader stages must follow
+        g.step(25, 25)
+        res.append((g.get_spins(), g.field()))
+    print(case, enc)
+    for r in res[1:]:
+        assert np.array_equal(r[0], res[0][0])
+        assert np.array_equal(r[1], res[0][1])
+    if case in ("fcc", "sc64open", "tensor"):
+        g, o = run_pair(sysd, s0, 40, T=0.0, B=(0.0, 0.2, 0.5), launch=(512, 1, 2, 1, 1, 2))
+        assert _maxdiff(g.get_spins(), o.get_spins()) <= 1e-4
+
+
 def _fftx_system(prec, dip, seed=7, dims=(6, 5, 4)):
     """Open sc lattice, two species on a checkerboard, random anisotropic/antisymmetric 3x3
     couplings on the 6 nearest and 12 next-nearest mesh vectors; the same couplings as an
