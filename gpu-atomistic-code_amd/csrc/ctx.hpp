@@ -43,7 +43,7 @@ constexpr int tile_head(int nw) { return (2 + 3 * nw + 7) / 8 * 8; }  // start o
 enum ExchFmt : int {
     FMT_NONE = 0,
     FMT_DIA = 1,
-    FMT_CSR = 2,        // FP64 plain: CSR u32 + FP64 values
+    FMT_CSR = 2,        // FP64 plain: CSR rows stored by 64-row slices, u32 columns + FP64 values
     FMT_STENCIL = 3,    // periodic stencil (shared or per-row values)
     FMT_SELL_PACK = 4,  // FP32: sliced ELL, u32 = (row offset << dbits) | dictionary index
     FMT_SELL_RAW = 5,   // FP32: sliced ELL, u32 col + FP32 values [ncomp][entries]

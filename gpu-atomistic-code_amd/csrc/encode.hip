@@ -137,13 +137,44 @@ int build_exchange_fp64(llg_ctx* c, const llg_exchange& e) {
     fill_common(x, h, c->n);
     X.encoding = "none";
     if (h.fmt == LLG_EXCH_CSR) {
+        // The CSR stored by slices of 64 rows: entry q of every row of a slice at [q][lane]
+        // (column-major within the slice, the slice as long as its longest row), so one load
+        // of entry q by a wave is 64 consecutive words.  Each row keeps its entries in CSR
+        // order and the kernels read a row's own count from the row pointers, so the sums are
+        // the CSR sums, operation for operation.
+        const uint32_t n = c->n, nsl = (n + kSlice - 1) / kSlice, nc = (uint32_t)h.ncomp;
+        std::vector<uint2> sl(nsl);
+        uint64_t tot = 0;
+        for (uint32_t t = 0; t < nsl; ++t) {
+            uint32_t len = 0;
+            for (uint32_t r = t * kSlice; r < std::min(n, (t + 1) * kSlice); ++r) len = std::max(len, h.xadj[r + 1] - h.xadj[r]);
+            sl[t] = make_uint2((uint32_t)tot, len);
+            tot += len;
+        }
+        if (tot > 0xFFFFFFFFull) SLLG_FAIL(LLG_E_INVALID, "exchange too large for the sliced CSR");
+        const size_t nent = (size_t)tot * kSlice;
+        std::vector<uint32_t> ent(nent);
+        std::vector<double> val((size_t)nc * nent, 0.0);
+        for (uint32_t r = 0; r < nsl * kSlice; ++r) {
+            const uint2 t = sl[r / kSlice];
+            const uint32_t rl = r < n ? h.xadj[r + 1] - h.xadj[r] : 0;
+            for (uint32_t q = 0; q < t.y; ++q) {
+                const size_t o = ((size_t)t.x + q) * kSlice + r % kSlice;
+                ent[o] = q < rl ? h.adj[h.xadj[r] + q] : (r < n ? r : 0);  // padding is never read
+                if (q < rl)
+                    for (uint32_t k = 0; k < nc; ++k) val[(size_t)k * nent + o] = h.vals[(size_t)k * h.nnz + h.xadj[r] + q];
+            }
+        }
         SLLG_TRY(upload(X.a, h.xadj.data(), h.xadj.size(), c->stream));
-        SLLG_TRY(upload(X.b, h.adj.data(), h.adj.size(), c->stream));
-        SLLG_TRY(upload(X.c, h.vals.data(), h.vals.size(), c->stream));
+        SLLG_TRY(upload(X.b, ent.data(), ent.size(), c->stream));
+        SLLG_TRY(upload(X.c, val.data(), val.size(), c->stream));
+        SLLG_TRY(upload(X.d, sl.data(), sl.size(), c->stream));
         x.xadj = X.a.as<uint32_t>();
-        x.adj = X.b.as<uint32_t>();
+        x.ent = X.b.as<uint32_t>();
         x.vals = X.c.as<double>();
-        X.encoding = "csr-u32-f64";
+        x.slice = X.d.as<uint2>();
+        x.nent = nent;
+        X.encoding = "csr-sliced64-u32-f64";
     } else if (h.fmt == LLG_EXCH_DIA) {
         SLLG_TRY(upload(X.a, h.doff.data(), h.doff.size(), c->stream));
         SLLG_TRY(upload(X.c, h.vals.data(), h.vals.size(), c->stream));

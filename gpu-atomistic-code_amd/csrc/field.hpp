@@ -92,11 +92,15 @@ __device__ __forceinline__ void exch_row_ref(const Exch<V>& e, uint32_t i, const
     dot[0] = dot[1] = dot[2] = R(0);
     R v[NC];
     if (e.fmt == FMT_CSR) {
-        const uint32_t q0 = e.xadj[i], q1 = e.xadj[i + 1];
-        for (uint32_t q = q0; q < q1; ++q) {
-            const uint32_t j = e.adj[q];
+        // CSR rows stored by 64-row slices, entry q of lane l at [q][l] (encode.hip): the
+        // row's entries in CSR order, its own count from the row pointers
+        const uint32_t rl = e.xadj[i + 1] - e.xadj[i];
+        const size_t o0 = (size_t)e.slice[i >> 6].x * 64u + (i & 63u);
+        for (uint32_t q = 0; q < rl; ++q) {
+            const size_t o = o0 + (size_t)q * 64u;
+            const uint32_t j = e.ent[o];
 #pragma unroll
-            for (int k = 0; k < NC; ++k) v[k] = (R)e.vals[(size_t)k * e.nnz + q];
+            for (int k = 0; k < NC; ++k) v[k] = (R)e.vals[(size_t)k * e.nent + o];
             const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
             accum<NC, R>(v, s, dot);
         }

@@ -695,38 +695,47 @@ __device__ __forceinline__ void tile_stage(const TileRow& r, const float* __rest
     }
 }
 
-// The G windows of one record from LDS (byte offsets of their lines in w[first, first + G)).
+// The G windows of one record from LDS (byte offsets of their lines in ws[0, G)); lds4 is
+// this lane's byte address in the first line.
 template <int NC>
-__device__ __forceinline__ void tile_windows(const float* lds, uint32_t lane, const uint32_t* ws, u32x3* x) {
+__device__ __forceinline__ void tile_windows(const char* lds4, const uint32_t* ws, u32x3* x) {
 #pragma unroll
     for (int u = 0; u < line_group(NC); ++u) {
-        const float* p = lds + (ws[u] >> 2) + lane;
+        const float* p = reinterpret_cast<const float*>(lds4 + ws[u]);
         x[u].x = __float_as_uint(p[0]);
         x[u].y = __float_as_uint(p[64]);
         x[u].z = __float_as_uint(p[128]);
     }
 }
 
-// Pair exchange of this wave's slice from LDS; the next record's scalar load is issued
-// before the current record's arithmetic (the stream has one spare record at its end).
+template <int NC>
+__device__ __forceinline__ void tile_group(const char* lds4, const LineRecW<NC>& c, float p[3][3]) {
+    u32x3 x[LineRecW<NC>::G];
+    tile_windows<NC>(lds4, c.w + 1, x);
+    line_group_sum<NC>(c, x, p);
+}
+
+// Pair exchange of this wave's slice from LDS.  Two records per iteration in two register
+// sets, each record's scalar load issued one group ahead (the stream has one spare record
+// at its end, so the look-ahead past a slice's last record stays in bounds).
 template <int NC, int NW, bool OPEN>
 __device__ __forceinline__ void exch_row_tile(const TileRow& r, const float* lds, float dot[3]) {
     using RW = LineRecW<NC>;
     constexpr int W = RW::W;
     const uint32_t* __restrict__ rp = r.ts + __builtin_amdgcn_readfirstlane(r.ts[2 + r.k]);
     const uint32_t ng = __builtin_amdgcn_readfirstlane(r.ts[2 + NW + r.k]);
+    const char* lds4 = reinterpret_cast<const char*>(lds) + 4u * r.lane;
     float p[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
     RW c0 = load_line_rec<NC>(rp);
-    for (uint32_t g = 0; g < ng; ++g) {
-        // LDS reads and scalar loads share one wait counter: the next record's load is
-        // issued with this record's window reads, so their latencies overlap
+    uint32_t g = 0;
+    for (; g + 1 < ng; g += 2) {
         const RW c1 = load_line_rec<NC>(rp + (size_t)(g + 1) * W);
-        u32x3 x[RW::G];
-        tile_windows<NC>(lds, r.lane, c0.w + 1, x);
-        __builtin_amdgcn_sched_barrier(0);
-        line_group_sum<NC>(c0, x, p);
-        c0 = c1;
+        tile_group<NC>(lds4, c0, p);
+        const RW c2 = load_line_rec<NC>(rp + (size_t)(g + 2) * W);
+        tile_group<NC>(lds4, c1, p);
+        c0 = c2;
     }
+    if (g < ng) tile_group<NC>(lds4, c0, p);
 #pragma unroll
     for (int c = 0; c < 3; ++c) dot[c] = p[0][c] + line_next<64, OPEN>(p[1][c]) + line_prev<64, OPEN>(p[2][c]);
 }
