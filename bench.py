@@ -23,7 +23,7 @@ Also measured in the same run and carried in the same JSON line:
                  solves of a seeded sample of the stream (every 256th request, run back to back)
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
-                  [--mode headline|copy-probe|traffic-probe|accuracy|rk4|batched|serve|fft-exchange]
+                  [--mode headline|copy-probe|traffic-probe|plain-probe|accuracy|rk4|batched|serve|fft-exchange]
   python bench.py --summarise trace|pmc <rocprofv3 results.db> [kernel ...]
 """
 import argparse
@@ -171,6 +171,25 @@ def traffic_probe(args):
         y.copy_(x)
     torch.cuda.synchronize()
     print("traffic probe done", flush=True)
+
+
+def plain_probe(args):
+    """The FP64 plain path on the headline lattice (CSR with 3 FP64 values per entry) for 20
+    steps after 3, for a kernel trace of its SpMV and integrator kernels."""
+    import torch
+    import sllg
+    from sllg import workloads as W
+
+    sysp, sp0 = W.fcc_3shell(L=args.L, precision="fp64", ncomp=3)
+    p = sllg.Context(sysp)
+    p.set_spins(sp0)
+    p.set_temperature(args.T)
+    p.step(0, 3)
+    p.step(3, 20)
+    p.sync()
+    p.close()
+    torch.cuda.synchronize()
+    print("plain probe done", flush=True)
 
 
 def measure_traffic(args, tune):
@@ -649,8 +668,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--mode", default="headline",
-                    choices=["headline", "copy-probe", "traffic-probe", "accuracy", "rk4", "batched", "serve",
-                             "fft-exchange"])
+                    choices=["headline", "copy-probe", "traffic-probe", "plain-probe", "accuracy", "rk4", "batched",
+                             "serve", "fft-exchange"])
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--T", type=float, default=300.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -690,7 +709,8 @@ def main():
               "(replicas and requests would be placed whole per GPU, one large system split into slabs "
               "with a halo exchange per field evaluation); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    modes = {"copy-probe": copy_probe, "traffic-probe": traffic_probe, "accuracy": accuracy, "rk4": rk4_dmi,
+    modes = {"copy-probe": copy_probe, "traffic-probe": traffic_probe, "plain-probe": plain_probe,
+             "accuracy": accuracy, "rk4": rk4_dmi,
              "batched": batched, "serve": serve_mode, "headline": headline, "fft-exchange": fft_exchange}
     modes[args.mode](args)
 
