@@ -63,6 +63,22 @@ struct Spin3 {  // one spin of an AoS3 copy
     float x, y, z;
 };
 
+// A global-memory view of p.  Pointers the kernels read from memory (a batch member's
+// buffers and coupling arrays, read from the member table) are generic to the compiler, and
+// loads and stores through generic pointers become flat instructions: VGPR addresses only,
+// no scalar loads for uniform data, and counted against the LDS wait counter as well.  All
+// these buffers are device memory.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gview(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gview(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 struct SpinBuf {
     __amdgpu_buffer_rsrc_t r;
     uint32_t o1, o2;
@@ -131,14 +147,14 @@ __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_
     // rows come in whole waves (row = 64 k + lane in every caller), so the slice, its
     // descriptor and the loop bounds are wave-uniform: scalar loads and uniform branches
     const uint32_t sl = __builtin_amdgcn_readfirstlane(i >> 6), lane = i & 63u;
-    const uint2 sd = e.slice[sl];
+    const u32x2 sd = gview(reinterpret_cast<const u32x2*>(e.slice))[sl];
     const uint32_t q0 = sd.x, nch = sd.y >> 2;
-    const uint4* __restrict__ p = reinterpret_cast<const uint4*>(e.ent + (size_t)q0 * kSlice) + lane;
+    const auto* __restrict__ p = gview(reinterpret_cast<const u32x4*>(e.ent + (size_t)q0 * kSlice)) + lane;
     const uint32_t db = e.dbits;
     const SpinBuf sb = spin_buf(S, e.n);
     uint32_t c = 0;
     for (; c + UNR <= nch; c += UNR) {
-        uint4 cur[UNR];
+        u32x4 cur[UNR];
 #pragma unroll
         for (int u = 0; u < UNR; ++u) cur[u] = p[(size_t)(c + u) * kSlice];
 #pragma unroll
@@ -150,14 +166,14 @@ __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_
         }
     }
     for (; c < nch; ++c) {
-        const uint4 cur = p[(size_t)c * kSlice];
+        const u32x4 cur = p[(size_t)c * kSlice];
         pack_entry<NC, G3>(cur.x, i, db, sb, sdict, dot);
         pack_entry<NC, G3>(cur.y, i, db, sb, sdict, dot);
         pack_entry<NC, G3>(cur.z, i, db, sb, sdict, dot);
         pack_entry<NC, G3>(cur.w, i, db, sb, sdict, dot);
     }
     if (sd.y & 2u) {  // tail of 2 entries per lane, [lane][2]
-        const uint2 t = reinterpret_cast<const uint2*>(p - lane + (size_t)nch * kSlice)[lane];
+        const u32x2 t = ((const __attribute__((address_space(1))) u32x2*)(p - lane + (size_t)nch * kSlice))[lane];
         pack_entry<NC, G3>(t.x, i, db, sb, sdict, dot);
         pack_entry<NC, G3>(t.y, i, db, sb, sdict, dot);
     }
@@ -180,11 +196,11 @@ struct LineRecW {
 // Whole 16-byte units, so the record arrives in few wide scalar loads.
 template <int NC>
 __device__ __forceinline__ LineRecW<NC> load_line_rec(const uint32_t* __restrict__ p) {
-    const uint4* __restrict__ q = reinterpret_cast<const uint4*>(p);
+    const auto* __restrict__ q = gview(reinterpret_cast<const u32x4*>(p));
     LineRecW<NC> o;
 #pragma unroll
     for (int k = 0; k < LineRecW<NC>::W / 4; ++k) {
-        const uint4 v = q[k];
+        const u32x4 v = q[k];
         o.w[4 * k] = v.x;
         o.w[4 * k + 1] = v.y;
         o.w[4 * k + 2] = v.z;
@@ -276,7 +292,7 @@ __device__ __forceinline__ void exch_row_line(const Exch<float>& e, uint32_t i, 
     using RW = LineRecW<NC>;
     constexpr int G = RW::G, W = RW::W, NX = RW::NX;
     const uint32_t sl = __builtin_amdgcn_readfirstlane(i >> 6);
-    const uint2 sd = e.lslice[sl];
+    const u32x2 sd = gview(reinterpret_cast<const u32x2*>(e.lslice))[sl];
     if constexpr (LFB) {
         if (sd.y == kLineFallback) {
             exch_row_pack_fast<NC, 2, G3>(e, i, S, sdict, dot);
@@ -414,7 +430,17 @@ __device__ __forceinline__ bool row_live(const FusedArgs& a, uint32_t i) {
 // through a pointer that a per-spin species index would need.
 __device__ __forceinline__ Species<float> species_of(const FusedArgs& a, uint32_t i) {
     Species<float> sp = a.sp0;
-    if (a.multi) sp = a.sp[a.spec[i]];
+    if (a.multi) {
+        const auto* q = gview(a.sp) + gview(a.spec)[i];
+        sp.llgpf = q->llgpf;
+        sp.lambda = q->lambda;
+        sp.sigma = q->sigma;
+        sp.k1u = q->k1u;
+        sp.kc = q->kc;
+        sp.k1dir[0] = q->k1dir[0];
+        sp.k1dir[1] = q->k1dir[1];
+        sp.k1dir[2] = q->k1dir[2];
+    }
     return sp;
 }
 
@@ -426,9 +452,9 @@ __device__ __forceinline__ void thermal_field(const FusedArgs& a, uint32_t i, ui
     const uint32_t n = a.n;
     if (a.noise == 1) {
         if (a.th_in) {
-            th[0] = a.th_in[i];
-            th[1] = a.th_in[n + i];
-            th[2] = a.th_in[2 * (size_t)n + i];
+            th[0] = gview(a.th_in)[i];
+            th[1] = gview(a.th_in)[n + i];
+            th[2] = gview(a.th_in)[2 * (size_t)n + i];
         } else {
             float z[3];
             normals3_f32(a.seed, a.stream, iref, t, z);
@@ -437,16 +463,16 @@ __device__ __forceinline__ void thermal_field(const FusedArgs& a, uint32_t i, ui
             th[1] = z[1] * TP;
             th[2] = z[2] * TP;
             if (a.th_out) {
-                a.th_out[i] = th[0];
-                a.th_out[n + i] = th[1];
-                a.th_out[2 * (size_t)n + i] = th[2];
+                gview(a.th_out)[i] = th[0];
+                gview(a.th_out)[n + i] = th[1];
+                gview(a.th_out)[2 * (size_t)n + i] = th[2];
             }
         }
     } else {
         const float TP = a.sqrtT * sp.sigma;
-        th[0] = (float)a.xi[i] * TP;
-        th[1] = (float)a.xi[n + i] * TP;
-        th[2] = (float)a.xi[2 * (size_t)n + i] * TP;
+        th[0] = (float)gview(a.xi)[i] * TP;
+        th[1] = (float)gview(a.xi)[n + i] * TP;
+        th[2] = (float)gview(a.xi)[2 * (size_t)n + i] * TP;
     }
 }
 
@@ -468,18 +494,18 @@ __device__ __forceinline__ void fused_tail(const FusedArgs& a, uint32_t i, uint3
     }
     if (a.dem) {
         if constexpr (!EDEM) {
-            hd[0] = a.dem[i];
-            hd[1] = a.dem[n + i];
-            hd[2] = a.dem[2 * (size_t)n + i];
+            hd[0] = gview(a.dem)[i];
+            hd[1] = gview(a.dem)[n + i];
+            hd[2] = gview(a.dem)[2 * (size_t)n + i];
         }
         h[0] += hd[0];
         h[1] += hd[1];
         h[2] += hd[2];
     }
     if (a.stg) {
-        h[0] += a.stg[i];
-        h[1] += a.stg[n + i];
-        h[2] += a.stg[2 * (size_t)n + i];
+        h[0] += gview(a.stg)[i];
+        h[1] += gview(a.stg)[n + i];
+        h[2] += gview(a.stg)[2 * (size_t)n + i];
     }
     const Species<float> sp = species_of(a, i);
     if (a.noise) {
@@ -507,20 +533,20 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
                                           uint64_t t) {
     const uint32_t n = a.n;
     if constexpr (G3) {
-        const Spin3 v = reinterpret_cast<const Spin3*>(gsrc)[i];
-        s[0] = v.x;
-        s[1] = v.y;
-        s[2] = v.z;
+        const auto* v = gview(reinterpret_cast<const Spin3*>(gsrc)) + i;
+        s[0] = v->x;
+        s[1] = v->y;
+        s[2] = v->z;
     } else {
-        s[0] = src[i];
-        s[1] = src[n + i];
-        s[2] = src[2 * (size_t)n + i];
+        s[0] = gview(src)[i];
+        s[1] = gview(src)[n + i];
+        s[2] = gview(src)[2 * (size_t)n + i];
     }
     float hd[3] = {0.f, 0.f, 0.f};
     if (EDEM && a.dem) {
-        hd[0] = a.dem[i];
-        hd[1] = a.dem[n + i];
-        hd[2] = a.dem[2 * (size_t)n + i];
+        hd[0] = gview(a.dem)[i];
+        hd[1] = gview(a.dem)[n + i];
+        hd[2] = gview(a.dem)[2 * (size_t)n + i];
     }
     float h[3];
     if constexpr (FMT == FMT_SELL_PACK)
@@ -544,7 +570,7 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
     if constexpr (FMT_ == FMT_SELL_PACK || is_line_fmt(FMT_)) {                   \
         if (FMT_ == FMT_SELL_PACK || (a_).ex.lfall) {                             \
             const uint32_t m_ = (a_).ex.ndict * NC_;                              \
-            for (uint32_t k_ = threadIdx.x; k_ < m_; k_ += blockDim.x) sdict[k_] = (a_).ex.dict[k_]; \
+            for (uint32_t k_ = threadIdx.x; k_ < m_; k_ += blockDim.x) sdict[k_] = gview((a_).ex.dict)[k_]; \
             __syncthreads();                                                      \
         }                                                                         \
     }
@@ -852,24 +878,34 @@ __global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restric
         for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
         const float r = rsqrtf(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) bm.A[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
+        for (int c = 0; c < 3; ++c) gview(bm.A)[(size_t)c * n + i] = s[c] + a.half_rdt * f[c];
         if (!G3 || bm.ysoa) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) bm.Y[(size_t)c * n + i] = y[c] * r;
+            for (int c = 0; c < 3; ++c) gview(bm.Y)[(size_t)c * n + i] = y[c] * r;
         }
-        if constexpr (G3) reinterpret_cast<Spin3*>(bm.Y3)[i] = Spin3{y[0] * r, y[1] * r, y[2] * r};
+        if constexpr (G3) {
+            auto* o = gview(reinterpret_cast<Spin3*>(bm.Y3)) + i;
+            o->x = y[0] * r;
+            o->y = y[1] * r;
+            o->z = y[2] * r;
+        }
     } else {
         float av[3];  // issued ahead of the field evaluation
 #pragma unroll
-        for (int c = 0; c < 3; ++c) av[c] = bm.A[(size_t)c * n + i];
+        for (int c = 0; c < 3; ++c) av[c] = gview(bm.A)[(size_t)c * n + i];
         fused_rhs<FMT, NC, 2, G3, true, true>(a, i, a.perm.to_ref(i), bm.Y, G3 ? bm.Y3 : bm.Y, sdict, s, f, t);
         float p[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[c] = av[c] + a.half_rdt * f[c];
         const float r = rsqrtf(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) bm.S[(size_t)c * n + i] = p[c] * r;
-        if constexpr (G3) reinterpret_cast<Spin3*>(bm.S3)[i] = Spin3{p[0] * r, p[1] * r, p[2] * r};
+        for (int c = 0; c < 3; ++c) gview(bm.S)[(size_t)c * n + i] = p[c] * r;
+        if constexpr (G3) {
+            auto* o = gview(reinterpret_cast<Spin3*>(bm.S3)) + i;
+            o->x = p[0] * r;
+            o->y = p[1] * r;
+            o->z = p[2] * r;
+        }
     }
 }
 
