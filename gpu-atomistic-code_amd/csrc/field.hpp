@@ -10,6 +10,22 @@
 
 namespace sllg {
 
+// A global-memory view of p.  Pointers the kernels read from memory (a batch member's
+// buffers and coupling arrays, read from the member table) are generic to the compiler, and
+// loads and stores through generic pointers become flat instructions: VGPR addresses only,
+// no scalar loads for uniform data, and counted against the LDS wait counter as well.  All
+// these buffers are device memory.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gview(const T* p) {
+    return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gview(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 // dot += J s for one neighbour: isotropic (NC 1), diagonal (NC 3), full tensor (NC 9,
 // val[co][ro] * s[ro] for ro = 0,1,2 in turn as in src/cufields.cu:233-238).
 template <int NC, class R>
@@ -68,11 +84,13 @@ __device__ __forceinline__ void fourspin_row(const Exch<V>& e, uint32_t i, const
     const uint32_t n = e.n;
     h[0] = h[1] = h[2] = R(0);
     const R jq = (R)e.fs_jq;
-    for (uint32_t q = e.fs_xadj[i]; q < e.fs_xadj[i + 1]; ++q) {
-        const uint4 t = e.fs_jkl[q];
-        const R sj[3] = {(R)S[t.x], (R)S[n + t.x], (R)S[2 * (size_t)n + t.x]};
-        const R sk[3] = {(R)S[t.y], (R)S[n + t.y], (R)S[2 * (size_t)n + t.y]};
-        const R sl[3] = {(R)S[t.z], (R)S[n + t.z], (R)S[2 * (size_t)n + t.z]};
+    const auto* __restrict__ Sg = gview(S);
+    const auto* __restrict__ xq = gview(e.fs_xadj);
+    for (uint32_t q = xq[i]; q < xq[i + 1]; ++q) {
+        const u32x4 t = gview(reinterpret_cast<const u32x4*>(e.fs_jkl))[q];
+        const R sj[3] = {(R)Sg[t.x], (R)Sg[n + t.x], (R)Sg[2 * (size_t)n + t.x]};
+        const R sk[3] = {(R)Sg[t.y], (R)Sg[n + t.y], (R)Sg[2 * (size_t)n + t.y]};
+        const R sl[3] = {(R)Sg[t.z], (R)Sg[n + t.z], (R)Sg[2 * (size_t)n + t.z]};
         const R skdotsl = sk[0] * sl[0] + sk[1] * sl[1] + sk[2] * sl[2];
         const R sjdotsl = sj[0] * sl[0] + sj[1] * sl[1] + sj[2] * sl[2];
         const R skdotsj = sk[0] * sj[0] + sk[1] * sj[1] + sk[2] * sj[2];
@@ -90,45 +108,49 @@ __device__ __forceinline__ void exch_row_ref(const Exch<V>& e, uint32_t i, const
                                              R dot[3]) {
     const uint32_t n = e.n;
     dot[0] = dot[1] = dot[2] = R(0);
+    const auto* __restrict__ Sg = gview(S);
+    const auto* __restrict__ vals = gview(e.vals);
     R v[NC];
     if (e.fmt == FMT_CSR) {
         // CSR rows stored by 64-row slices, entry q of lane l at [q][l] (encode.hip): the
         // row's entries in CSR order, its own count from the row pointers
-        const uint32_t rl = e.xadj[i + 1] - e.xadj[i];
-        const size_t o0 = (size_t)e.slice[i >> 6].x * 64u + (i & 63u);
+        const auto* __restrict__ xadj = gview(e.xadj);
+        const uint32_t rl = xadj[i + 1] - xadj[i];
+        const size_t o0 = (size_t)gview(reinterpret_cast<const u32x2*>(e.slice))[i >> 6].x * 64u + (i & 63u);
         for (uint32_t q = 0; q < rl; ++q) {
             const size_t o = o0 + (size_t)q * 64u;
-            const uint32_t j = e.ent[o];
+            const uint32_t j = gview(e.ent)[o];
 #pragma unroll
-            for (int k = 0; k < NC; ++k) v[k] = (R)e.vals[(size_t)k * e.nent + o];
-            const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
+            for (int k = 0; k < NC; ++k) v[k] = (R)vals[(size_t)k * e.nent + o];
+            const R s[3] = {(R)Sg[j], (R)Sg[n + j], (R)Sg[2 * (size_t)n + j]};
             accum<NC, R>(v, s, dot);
         }
     } else if (e.fmt == FMT_DIA) {
         for (uint32_t q = 0; q < e.ndiag; ++q) {
-            const int64_t col = (int64_t)i + e.doff[q];
+            const int64_t col = (int64_t)i + gview(e.doff)[q];
             if (col >= 0 && col < (int64_t)n) {
 #pragma unroll
-                for (int k = 0; k < NC; ++k) v[k] = (R)e.vals[((size_t)k * e.ndiag + q) * n + i];
-                const R s[3] = {(R)S[col], (R)S[n + col], (R)S[2 * (size_t)n + col]};
+                for (int k = 0; k < NC; ++k) v[k] = (R)vals[((size_t)k * e.ndiag + q) * n + i];
+                const R s[3] = {(R)Sg[col], (R)Sg[n + col], (R)Sg[2 * (size_t)n + col]};
                 accum<NC, R>(v, s, dot);
             }
         }
     } else if (e.fmt == FMT_STENCIL) {
         const uint32_t t = i % e.na;
-        const uint32_t cnt = e.scount[t];
+        const uint32_t cnt = gview(e.scount)[t];
         for (uint32_t q = 0; q < cnt; ++q) {
-            const int4 en = e.sent[t * e.sk + q];
+            const auto* en4 = gview(e.sent) + (t * e.sk + q);
+            const int4 en = make_int4(en4->x, en4->y, en4->z, en4->w);
             uint32_t j;
             if (!stencil_neighbour(e, i, en, j)) continue;
             if (e.per_row) {
 #pragma unroll
-                for (int k = 0; k < NC; ++k) v[k] = (R)e.vals[((size_t)k * e.sk + q) * n + i];
+                for (int k = 0; k < NC; ++k) v[k] = (R)vals[((size_t)k * e.sk + q) * n + i];
             } else {
 #pragma unroll
-                for (int k = 0; k < NC; ++k) v[k] = (R)e.vals[((size_t)k * e.na + t) * e.sk + q];
+                for (int k = 0; k < NC; ++k) v[k] = (R)vals[((size_t)k * e.na + t) * e.sk + q];
             }
-            const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
+            const R s[3] = {(R)Sg[j], (R)Sg[n + j], (R)Sg[2 * (size_t)n + j]};
             accum<NC, R>(v, s, dot);
         }
     }

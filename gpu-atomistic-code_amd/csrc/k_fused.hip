@@ -28,29 +28,30 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
     const uint32_t n = e.n;
     dot[0] = dot[1] = dot[2] = R(0);
     const uint32_t sl = i >> 6, lane = i & 63u;
-    const uint2 sd = e.slice[sl];
+    const u32x2 sd = gview(reinterpret_cast<const u32x2*>(e.slice))[sl];
     const uint32_t q0 = sd.x, len = sd.y;
+    const auto* __restrict__ Sg = gview(S);
     R v[NC];
     if (e.fmt == FMT_SELL_PACK) {
-        const uint32_t* __restrict__ ent = e.ent + (size_t)q0 * kSlice;
+        const auto* __restrict__ ent = gview(e.ent) + (size_t)q0 * kSlice;
         for (uint32_t q = 0; q < len; ++q) {
             const uint32_t w = ent[pack_pos(q, lane, len)];
             const uint32_t j = pack_nb(w, i, e.dbits);
-            const V* dv = e.dict + (size_t)pack_dict(w, e.dbits) * NC;
+            const auto* dv = gview(e.dict) + (size_t)pack_dict(w, e.dbits) * NC;
 #pragma unroll
             for (int k = 0; k < NC; ++k) v[k] = (R)dv[k];
-            const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
+            const R s[3] = {(R)Sg[j], (R)Sg[n + j], (R)Sg[2 * (size_t)n + j]};
             accum<NC, R>(v, s, dot);
         }
     } else {
-        const uint32_t* __restrict__ ent = e.ent + (size_t)q0 * kSlice + lane;
-        const V* __restrict__ val = e.dict + (size_t)q0 * kSlice + lane;
+        const auto* __restrict__ ent = gview(e.ent) + (size_t)q0 * kSlice + lane;
+        const auto* __restrict__ val = gview(e.dict) + (size_t)q0 * kSlice + lane;
 #pragma unroll 4
         for (uint32_t q = 0; q < len; ++q) {
             const uint32_t j = ent[(size_t)q * kSlice];
 #pragma unroll
             for (int k = 0; k < NC; ++k) v[k] = (R)val[(size_t)k * e.nent + (size_t)q * kSlice];
-            const R s[3] = {(R)S[j], (R)S[n + j], (R)S[2 * (size_t)n + j]};
+            const R s[3] = {(R)Sg[j], (R)Sg[n + j], (R)Sg[2 * (size_t)n + j]};
             accum<NC, R>(v, s, dot);
         }
     }
@@ -62,22 +63,6 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
 struct Spin3 {  // one spin of an AoS3 copy
     float x, y, z;
 };
-
-// A global-memory view of p.  Pointers the kernels read from memory (a batch member's
-// buffers and coupling arrays, read from the member table) are generic to the compiler, and
-// loads and stores through generic pointers become flat instructions: VGPR addresses only,
-// no scalar loads for uniform data, and counted against the LDS wait counter as well.  All
-// these buffers are device memory.
-template <class T>
-__device__ __forceinline__ const __attribute__((address_space(1))) T* gview(const T* p) {
-    return (const __attribute__((address_space(1))) T*)p;
-}
-template <class T>
-__device__ __forceinline__ __attribute__((address_space(1))) T* gview(T* p) {
-    return (__attribute__((address_space(1))) T*)p;
-}
-typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 struct SpinBuf {
     __amdgpu_buffer_rsrc_t r;
