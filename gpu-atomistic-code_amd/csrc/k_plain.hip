@@ -60,11 +60,12 @@ __global__ __launch_bounds__(kTPB) void k_plain_noise(uint32_t n, uint64_t seed,
     xi[2 * (size_t)n + i] = z[2];
 }
 
-// Field at spin i (own spin s, exchange+demag hex): h = ((hex + stg) + th) + B, then local terms.
+// Field at spin i (own spin s, exchange+demag hex): h = ((hex + stg) + th) + B, then local terms
+// (th = 0 when noise is false: the deterministic field).
 __device__ __forceinline__ void plain_field(const PlainArgs& a, uint32_t i, const double s[3], const double hex[3],
-                                            double th[3], double h[3]) {
+                                            double th[3], double h[3], bool noise = true) {
     const uint32_t n = a.n;
-    if (a.xi) {
+    if (noise && a.xi) {
         const double TP = sqrt(a.T) * a.sigma[i];
         for (int c = 0; c < 3; ++c) th[c] = a.xi[(size_t)c * n + i] * TP;
     } else {
@@ -207,9 +208,7 @@ __global__ __launch_bounds__(kTPB) void k_plain_detfield(PlainArgs a, const doub
     const double s[3] = {Sp[i], Sp[(size_t)n + i], Sp[2 * (size_t)n + i]};
     const double hex[3] = {H[i], H[(size_t)n + i], H[2 * (size_t)n + i]};
     double th[3], h[3];
-    PlainArgs b = a;
-    b.xi = nullptr;
-    plain_field(b, i, s, hex, th, h);
+    plain_field(a, i, s, hex, th, h, false);
     for (int c = 0; c < 3; ++c) out[(size_t)c * n + i] = h[c];
 }
 
