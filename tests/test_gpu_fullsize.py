@@ -39,6 +39,33 @@ def test_fcc_1m_fp32_tracks_fp64_plain(gpu):
     assert d <= 1e-4 and dm <= 1e-5
 
 
+def test_fcc_1m_tile_reader_equals_line_reader(gpu):
+    """The headline lattice's reader choices at full size: the LDS-staged line reader (tile
+    streams, 8 slices per workgroup) gives the line reader's bits over 60 thermal steps at
+    300 K, with and without the thermal hand-off and across a set_spins; both stay within FP32
+    rounding of the packed entries."""
+    sf, s0 = _fcc("fp32")
+    g = sllg.Context(sf)
+    enc = g.info()["exch_encoding"]
+    assert "line64" in enc and "-tile8x" in enc, enc
+    res = {}
+    for name, launch in (("line", (256, 1, 2, 0, 1, 1)), ("tile", (512, 1, 2, 0, 1, 2)),
+                         ("tile_h", (512, 0, 2, 1, 1, 2)), ("packed", (256, 1, 2, 0, 1, 0))):
+        g.set_launch(*launch)
+        g.set_spins(s0)
+        g.set_temperature(300.0)
+        g.step(0, 30)
+        g.set_spins(g.get_spins()[:, ::-1].copy())
+        g.step(30, 30)
+        res[name] = g.get_spins()
+    g.close()
+    assert np.array_equal(res["tile"], res["line"])
+    assert np.array_equal(res["tile_h"], res["line"])
+    d = float(np.abs(res["tile"] - res["packed"]).max())
+    print("1M fcc tile vs packed, 60 steps at 300 K: max|dS|", d)
+    assert d <= 2e-5
+
+
 @pytest.mark.timeout(900)
 def test_fcc_1m_fp32_ordered_start_10k_steps(gpu):
     """The accuracy target of the FP32 form at full size: 10^4 Heun steps at T = 0 of the 1M-spin
