@@ -559,6 +559,54 @@ def test_tile_reader(gpu, case):
         assert _maxdiff(g.get_spins(), o.get_spins()) <= 1e-4
 
 
+def _basis_system(case):
+    """CSR systems of other bases for the tile-stream shapes: bcc (two sublattices, two
+    shells), a three-atom basis on the cube diagonal (nearest neighbours along it), and fcc
+    with three lines per sublattice."""
+    if case == "bcc":
+        uc, dims, J = sllg.unit_cell("bcc"), (2, 2, 64), [4e-21, 1e-21]
+    elif case == "three_atoms":
+        xred = np.array([[0, 0, 0], [1 / 3, 1 / 3, 1 / 3], [2 / 3, 2 / 3, 2 / 3]], float)
+        uc = sllg.UnitCellSpec(xred, np.eye(3), (1, 1, 1), np.zeros((3, 3), np.uint32))
+        dims, J = (2, 4, 64), [4e-21, 1e-21]
+    else:
+        uc, dims, J = sllg.unit_cell("fcc"), (3, 1, 64), [5e-21]
+    na = len(uc.xred)
+    ints, _ = sllg.shells(uc, J)
+    xadj, adj, vals = sllg.build_csr(uc, dims, ints, [2.22] * na)
+    n = int(np.prod(dims)) * na
+    sysd = sllg.System(n=n, species=[dict(mu=2.22, lambda_=0.1)], anis=dict(kc=2e-24),
+                       exch=dict(format=sllg.EXCH_CSR, xadj=xadj, adjncy=adj, vals=vals, n_atoms=na),
+                       dt=1e-16, precision="fp32")
+    return sysd, sllg.random_spins(n, 5, 105)
+
+
+@pytest.mark.parametrize("case", ["bcc", "three_atoms", "fcc_3_lines"])
+def test_tile_reader_shapes(gpu, case):
+    """Tile streams where the workgroup shape fits (bcc: 8 slices = 4 lines of each of the 2
+    sublattices) and none where it does not (3 sublattices: 8 and 16 slices are no whole number
+    of lines per sublattice; fcc with 3 lines per sublattice and 12 slices: no workgroup of 8
+    or 16 divides them); line_mode 2 then runs the line reader.  Either way the bits are the
+    line reader's, and the trajectory matches the oracle to FP32 rounding."""
+    sysd, s0 = _basis_system(case)
+    g = sllg.Context(sysd)
+    enc = g.info()["exch_encoding"]
+    print(case, enc)
+    assert "line64" in enc
+    assert ("-tile8x" in enc) == (case == "bcc")
+    res = []
+    for line_mode in (1, 2):
+        g.set_launch(512, 1, 2, 0, 1, line_mode)
+        g.set_spins(s0)
+        g.set_temperature(200.0)
+        g.step(0, 40)
+        res.append(g.get_spins())
+    g.close()
+    assert np.array_equal(res[0], res[1])
+    g, o = run_pair(sysd, s0, 40, T=0.0, B=(0.0, 0.1, 0.3), launch=(512, 1, 2, 0, 1, 2))
+    assert _maxdiff(g.get_spins(), o.get_spins()) <= 1e-4
+
+
 def _fftx_system(prec, dip, seed=7, dims=(6, 5, 4)):
     """Open sc lattice, two species on a checkerboard, random anisotropic/antisymmetric 3x3
     couplings on the 6 nearest and 12 next-nearest mesh vectors; the same couplings as an
