@@ -39,6 +39,9 @@ constexpr int line_words(int nc) { return (1 + 2 * line_group(nc) + 3 * line_gro
 constexpr uint32_t kTileLineBytes = 3 * 64 * 4;
 constexpr uint32_t kTileLdsMax = 64 * 1024;  // dynamic LDS of one workgroup
 constexpr int tile_head(int nw) { return (2 + 3 * nw + 7) / 8 * 8; }  // start of the staged-line list
+// Tile records: line_group(nc) windows (their LDS byte offsets, the shift mask in the low 3
+// bits of the first) and their couplings, tile_words(nc) words (12 for isotropic couplings).
+constexpr int tile_words(int nc) { return (line_group(nc) + 3 * line_group(nc) * nc + 3) / 4 * 4; }
 
 enum ExchFmt : int {
     FMT_NONE = 0,

@@ -209,6 +209,36 @@ struct TupleHash {
     }
 };
 
+// One neighbour window of a line-mode slice: its first row relative to the slice's, the
+// shifts it has couplings for (bit 0/1/2: dz = 0/+1/-1) and the couplings [shift][component].
+struct Line {
+    int32_t lrel;
+    uint32_t mask;
+    double v[3][9];
+};
+
+// Windows in groups of G: full groups of one shift mask first (masks 7 down to 1), then the
+// remainders together under the union of their masks; -1 marks an unused slot.
+std::vector<std::vector<int>> group_windows(const std::vector<Line>& lines, int G) {
+    std::vector<std::vector<int>> grp;
+    std::vector<int> rest;
+    for (uint32_t m = 7; m >= 1; --m) {
+        std::vector<int> cls;
+        for (size_t l = 0; l < lines.size(); ++l)
+            if (lines[l].mask == m) cls.push_back((int)l);
+        size_t k = 0;
+        for (; k + (size_t)G <= cls.size(); k += (size_t)G)
+            grp.emplace_back(cls.begin() + (ptrdiff_t)k, cls.begin() + (ptrdiff_t)k + G);
+        rest.insert(rest.end(), cls.begin() + (ptrdiff_t)k, cls.end());
+    }
+    for (size_t k = 0; k < rest.size(); k += (size_t)G) {
+        std::vector<int> g(rest.begin() + (ptrdiff_t)k, rest.begin() + (ptrdiff_t)std::min(k + G, rest.size()));
+        g.resize((size_t)G, -1);
+        grp.push_back(g);
+    }
+    return grp;
+}
+
 // Tile streams of the LDS-staged line reader, made from the line records of a 64-cell line
 // encoding in which every slice is a line.  A workgroup takes nw slices: nw / na
 // consecutive lines of each of the na sublattice blocks (slice q = w nw + k of workgroup w
@@ -220,22 +250,33 @@ struct TupleHash {
 //   [2 + 2 nw + k]      LDS byte offset of slice k's own line
 //   [H, H + nU)         first rows of the staged lines relative to the workgroup's first row
 //                       (line q0 / na of sublattice 0, q0 = w nw), int32; H = tile_head(nw)
-//   then slice k's line records, every window start (current and next group) replaced by
-//   the LDS byte offset of its line, kTileLineBytes per line in staging order.
+//   then slice k's records, tile_words(nc) words each: the LDS byte offsets of the record's
+//   line_group(nc) windows (kTileLineBytes per line in staging order, so multiples of 8; the
+//   first carries the record's shift mask in its low 3 bits), then the couplings
+//   [window][dz 0, +1, -1][component] (FP32, zero where a window has none).  The windows are
+//   grouped exactly as in the line records (group_windows, line_group(nc) to a group), so
+//   both readers form the same sums.
 // Streams of equal content are stored once (interior workgroups of a periodic lattice
-// repeat); W zero words follow the last stream, so a reader may fetch one record past a
-// slice's last.  Chooses nw = 8, else 16: the first that divides the slices evenly
+// repeat); tile_words(nc) zero words follow the last stream, so a reader may fetch one
+// record past a slice's last.  kept: the windows of each stored line-record list, by its
+// first word.  Chooses nw = 8, else 16: the first that divides the slices evenly
 // and stages at most kTileLdsMax bytes.  On failure the encoding keeps no tile streams.
-void build_tiles(llg_ctx* c, const std::vector<uint32_t>& words, const std::vector<uint2>& table, int G, int W) {
+void build_tiles(llg_ctx* c, const std::vector<uint2>& table,
+                 const std::unordered_map<uint32_t, std::vector<Line>>& kept) {
     ExchStore& X = *c->xs;
     Exch<float>& x = X.ex32;
     const uint32_t n = c->n, nsl = n / kSlice, na = X.perm.na;
+    const int nc = X.h.ncomp, G = line_group(nc), TW = tile_words(nc);
     if (na > 1 && X.perm.ncell % kSlice) return;
     const uint32_t lps = na > 1 ? X.perm.ncell / kSlice : nsl;
+    // the groups of each stored window list, as the line records have them
+    std::unordered_map<uint32_t, std::vector<std::vector<int>>> groups;
+    for (const auto& kv : kept) groups.emplace(kv.first, group_windows(kv.second, G));
+    const std::vector<std::vector<int>> nogroups;
     for (uint32_t nw : {8u, 16u}) {
         if (nw % na || nsl % nw || lps % (nw / na)) continue;
         const uint32_t kk = nw / na, nwg = nsl / nw, H = (uint32_t)tile_head((int)nw);
-        std::vector<uint32_t> tw, tg(nwg), st;
+        std::vector<uint32_t> tw, st;
         std::unordered_map<uint64_t, std::vector<uint32_t>> seen;  // stream hash -> workgroups
         std::vector<uint32_t> offs(nwg);
         uint32_t umax = 0;
@@ -260,16 +301,29 @@ void build_tiles(llg_ctx* c, const std::vector<uint32_t>& words, const std::vect
                 const uint32_t q = w * nw + k, s = (q % na) * lps + q / na;
                 const int64_t base = (int64_t)s * kSlice;
                 const uint2 t = table[s];
-                st[2 + nw + k] = t.y;
+                const auto gi = t.y ? groups.find(t.x) : groups.end();
+                const std::vector<std::vector<int>>& grp = gi != groups.end() ? gi->second : nogroups;
+                st[2 + nw + k] = (uint32_t)grp.size();
                 st[2 + 2 * nw + k] = slot_of(base) * kTileLineBytes;
-                st[2 + k] = (uint32_t)recs.size();  // made absolute below
-                for (uint32_t g = 0; g < t.y; ++g) {
-                    const uint32_t* r = words.data() + t.x + (size_t)g * W;
-                    const size_t o = recs.size();
-                    recs.insert(recs.end(), r, r + W);
-                    for (int u = 0; u < G; ++u) {
-                        recs[o + 1 + u] = slot_of(base + (int32_t)r[1 + u]) * kTileLineBytes;
-                        recs[o + W - G + u] = slot_of(base + (int32_t)r[W - G + u]) * kTileLineBytes;
+                st[2 + k] = (uint32_t)recs.size();  // made relative to the stream below
+                if (!grp.empty()) {
+                    const std::vector<Line>& lines = kept.at(t.x);
+                    for (const std::vector<int>& g : grp) {
+                        const size_t o = recs.size();
+                        recs.resize(o + (size_t)TW, 0u);
+                        uint32_t mask = 0;
+                        for (int u = 0; u < G; ++u) {
+                            const Line& ln = lines[(size_t)(g[(size_t)u] >= 0 ? g[(size_t)u] : g[0])];
+                            recs[o + (size_t)u] = slot_of(base + ln.lrel) * kTileLineBytes;
+                            if (g[(size_t)u] < 0) continue;
+                            mask |= ln.mask;
+                            for (int kz = 0; kz < 3; ++kz)
+                                for (int a = 0; a < nc; ++a) {
+                                    const float v = (float)ln.v[kz][a];
+                                    std::memcpy(&recs[o + (size_t)(G + (u * 3 + kz) * nc + a)], &v, 4);
+                                }
+                        }
+                        recs[o] |= mask;
                     }
                 }
             }
@@ -306,7 +360,7 @@ void build_tiles(llg_ctx* c, const std::vector<uint32_t>& words, const std::vect
             }
         }
         if (!ok) continue;
-        tw.insert(tw.end(), (size_t)W, 0u);
+        tw.insert(tw.end(), (size_t)TW, 0u);
         if (upload(X.f, tw.data(), tw.size(), c->stream)) return;
         if (upload(X.g, offs.data(), offs.size(), c->stream)) return;
         x.tent = X.f.as<uint32_t>();
@@ -359,16 +413,12 @@ bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vec
     std::vector<uint32_t> words((size_t)W, 0u);  // the zero record
     std::vector<uint2> table(nsl);
     std::unordered_map<uint64_t, std::vector<uint32_t>> seen;  // record-list hash -> stored slices
+    std::unordered_map<uint32_t, std::vector<Line>> kept;     // windows of each stored record list
     std::vector<uint32_t> st;
     stored = 0;
     nfall = 0;
     uint64_t lines_total = 0, lined = 0;
     const int32_t lm = (int32_t)len - 1;
-    struct Line {
-        int32_t lrel;
-        uint32_t mask;
-        double v[3][9];
-    };
     // key of one coupling of row d0 + l: window start relative to d0, shift dz, tuple
     using LKey = std::tuple<int64_t, int, uint32_t>;
     // keys of row d0 + l in CSR order (csr, optional) and sorted (out, for comparing lanes)
@@ -444,25 +494,7 @@ bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vec
         }
         ++lined;
         st.clear();
-        // groups of G windows: full groups of one shift mask, then the remainders together
-        std::vector<std::vector<int>> grp;  // window indices (-1: unused slot)
-        {
-            std::vector<int> rest;
-            for (uint32_t m = 7; m >= 1; --m) {
-                std::vector<int> cls;
-                for (size_t l = 0; l < lines.size(); ++l)
-                    if (lines[l].mask == m) cls.push_back((int)l);
-                size_t k = 0;
-                for (; k + (size_t)G <= cls.size(); k += (size_t)G)
-                    grp.emplace_back(cls.begin() + (ptrdiff_t)k, cls.begin() + (ptrdiff_t)k + G);
-                rest.insert(rest.end(), cls.begin() + (ptrdiff_t)k, cls.end());
-            }
-            for (size_t k = 0; k < rest.size(); k += (size_t)G) {
-                std::vector<int> g(rest.begin() + (ptrdiff_t)k, rest.begin() + (ptrdiff_t)std::min(k + G, rest.size()));
-                g.resize((size_t)G, -1);
-                grp.push_back(g);
-            }
-        }
+        const std::vector<std::vector<int>> grp = group_windows(lines, G);
         const size_t ng = grp.size();
         auto start_of = [&](const std::vector<int>& g, int u) -> uint32_t {
             return (uint32_t)lines[(size_t)(g[(size_t)u] >= 0 ? g[(size_t)u] : g[0])].lrel;
@@ -504,6 +536,7 @@ bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vec
             cand.push_back(s);
             table[s] = make_uint2((uint32_t)words.size(), nrec);
             words.insert(words.end(), st.begin(), st.end());
+            kept.emplace(table[s].x, lines);
             ++stored;
         }
     }
@@ -513,7 +546,7 @@ bool build_lines(llg_ctx* c, const std::vector<uint32_t>& ref_of, const std::vec
     if (upload(X.e, table.data(), table.size(), c->stream)) return false;
     Exch<float>& x = X.ex32;
     x.tnw = x.tumax = x.tlps = 0;
-    if (len == 64 && nfall == 0) build_tiles(c, words, table, G, W);
+    if (len == 64 && nfall == 0) build_tiles(c, table, kept);
     x.fmt = len == 64 ? (open ? FMT_SELL_LINE_OPEN : FMT_SELL_LINE) : (open ? FMT_SELL_LINE16_OPEN : FMT_SELL_LINE16);
     x.lslice = X.e.as<uint2>();
     x.lent = X.d.as<uint32_t>();

@@ -177,14 +177,24 @@ template <int NC>
 struct LineRecW {
     static constexpr int G = line_group(NC), W = line_words(NC), NX = W - G, CO = 1 + G;
     uint32_t w[W];
+    __device__ uint32_t mask() const { return w[0]; }
+};
+// Record of the tile reader: w[0, G) LDS byte offsets of the group's windows (the shift mask
+// in the low 3 bits of w[0]), w[G, G + 3 G NC) couplings [window][shift][component].
+template <int NC>
+struct TileRecW {
+    static constexpr int G = line_group(NC), W = tile_words(NC), CO = G;
+    uint32_t w[W];
+    __device__ uint32_t mask() const { return w[0] & 7u; }
+    __device__ uint32_t window(int u) const { return u == 0 ? w[0] & ~7u : w[u]; }
 };
 // Whole 16-byte units, so the record arrives in few wide scalar loads.
-template <int NC>
-__device__ __forceinline__ LineRecW<NC> load_line_rec(const uint32_t* __restrict__ p) {
+template <class RW>
+__device__ __forceinline__ RW load_rec(const uint32_t* __restrict__ p) {
     const auto* __restrict__ q = gview(reinterpret_cast<const u32x4*>(p));
-    LineRecW<NC> o;
+    RW o;
 #pragma unroll
-    for (int k = 0; k < LineRecW<NC>::W / 4; ++k) {
+    for (int k = 0; k < RW::W / 4; ++k) {
         const u32x4 v = q[k];
         o.w[4 * k] = v.x;
         o.w[4 * k + 1] = v.y;
@@ -239,9 +249,8 @@ __device__ __forceinline__ float line_prev(float v) {
 
 // Shift K of one group: every window's couplings for that shift against its unshifted
 // spins, into the partial sum of shift K.
-template <int NC, int K>
-__device__ __forceinline__ void line_shift_sum(const LineRecW<NC>& r, const u32x3* x, float p[3]) {
-    using RW = LineRecW<NC>;
+template <int NC, int K, class RW>
+__device__ __forceinline__ void line_shift_sum(const RW& r, const u32x3* x, float p[3]) {
 #pragma unroll
     for (int u = 0; u < RW::G; ++u) {
         const float s[3] = {__uint_as_float(x[u].x), __uint_as_float(x[u].y), __uint_as_float(x[u].z)};
@@ -253,9 +262,9 @@ __device__ __forceinline__ void line_shift_sum(const LineRecW<NC>& r, const u32x
 }
 
 // One group's arithmetic: one uniform test per shift.
-template <int NC>
-__device__ __forceinline__ void line_group_sum(const LineRecW<NC>& r, const u32x3* x, float p[3][3]) {
-    const uint32_t m = r.w[0];
+template <int NC, class RW>
+__device__ __forceinline__ void line_group_sum(const RW& r, const u32x3* x, float p[3][3]) {
+    const uint32_t m = r.mask();
     if (m & 1u) line_shift_sum<NC, 0>(r, x, p[0]);
     if (m & 2u) line_shift_sum<NC, 1>(r, x, p[1]);
     if (m & 4u) line_shift_sum<NC, 2>(r, x, p[2]);
@@ -293,20 +302,20 @@ __device__ __forceinline__ void exch_row_line(const Exch<float>& e, uint32_t i, 
         // two window buffers used in turn (no register copies, so no early waits on the
         // loads in flight); pairs of groups run in the loop, the last one or two after it
         u32x3 xa[G], xb[G];
-        RW c0 = load_line_rec<NC>(rp);
+        RW c0 = load_rec<LineRecW<NC>>(rp);
         line_loads<NC, G3>(sb, lane, base, c0.w + 1, xa);
         uint32_t r = 0;
         for (; r + 2 < ngrp; r += 2) {
-            const RW c1 = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
+            const RW c1 = load_rec<LineRecW<NC>>(rp + (size_t)(r + 1) * W);
             line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
             line_group_sum<NC>(c0, xa, p);
-            const RW c2 = load_line_rec<NC>(rp + (size_t)(r + 2) * W);
+            const RW c2 = load_rec<LineRecW<NC>>(rp + (size_t)(r + 2) * W);
             line_loads<NC, G3>(sb, lane, base, c1.w + NX, xa);
             line_group_sum<NC>(c1, xb, p);
             c0 = c2;
         }
         if (r + 1 < ngrp) {
-            const RW c1 = load_line_rec<NC>(rp + (size_t)(r + 1) * W);
+            const RW c1 = load_rec<LineRecW<NC>>(rp + (size_t)(r + 1) * W);
             line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
             line_group_sum<NC>(c0, xa, p);
             line_group_sum<NC>(c1, xb, p);
@@ -706,43 +715,38 @@ __device__ __forceinline__ void tile_stage(const TileRow& r, const float* __rest
     }
 }
 
-// The G windows of one record from LDS (byte offsets of their lines in ws[0, G)); lds4 is
-// this lane's byte address in the first line.
+// The windows of one record from LDS; lds4 is this lane's byte address in the first line.
 template <int NC>
-__device__ __forceinline__ void tile_windows(const char* lds4, const uint32_t* ws, u32x3* x) {
+__device__ __forceinline__ void tile_group(const char* lds4, const TileRecW<NC>& c, float p[3][3]) {
+    u32x3 x[TileRecW<NC>::G];
 #pragma unroll
-    for (int u = 0; u < line_group(NC); ++u) {
-        const float* p = reinterpret_cast<const float*>(lds4 + ws[u]);
-        x[u].x = __float_as_uint(p[0]);
-        x[u].y = __float_as_uint(p[64]);
-        x[u].z = __float_as_uint(p[128]);
+    for (int u = 0; u < TileRecW<NC>::G; ++u) {
+        const float* q = reinterpret_cast<const float*>(lds4 + c.window(u));
+        x[u].x = __float_as_uint(q[0]);
+        x[u].y = __float_as_uint(q[64]);
+        x[u].z = __float_as_uint(q[128]);
     }
-}
-
-template <int NC>
-__device__ __forceinline__ void tile_group(const char* lds4, const LineRecW<NC>& c, float p[3][3]) {
-    u32x3 x[LineRecW<NC>::G];
-    tile_windows<NC>(lds4, c.w + 1, x);
     line_group_sum<NC>(c, x, p);
 }
 
 // Pair exchange of this wave's slice from LDS.  Two records per iteration in two register
-// sets, each record's scalar load issued one group ahead (the stream has one spare record
-// at its end, so the look-ahead past a slice's last record stays in bounds).
+// sets, each record's scalar load issued one record ahead (the stream has one spare record
+// at its end, so the look-ahead past a slice's last record stays in bounds).  Per shift the
+// contributions are summed at the unshifted lanes and moved once, as in the line reader.
 template <int NC, int NW, bool OPEN>
 __device__ __forceinline__ void exch_row_tile(const TileRow& r, const float* lds, float dot[3]) {
-    using RW = LineRecW<NC>;
+    using RW = TileRecW<NC>;
     constexpr int W = RW::W;
     const uint32_t* __restrict__ rp = r.ts + __builtin_amdgcn_readfirstlane(r.ts[2 + r.k]);
     const uint32_t ng = __builtin_amdgcn_readfirstlane(r.ts[2 + NW + r.k]);
     const char* lds4 = reinterpret_cast<const char*>(lds) + 4u * r.lane;
     float p[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
-    RW c0 = load_line_rec<NC>(rp);
+    RW c0 = load_rec<RW>(rp);
     uint32_t g = 0;
     for (; g + 1 < ng; g += 2) {
-        const RW c1 = load_line_rec<NC>(rp + (size_t)(g + 1) * W);
+        const RW c1 = load_rec<RW>(rp + (size_t)(g + 1) * W);
         tile_group<NC>(lds4, c0, p);
-        const RW c2 = load_line_rec<NC>(rp + (size_t)(g + 2) * W);
+        const RW c2 = load_rec<RW>(rp + (size_t)(g + 2) * W);
         tile_group<NC>(lds4, c1, p);
         c0 = c2;
     }
