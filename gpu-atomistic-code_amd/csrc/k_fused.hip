@@ -647,10 +647,10 @@ __global__ __launch_bounds__(TPB) void k_heun_corr(FusedArgs a, const float* __r
 // LDS-staged line reader of the Heun pair (64-cell line encodings in which every slice is a
 // line, AoS3 spins).  A workgroup of NW waves takes the NW slices its stream describes
 // (encode.hip, build_tiles), loads every line they read once from the AoS3 copy into LDS
-// as [line][component][64] floats, and each wave then runs its slice's line records with
-// the windows read from LDS: a line shared by the workgroup's slices leaves the cache
-// hierarchy once per workgroup instead of once per slice that reads it.  The records,
-// their grouping and the order of the sums are the line reader's, so both readers give
+// as [line][component][64] floats, and each wave then sums its slice's records with the
+// windows read from LDS: a line shared by the workgroup's slices leaves the cache hierarchy
+// once per workgroup instead of once per slice that reads it.  The records group the
+// windows as the line records do and the sums run in the same order, so both readers give
 // the same bits.
 
 struct TileRow {
