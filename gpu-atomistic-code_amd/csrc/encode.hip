@@ -354,6 +354,10 @@ void build_tiles(llg_ctx* c, const std::vector<uint2>& table,
                     break;
                 }
             if (!found) {
+                if (tw.size() + st.size() + (size_t)TW >= (1ull << 32)) {  // offsets are 32-bit
+                    ok = false;
+                    break;
+                }
                 cand.push_back(w);
                 offs[w] = (uint32_t)tw.size();
                 tw.insert(tw.end(), st.begin(), st.end());
