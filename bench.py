@@ -6,6 +6,10 @@ FP32 fused field assembly + integrator, T = 300 K counter-based thermal noise, s
 random from seed 42.  A "step" is one full Heun step (predictor + corrector) of all
 spins; value = spin-updates/s = n * K / (timed seconds) with the state resident in HBM.
 
+The launch shape (reader: LDS-staged line reader, line reader or packed entries; block
+size; XCD order; thermal hand-off; AoS3 copy) is picked by the library's autotuner before
+timing and reported in launch_tune.
+
 Also measured in the same run and carried in the same JSON line:
   roofline       the corrector's algorithmic bytes / its share of the timed step, and its HBM
                  traffic per launch from two rocprofv3 PMC child runs of this file (FETCH_SIZE,
@@ -13,8 +17,11 @@ Also measured in the same run and carried in the same JSON line:
   by_temperature the same system at T = 0, 300 and 600 K
   beyond_cache   fcc 192 x 192 x 64 x 4 = 9,437,184 spins, whose stepping state (S, Y, A: 340 MB)
                  exceeds the 256 MB Infinity Cache
+  unshared_encoding  the same system with every 64-row slice's entries stored (no translation
+                 sharing), packed reader: the HBM-bound form of the step
   fp64_plain     the FP64 plain path (reference kernel structure, CSR with 3 FP64 values per
-                 nnz as the reference stores diagonal exchange) on the 1M-spin lattice
+                 nnz as the reference stores diagonal exchange, rows stored by 64-row slices)
+                 on the 1M-spin lattice
   cpu_baseline   the FP64 oracle on one host core over a bounded sample
   serving        the serving stream: 4,096 Poisson requests at 200/s of simple-cubic M(T)
                  and field-ramp solves, L log-uniform in [10, 100], 2,000-20,000 Heun steps,
