@@ -582,7 +582,8 @@ def rk4_dmi(args):
 def batched(args):
     """64 replicas of the 128x128x16 film (NN exchange + FFT dipolar on the 256x256x32 padded
     grid) at T_r = 900 r/63 K, Heun FP32, one batched launch per half step, chunks of 100 steps
-    replayed from hipGraphs that hold the dipolar refreshes (every `update` = 100 steps)."""
+    replayed from hipGraphs that hold the dipolar refreshes (every `update` = 100 steps); then
+    the same steps as persistent chunks (one launch per 100 steps)."""
     import sllg
     from sllg import batch as SB
     from sllg import workloads as W
@@ -609,10 +610,27 @@ def batched(args):
     m = B.reduce(ids)  # synchronises
     sec = time.perf_counter() - t
     info = B.info()
+    # the same number of steps again as persistent chunks (one cooperative launch per chunk of
+    # 100 steps, grid barriers between the half steps, refreshes between the chunks)
+    B.set_persistent(True)
+    B.step(100)
+    B.reduce(ids)
+    t = time.perf_counter()
+    done = 0
+    while done < steps:
+        k = min(100, steps - done)
+        B.step(k)
+        done += k
+    B.reduce(ids)
+    sec_p = time.perf_counter() - t
+    pgrid = B.info()["persist_grid"]
     print(json.dumps({"mode": "batched", "replicas": R, "spins_total": info["spins"], "steps": steps,
                       "spin_updates_per_s": info["spins"] * steps / sec, "replica_steps_per_s": R * steps / sec,
                       "ms_per_step": sec * 1e3 / steps, "setup_s": setup, "device_bytes": info["device_bytes"],
                       "launch_tune": tune,
+                      "persistent": {"ms_per_step": sec_p * 1e3 / steps,
+                                     "spin_updates_per_s": info["spins"] * steps / sec_p, "grid": pgrid,
+                                     "tiles": info["tiles"]},
                       "m_z_lowest_T": float(m[0][2]), "m_z_highest_T": float(m[-1][2])}), flush=True)
 
 

@@ -58,6 +58,10 @@ struct llg_batch {
     int xcd = 1;         // XCD-aware block order
     int line = 1;        // line reader for members with a line encoding (else their packed entries)
     int g3 = 0;          // gather from AoS3 copies of S and Y (packed and line readers)
+    int persist = 0;     // chunks as persistent launches (grid barriers between half steps)
+    uint32_t spin_limit = LLG_BATCH_SPIN_DEFAULT;  // polls per grid-barrier wait
+    DevBuf bar;          // GridBar of the persistent launches
+    uint32_t persist_grid = 0;  // blocks of the last persistent launch
     DevBuf red;
     std::vector<std::shared_ptr<ExchStore>> pinned;  // coupling stores kept for future members
     std::map<const ExchStore*, uint64_t> store_est;  // admission estimate of each coupling store
@@ -168,6 +172,44 @@ int launch_chunk(llg_batch* b, uint32_t nsteps, bool dipolar) {
     return 0;
 }
 
+// A chunk as persistent launches: one per group and per run of steps between dipolar
+// refreshes (a refresh reads every member's spins, so it runs between launches, before the
+// predictor of its step, as in launch_chunk).  Batches with refreshes at the predicted spins
+// (every stage) take launch_chunk.
+int launch_chunk_persistent(llg_batch* b, uint32_t nsteps, bool dipolar) {
+    for (auto& m : b->m)
+        if (m.c) {
+            m.c->g3_valid = b->g3 != 0;
+            m.c->y_soa_valid = !b->g3 || batch_needs_y_soa(m.c);
+        }
+    if (!b->bar.p) {
+        SLLG_TRY(b->bar.alloc(sizeof(GridBar)));
+        SLLG_HIP(hipMemsetAsync(b->bar.p, 0, sizeof(GridBar), b->stream));
+    }
+    auto refresh_at = [&](uint32_t k) {
+        for (auto& mm : b->m)
+            if (mm.active() && mm.c->dip && (mm.t + k) % mm.c->dip_refresh == 0) return true;
+        return false;
+    };
+    for (uint32_t k0 = 0; k0 < nsteps;) {
+        if (dipolar)
+            for (auto& mm : b->m)
+                if (mm.active() && mm.c->dip && (mm.t + k0) % mm.c->dip_refresh == 0)
+                    SLLG_TRY(dipolar_refresh(mm.c, mm.c->S.p, mm.c->DEM.p));
+        uint32_t k1 = k0 + 1;
+        if (dipolar)
+            while (k1 < nsteps && !refresh_at(k1)) ++k1;
+        else
+            k1 = nsteps;
+        for (auto& g : b->groups)
+            SLLG_TRY(batch_heun_persistent(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k0, k1 - k0,
+                                           b->tpb, b->xcd, b->g3 != 0, static_cast<GridBar*>(b->bar.p),
+                                           b->spin_limit, &b->persist_grid));
+        k0 = k1;
+    }
+    return 0;
+}
+
 int check_member(llg_batch* b, int k) {
     if (!b || k < 0 || (size_t)k >= b->m.size() || !b->m[k].live) SLLG_FAIL(LLG_E_INVALID, "no such member %d", k);
     return 0;
@@ -224,6 +266,7 @@ int llg_batch_create(int device, uint64_t hbm_budget_bytes, uint32_t flags, llg_
     b->device = device;
     b->budget = hbm_budget_bytes;
     b->flags = flags;
+    b->persist = (flags & LLG_BATCH_PERSISTENT) ? 1 : 0;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete b;
@@ -349,15 +392,28 @@ int llg_batch_step(llg_batch* b, uint32_t nsteps) {
     if (nsteps == 0) return 0;
     SLLG_HIP(hipSetDevice(b->device));
     if (b->dirty) SLLG_TRY(regroup(b));
-    bool dip = false, dip_graph = true;
+    bool dip = false, dip_graph = true, dip_stage = false;
     for (auto& mm : b->m)
         if (mm.active() && mm.c->dip) {
             dip = true;
             // refreshes made of this library's own kernels only (no hipFFT plan) are captured
             dip_graph = dip_graph && dipolar_capturable(mm.c);
+            dip_stage = dip_stage || mm.c->dip_every_stage;
         }
     SLLG_TRY(upload_tables(b));
-    if ((b->flags & LLG_BATCH_GRAPH) && dip_graph) {
+    if (b->persist && !dip_stage) {
+        SLLG_TRY(launch_chunk_persistent(b, nsteps, dip));
+        GridBar hb{};
+        SLLG_HIP(hipMemcpyAsync(&hb, b->bar.p, sizeof(hb), hipMemcpyDeviceToHost, b->stream));
+        SLLG_HIP(hipStreamSynchronize(b->stream));
+        if (hb.abort) {
+            SLLG_HIP(hipMemsetAsync(b->bar.p, 0, sizeof(GridBar), b->stream));
+            SLLG_HIP(hipStreamSynchronize(b->stream));
+            SLLG_FAIL(LLG_E_DEVICE,
+                      "persistent launch abandoned: a grid barrier wait passed %u polls (the members' spins are "
+                      "partly advanced; their step counters are unchanged)", b->spin_limit);
+        }
+    } else if ((b->flags & LLG_BATCH_GRAPH) && dip_graph) {
         std::vector<uint64_t> sig = refresh_signature(b);
         if (!b->gexec || b->gsteps != nsteps || sig != b->gsig) {
             drop_graph(b);
@@ -417,6 +473,13 @@ int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd, int line_mode, int
     b->tpb = tpb;
     b->xcd = xcd ? 1 : 0;
     b->line = line_mode ? 1 : 0;
+    return 0;
+}
+
+int llg_batch_set_persistent(llg_batch* b, int on, uint32_t spin_limit) {
+    if (!b) SLLG_FAIL(LLG_E_INVALID, "NULL batch");
+    b->persist = on ? 1 : 0;
+    b->spin_limit = spin_limit;
     return 0;
 }
 
@@ -595,6 +658,7 @@ int llg_batch_get_info(llg_batch* b, llg_batch_info* info) {
     info->device_bytes = used_bytes(b);
     info->budget_bytes = b->budget;
     info->graphs_built = b->graphs_built;
+    info->persist_grid = b->persist_grid;
     return 0;
 }
 

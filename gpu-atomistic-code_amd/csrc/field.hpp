@@ -23,6 +23,17 @@ template <class T>
 __device__ __forceinline__ __attribute__((address_space(1))) T* gview(T* p) {
     return (__attribute__((address_space(1))) T*)p;
 }
+// A constant-memory view of p, for device memory that no kernel writes while it runs (slice
+// tables, member tables).  A uniform load through a global pointer becomes a scalar load only
+// when the compiler can show that nothing in the kernel has written memory before it; a kernel
+// that stores, fences or takes atomics in a loop (the persistent batched kernel) loses that, and
+// such loads turn into vector loads plus readfirstlane.  Loads from the constant address space
+// are scalar whenever the address is uniform.  (The record streams keep the global view: on
+// the per-launch kernels it measured slightly faster.)
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* cview(const T* p) {
+    return (const __attribute__((address_space(4))) T*)p;
+}
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 

@@ -11,6 +11,8 @@
 //
 // RK4 / Cash-Karp RK5 stages (src/cuintRK4.cu, src/cuintRK5.cu) are one pass per stage
 // with ping-pong stage inputs, textbook stage formation.
+#include <mutex>
+
 #include "field.hpp"
 #include "kernels.hpp"
 #include "philox.hpp"
@@ -28,7 +30,7 @@ __device__ __forceinline__ void exch_row_sell(const Exch<V>& e, uint32_t i, cons
     const uint32_t n = e.n;
     dot[0] = dot[1] = dot[2] = R(0);
     const uint32_t sl = i >> 6, lane = i & 63u;
-    const u32x2 sd = gview(reinterpret_cast<const u32x2*>(e.slice))[sl];
+    const u32x2 sd = cview(reinterpret_cast<const u32x2*>(e.slice))[sl];
     const uint32_t q0 = sd.x, len = sd.y;
     const auto* __restrict__ Sg = gview(S);
     R v[NC];
@@ -132,7 +134,7 @@ __device__ __forceinline__ void exch_row_pack_fast(const Exch<float>& e, uint32_
     // rows come in whole waves (row = 64 k + lane in every caller), so the slice, its
     // descriptor and the loop bounds are wave-uniform: scalar loads and uniform branches
     const uint32_t sl = __builtin_amdgcn_readfirstlane(i >> 6), lane = i & 63u;
-    const u32x2 sd = gview(reinterpret_cast<const u32x2*>(e.slice))[sl];
+    const u32x2 sd = cview(reinterpret_cast<const u32x2*>(e.slice))[sl];
     const uint32_t q0 = sd.x, nch = sd.y >> 2;
     const auto* __restrict__ p = gview(reinterpret_cast<const u32x4*>(e.ent + (size_t)q0 * kSlice)) + lane;
     const uint32_t db = e.dbits;
@@ -286,7 +288,7 @@ __device__ __forceinline__ void exch_row_line(const Exch<float>& e, uint32_t i, 
     using RW = LineRecW<NC>;
     constexpr int G = RW::G, W = RW::W, NX = RW::NX;
     const uint32_t sl = __builtin_amdgcn_readfirstlane(i >> 6);
-    const u32x2 sd = gview(reinterpret_cast<const u32x2*>(e.lslice))[sl];
+    const u32x2 sd = cview(reinterpret_cast<const u32x2*>(e.lslice))[sl];
     if constexpr (LFB) {
         if (sd.y == kLineFallback) {
             exch_row_pack_fast<NC, 2, G3>(e, i, S, sdict, dot);
@@ -846,18 +848,11 @@ struct BatchMember {
     uint64_t t0;  // step of the first step of the current launch chunk
 };
 
-template <int PHASE, int FMT, int NC, int TPB, bool G3>
-__global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restrict__ mem,
-                                                    const uint32_t* __restrict__ tile_member, uint32_t step_off,
-                                                    int xcd) {
-    const uint32_t tile = tile_index(xcd);
-    const uint32_t m = __builtin_amdgcn_readfirstlane(tile_member[tile]);
-    const BatchMember& bm = mem[m];
+// One half step of Heun for row i of member bm (a live row): the predictor (PHASE 0) or the
+// corrector (PHASE 1) at step t.
+template <int PHASE, int FMT, int NC, bool G3>
+__device__ __forceinline__ void batch_row(const BatchMember& bm, uint32_t i, uint64_t t, const float* sdict) {
     const FusedArgs& a = bm.a;  // read in place (uniform scalar loads); a local copy lands in scratch
-    const uint64_t t = bm.t0 + step_off;
-    SLLG_STAGE_DICT(FMT, NC, a)
-    const uint32_t i = (tile - bm.tile0) * TPB + threadIdx.x;
-    if (!row_live<FMT>(a, i)) return;
     const uint32_t n = a.n;
     float s[3], f[3];
     if constexpr (PHASE == 0) {
@@ -894,6 +889,109 @@ __global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restric
             o->x = p[0] * r;
             o->y = p[1] * r;
             o->z = p[2] * r;
+        }
+    }
+}
+
+template <int PHASE, int FMT, int NC, int TPB, bool G3>
+__global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restrict__ mem,
+                                                    const uint32_t* __restrict__ tile_member, uint32_t step_off,
+                                                    int xcd) {
+    const uint32_t tile = tile_index(xcd);
+    const uint32_t m = __builtin_amdgcn_readfirstlane(tile_member[tile]);
+    const BatchMember& bm = mem[m];
+    SLLG_STAGE_DICT(FMT, NC, bm.a)
+    const uint32_t i = (tile - bm.tile0) * TPB + threadIdx.x;
+    if (!row_live<FMT>(bm.a, i)) return;
+    batch_row<PHASE, FMT, NC, G3>(bm, i, bm.t0 + step_off, sdict);
+}
+
+// Wait at the e-th barrier of the launch until every block of the grid has arrived.  Every
+// wave first waits until its own stores are acknowledged by the L2; then one wave releases the
+// block's results at agent scope (each XCD has an L2 of its own: the release writes its dirty
+// lines back), arrives, polls, and acquires at agent scope (invalidating the CU's L1 and the
+// XCD's L2) before the block reads what other blocks wrote.  A fence per wave would repeat
+// that cache maintenance for every wave.  The wait is bounded: after spin_limit polls the
+// block raises `abort`.  Returns false when the launch is being abandoned.
+__device__ __forceinline__ bool grid_sync(GridBar* bar, uint32_t e, uint32_t spin_limit) {
+    __shared__ uint32_t go;
+    __builtin_amdgcn_s_waitcnt(0);  // vmcnt, expcnt and lgkmcnt all zero: this wave's stores are in the L2
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t target = (e + 1u) * gridDim.x;
+        __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t it = 0, ok = 1;
+        while (__hip_atomic_load(&bar->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || it >= spin_limit) {
+                __hip_atomic_store(&bar->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = 0;
+                break;
+            }
+            ++it;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (ok && __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+        __hip_atomic_fetch_max(&bar->polls, it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        go = ok;
+    }
+    __syncthreads();
+    return go != 0;
+}
+
+// Persistent batched Heun: one launch advances every member of the table `nsteps` steps from
+// step offset step_off.  The grid holds only as many blocks as fit on the GPU at once (a
+// cooperative launch); block b takes tiles in a stride over the tile range of its XCD (b % 8,
+// with the XCD order) or over all tiles, the same tiles in both half steps, and a grid barrier
+// separates the half steps.  Each row computes exactly what k_heun_batch computes for it.
+template <int FMT, int NC, int TPB, bool G3>
+__global__ __launch_bounds__(TPB) void k_heun_batch_persist(const BatchMember* __restrict__ mem,
+                                                            const uint32_t* __restrict__ tile_member,
+                                                            uint32_t ntiles, uint32_t step_off, uint32_t nsteps,
+                                                            int xcd, GridBar* bar, uint32_t spin_limit) {
+    __shared__ float sdict_s[FMT == FMT_SELL_PACK ? kDictLds : 1];
+    extern __shared__ float sdict_d[];
+    float* const sdict = is_line_fmt(FMT) ? sdict_d : sdict_s;
+    uint32_t first, stride, end;
+    if (xcd) {  // gridDim.x is a multiple of 8 here (checked on the host)
+        const uint32_t x = blockIdx.x & 7u, per = ntiles >> 3, rem = ntiles & 7u;
+        first = x * per + (x < rem ? x : rem);
+        end = first + per + (x < rem ? 1u : 0u);
+        first += blockIdx.x >> 3;
+        stride = gridDim.x >> 3;
+    } else {
+        first = blockIdx.x;
+        stride = gridDim.x;
+        end = ntiles;
+    }
+    if (nsteps == 0) {  // residency probe: every block meets at one barrier, no data touched
+        (void)grid_sync(bar, 0, spin_limit);
+        return;
+    }
+    uint32_t staged = ~0u, e = 0;
+    for (uint32_t k = 0; k < nsteps; ++k) {
+        for (int phase = 0; phase < 2; ++phase) {
+            for (uint32_t tile = first; tile < end; tile += stride) {
+                const uint32_t m = cview(tile_member)[tile];
+                const BatchMember& bm = *(const BatchMember*)cview(mem + m);
+                if constexpr (FMT == FMT_SELL_PACK || is_line_fmt(FMT)) {
+                    if (m != staged && (FMT == FMT_SELL_PACK || bm.a.ex.lfall)) {
+                        const uint32_t nd = bm.a.ex.ndict * NC;
+                        __syncthreads();  // every wave is done with the previous member's dictionary
+                        for (uint32_t q = threadIdx.x; q < nd; q += TPB) sdict[q] = gview(bm.a.ex.dict)[q];
+                        __syncthreads();
+                        staged = m;
+                    }
+                }
+                const uint32_t i = (tile - bm.tile0) * TPB + threadIdx.x;
+                if (row_live<FMT>(bm.a, i)) {
+                    if (phase == 0) batch_row<0, FMT, NC, G3>(bm, i, bm.t0 + step_off + k, sdict);
+                    else batch_row<1, FMT, NC, G3>(bm, i, bm.t0 + step_off + k, sdict);
+                }
+            }
+            if (k + 1 == nsteps && phase == 1) break;  // the launch's end orders the last half step
+            if (!grid_sync(bar, e++, spin_limit)) return;
         }
     }
 }
@@ -1483,6 +1581,117 @@ int batch_heun_launch(hipStream_t s, int fmt, int nc, const void* dmem, const vo
 #undef SLLG_L
     SLLG_HIP(hipGetLastError());
     return 0;
+}
+
+// Clears the arrival count and the poll maximum of a grid barrier (its abort flag stays).
+__global__ void k_bar_reset(GridBar* bar) {
+    bar->count = 0;
+    bar->polls = 0;
+}
+
+// Polls of a residency probe's barrier: every block of a grid that fits is dispatched within
+// microseconds, and a block that cannot become resident keeps the barrier open until then.
+constexpr uint32_t kProbePolls = 1u << 16;
+constexpr int kProbeDevices = 64;
+
+// Blocks per CU of kernel f that are resident together, proven by probe launches: the
+// occupancy figure first, then one fewer per failed probe.  A probe is a cooperative launch of
+// blocks-per-CU x CUs blocks that only meets at one grid barrier with a bounded wait; when a
+// block cannot become resident the resident ones give up after kProbePolls polls, and the rest
+// leave at once when they start.  (The occupancy figure has been larger than what runs.)
+static int probe_resident(hipStream_t s, const void* f, int tpb, size_t lds, int per, int ncu, GridBar* bar,
+                          int* out) {
+    for (int p = per; p >= 1; --p) {
+        const uint32_t g = (uint32_t)p * (uint32_t)ncu;
+        k_bar_reset<<<1, 1, 0, s>>>(bar);
+        SLLG_HIP(hipGetLastError());
+        const BatchMember* nm = nullptr;
+        const uint32_t* nmap = nullptr;
+        uint32_t zero = 0, lim = kProbePolls;
+        int x = 0;
+        void* args[] = {(void*)&nm, (void*)&nmap, (void*)&g, (void*)&zero, (void*)&zero,
+                        (void*)&x, (void*)&bar, (void*)&lim};
+        SLLG_HIP(hipLaunchCooperativeKernel(f, dim3(g), dim3(tpb), args, (unsigned)lds, s));
+        GridBar hb{};
+        SLLG_HIP(hipMemcpyAsync(&hb, bar, sizeof(hb), hipMemcpyDeviceToHost, s));
+        SLLG_HIP(hipStreamSynchronize(s));
+        if (!hb.abort) {
+            *out = p;
+            return 0;
+        }
+        SLLG_HIP(hipMemsetAsync(bar, 0, sizeof(GridBar), s));
+        SLLG_HIP(hipStreamSynchronize(s));
+    }
+    SLLG_FAIL(LLG_E_DEVICE, "no grid of the persistent kernel was resident in a probe");
+}
+
+template <int F, int N, int TPB, bool G3>
+static int launch_persist(hipStream_t s, const BatchMember* m, const uint32_t* map, uint32_t ntiles,
+                          uint32_t step_off, uint32_t nsteps, int xcd, GridBar* bar, uint32_t spin_limit,
+                          uint32_t* grid_out) {
+    const void* f = reinterpret_cast<const void*>(&k_heun_batch_persist<F, N, TPB, G3>);
+    const size_t lds = is_line_fmt(F) ? kDictLds * sizeof(float) : 0;
+    int dev = 0, ncu = 0, coop = 0, per = 0;
+    SLLG_HIP(hipGetDevice(&dev));
+    SLLG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    if (!coop) SLLG_FAIL(LLG_E_UNSUPPORTED, "device %d has no cooperative launches", dev);
+    SLLG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    // blocks per CU proven resident, per device (0: not probed yet)
+    static std::mutex mu;
+    static int resident[kProbeDevices] = {};
+    if (dev < 0 || dev >= kProbeDevices) SLLG_FAIL(LLG_E_INVALID, "device %d beyond %d", dev, kProbeDevices);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!resident[dev]) {
+            SLLG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, TPB, lds));
+            if (per < 1 || ncu < 1) SLLG_FAIL(LLG_E_DEVICE, "no resident block of the persistent kernel (%d per CU)", per);
+            SLLG_TRY(probe_resident(s, f, TPB, lds, per, ncu, bar, &resident[dev]));
+        }
+        per = resident[dev];
+    }
+    uint32_t grid = std::min<uint64_t>(ntiles, (uint64_t)per * (uint64_t)ncu);
+    if (xcd && grid >= 8) grid &= ~7u;  // the XCD tile ranges need every XCD to hold blocks
+    else xcd = 0;
+    k_bar_reset<<<1, 1, 0, s>>>(bar);
+    SLLG_HIP(hipGetLastError());
+    void* args[] = {(void*)&m, (void*)&map, (void*)&ntiles, (void*)&step_off, (void*)&nsteps,
+                    (void*)&xcd, (void*)&bar, (void*)&spin_limit};
+    SLLG_HIP(hipLaunchCooperativeKernel(f, dim3(grid), dim3(TPB), args, (unsigned)lds, s));
+    if (grid_out) *grid_out = grid;
+    return 0;
+}
+
+template <int F, int N>
+static int persist_tpb(hipStream_t s, const BatchMember* m, const uint32_t* map, uint32_t ntiles, uint32_t step_off,
+                       uint32_t nsteps, uint32_t tpb, int xcd, bool g3, GridBar* bar, uint32_t spin_limit,
+                       uint32_t* grid) {
+    if constexpr (F == FMT_SELL_PACK || is_line_fmt(F)) {
+        if (g3) {
+            if (tpb == 128) return launch_persist<F, N, 128, true>(s, m, map, ntiles, step_off, nsteps, xcd, bar, spin_limit, grid);
+            if (tpb == 512) return launch_persist<F, N, 512, true>(s, m, map, ntiles, step_off, nsteps, xcd, bar, spin_limit, grid);
+            return launch_persist<F, N, 256, true>(s, m, map, ntiles, step_off, nsteps, xcd, bar, spin_limit, grid);
+        }
+    }
+    if (tpb == 128) return launch_persist<F, N, 128, false>(s, m, map, ntiles, step_off, nsteps, xcd, bar, spin_limit, grid);
+    if (tpb == 512) return launch_persist<F, N, 512, false>(s, m, map, ntiles, step_off, nsteps, xcd, bar, spin_limit, grid);
+    return launch_persist<F, N, 256, false>(s, m, map, ntiles, step_off, nsteps, xcd, bar, spin_limit, grid);
+}
+
+int batch_heun_persistent(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,
+                          uint32_t step_off, uint32_t nsteps, uint32_t tpb, int xcd, bool g3, GridBar* bar,
+                          uint32_t spin_limit, uint32_t* grid) {
+    if (ntiles == 0 || nsteps == 0) return 0;
+    if (tpb != 128 && tpb != 256 && tpb != 512) SLLG_FAIL(LLG_E_INVALID, "batch tile of %u threads", tpb);
+    if (!bar) SLLG_FAIL(LLG_E_INVALID, "persistent launch without a grid barrier");
+    if ((uint64_t)nsteps * 2u * ((uint64_t)ntiles + 1u) >= (1ull << 32))
+        SLLG_FAIL(LLG_E_INVALID, "persistent launch of %u steps over %u tiles: barrier count overflows", nsteps, ntiles);
+    const BatchMember* m = static_cast<const BatchMember*>(dmem);
+    const uint32_t* map = static_cast<const uint32_t*>(dmap);
+    int rc = 0;
+#define SLLG_L(F, N) rc = persist_tpb<F, N>(s, m, map, ntiles, step_off, nsteps, tpb, xcd, g3, bar, spin_limit, grid)
+    SLLG_DISPATCH_FMT_NC(fmt, nc, SLLG_L);
+#undef SLLG_L
+    return rc;
 }
 
 }  // namespace sllg

@@ -36,6 +36,22 @@ int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<ui
 int batch_heun_launch(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,
                       uint32_t step_off, int phase, uint32_t tpb, int xcd, bool g3);
 bool batch_needs_y_soa(const llg_ctx* c);  // the member's predictor must also write the SoA Y
+// Grid barrier of a persistent batched launch (16 bytes of device memory).  `count` counts
+// arrivals and is cleared before every launch; `abort` is raised by a block whose wait ran
+// out of polls and stays raised until the host clears it; `polls` is the longest wait.
+struct GridBar {
+    uint32_t count;
+    uint32_t abort;
+    uint32_t polls;
+    uint32_t pad_;
+};
+// Persistent batched Heun: nsteps steps (from step offset step_off) of every member of the
+// table in one cooperative launch of at most as many blocks as fit on the GPU at once, with a
+// grid barrier between half steps whose every wait ends after spin_limit polls.  Clears
+// bar->count first; *grid receives the launch's block count.
+int batch_heun_persistent(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,
+                          uint32_t step_off, uint32_t nsteps, uint32_t tpb, int xcd, bool g3, GridBar* bar,
+                          uint32_t spin_limit, uint32_t* grid);
 int dipolar_set_stream(llg_ctx* c);
 // Handle over an existing coupling store (capi.hip).
 int create_with_store(const llg_desc* d, const std::shared_ptr<ExchStore>& store, llg_ctx** out);

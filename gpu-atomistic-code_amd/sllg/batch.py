@@ -9,12 +9,14 @@ import numpy as np
 from . import Desc, LLGError, _check, _p, lib
 
 BATCH_GRAPH = 0x1
+BATCH_PERSISTENT = 0x2
+SPIN_DEFAULT = 0x1000000  # polls per grid-barrier wait of a persistent launch
 REQ_MT, REQ_RAMP, REQ_SW = 0, 1, 2
 
 
 class BatchInfo(C.Structure):
     _fields_ = [("members", C.c_uint32), ("tiles", C.c_uint32), ("spins", C.c_uint64), ("device_bytes", C.c_uint64),
-                ("budget_bytes", C.c_uint64), ("graphs_built", C.c_uint32), ("pad_", C.c_uint32)]
+                ("budget_bytes", C.c_uint64), ("graphs_built", C.c_uint32), ("persist_grid", C.c_uint32)]
 
 
 class Request(C.Structure):
@@ -79,6 +81,7 @@ def _bind():
     L.llg_batch_get_info.argtypes = [P, C.POINTER(BatchInfo)]
     L.llg_batch_set_launch.argtypes = [P, C.c_uint32, C.c_int, C.c_int, C.c_int]
     L.llg_batch_autotune.argtypes = [P, C.c_uint32, C.POINTER(BatchTune)]
+    L.llg_batch_set_persistent.argtypes = [P, C.c_int, C.c_uint32]
     L.llg_serve_run.argtypes = [C.POINTER(ServeConfig), C.POINTER(Request), C.c_uint32, C.POINTER(Result), dp,
                                 C.POINTER(C.c_uint64), dp, C.POINTER(C.c_uint64), C.POINTER(ServeStats)]
     _bound = True
@@ -143,6 +146,12 @@ class Batch:
     def set_launch(self, tpb, xcd=1, line_mode=1, gather_copy=0):
         _check(self.L.llg_batch_set_launch(self.h, int(tpb), int(xcd), int(line_mode), int(gather_copy)), self.L)
 
+    def set_persistent(self, on=True, spin_limit=SPIN_DEFAULT):
+        """Advance each chunk in persistent launches (one cooperative launch per group of
+        members and per run of steps between dipolar refreshes, grid barriers between half
+        steps, every wait bounded by spin_limit polls)."""
+        _check(self.L.llg_batch_set_persistent(self.h, 1 if on else 0, int(spin_limit)), self.L)
+
     def autotune(self, nsteps=10):
         """Time every tile size, block order and gather copy (and both readers when a member
         has a line encoding) on the live members (their spins and step counters are
@@ -173,7 +182,7 @@ class Batch:
     def info(self):
         i = BatchInfo()
         _check(self.L.llg_batch_get_info(self.h, C.byref(i)), self.L)
-        return {f: getattr(i, f) for f, _ in BatchInfo._fields_ if f != "pad_"}
+        return {f: getattr(i, f) for f, _ in BatchInfo._fields_}
 
 
 def make_requests(specs):

@@ -22,7 +22,9 @@
 extern "C" {
 #endif
 
-#define LLG_BATCH_GRAPH 0x1u /* replay captured hipGraphs of whole chunks */
+#define LLG_BATCH_GRAPH 0x1u      /* replay captured hipGraphs of whole chunks */
+#define LLG_BATCH_PERSISTENT 0x2u /* advance a chunk in persistent launches (llg_batch_set_persistent) */
+#define LLG_BATCH_SPIN_DEFAULT 0x1000000u /* polls per grid-barrier wait of a persistent launch */
 
 typedef struct llg_batch llg_batch;
 
@@ -33,7 +35,7 @@ typedef struct llg_batch_info {
     uint64_t device_bytes; /* bytes held by live members */
     uint64_t budget_bytes;
     uint32_t graphs_built; /* hipGraph instantiations so far */
-    uint32_t pad_;
+    uint32_t persist_grid; /* blocks of the last persistent launch (0: none yet) */
 } llg_batch_info;
 
 /* Upper bound of the device bytes llg_create would allocate for d. */
@@ -72,6 +74,22 @@ int llg_batch_member_status(llg_batch* b, int member, int* status);
  * gather copy do not change results (each thread owns one spin of one member, same sums);
  * the reader changes the order of the sums to FP32 rounding. */
 int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd, int line_mode, int gather_copy);
+
+/* Persistent chunks (on = 1; LLG_BATCH_PERSISTENT at creation sets it too): llg_batch_step
+ * advances each group of members through all steps of a chunk in ONE cooperative launch of
+ * as many blocks as fit on the GPU at once; each block takes a fixed set of tiles, and a grid
+ * barrier (agent-scope atomics) separates the half steps.  Dipolar refreshes split the chunk:
+ * a refresh runs between two persistent launches, before the predictor of its step; a batch
+ * with a member that refreshes at the predicted spins (every stage) uses per-half-step
+ * launches.  Results are those of the per-half-step launches, bit for bit.  Takes precedence
+ * over LLG_BATCH_GRAPH.  Every barrier wait ends after spin_limit polls of about a
+ * microsecond each (LLG_BATCH_SPIN_DEFAULT, the value LLG_BATCH_PERSISTENT sets, is 2^24; 0
+ * abandons the launch at the first barrier that is not complete on arrival, which tests the
+ * failure path): a wait that runs out abandons the launch, llg_batch_step returns
+ * LLG_E_DEVICE, the members' spins are left partly advanced and their step counters
+ * unchanged.  No reference counterpart (the reference's step launches a kernel
+ * per field term and per half step, src/cuda.cu:43-240). */
+int llg_batch_set_persistent(llg_batch* b, int on, uint32_t spin_limit);
 
 /* The batch's own autotuner (no reference counterpart; the reference takes
  * threadsperblock from its config, src/cuda.cu:322-324): times nsteps steps of the live

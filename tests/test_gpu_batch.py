@@ -97,6 +97,89 @@ def test_batch_graph_replay_equals_eager(gpu):
         assert np.array_equal(a, b)
 
 
+def _persist_members():
+    """Members of four reader groups: packed entries (sc 8^3, sc 12^3, fcc 4^3 x 4), 64-cell
+    lines (2 x 2 x 64) and a 3x3-tensor stencil with DMI (bct 8 x 8 x 4 x 2)."""
+    a, sa, _ = W.sc_ferro(precision="fp32", dims=(2, 2, 64), seed=21)
+    d, sd = W.bct_afm_dmi(Lx=8, Ly=8, Lz=4, precision="fp32", scheme="heun")
+    return _members() + [(a, sa, 200.0, (0.0, 0.0, 0.3)), (d, sd, 50.0, (0.0, 0.0, 0.0))]
+
+
+@pytest.mark.parametrize("shape", [(256, 1, 1, 0), (128, 0, 1, 1), (512, 1, 0, 1)])
+def test_batch_persistent_equals_per_half_step(gpu, shape):
+    """Persistent chunks (one cooperative launch per reader group, a grid barrier between the
+    half steps, each block a fixed set of tiles) give the per-half-step launches' trajectories
+    bit for bit: uneven members of four reader groups, one joining late, chunks of 17, 30 and 1
+    steps, for three launch shapes (threads per tile, XCD order, line reader, gather copy)."""
+    res = []
+    for persist in (False, True):
+        B = SB.Batch()
+        B.set_launch(*shape)
+        if persist:
+            B.set_persistent(True)
+        ms = _persist_members()
+        ids = [B.add(s, s0, T, Bf) for s, s0, T, Bf in ms[:-1]]
+        B.step(17)
+        late = B.add(*ms[-1])
+        B.step(30)
+        B.step(1)
+        info = B.info()
+        assert (info["persist_grid"] > 0) == persist
+        for k in ids:
+            assert B.get_step(k) == 48
+        assert B.get_step(late) == 31
+        res.append([B.get_spins(k) for k in ids + [late]])
+        B.close()
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+
+
+def test_batch_persistent_dipolar_segments(gpu):
+    """Dipolar members refreshing every 3 and every 5 steps (at the spins of the step's start):
+    the persistent chunks split at each refresh and give the per-half-step launches' bits.  A
+    batch with a member that also refreshes at the predicted spins (every stage) takes the
+    per-half-step launches, with the same bits."""
+    for every_stage in (0, 1):
+        res = []
+        for persist in (False, True):
+            B = SB.Batch()
+            if persist:
+                B.set_persistent(True)
+            s1, x1 = W.sc_film_dip(Lx=8, Ly=8, Lz=2, precision="fp32", refresh=3)
+            s2, x2 = W.sc_film_dip(Lx=16, Ly=8, Lz=2, precision="fp32", refresh=5, seed=4,
+                                   every_stage=every_stage)
+            a, sa, _ = W.sc_ferro(L=8, precision="fp32", seed=11)
+            ids = [B.add(s1, x1, 50.0), B.add(s2, x2, 100.0), B.add(a, sa, 300.0)]
+            B.step(7)
+            B.step(11)
+            assert (B.info()["persist_grid"] > 0) == (persist and not every_stage)
+            res.append([B.get_spins(k) for k in ids])
+            B.close()
+        for a, b in zip(*res):
+            assert np.array_equal(a, b)
+
+
+def test_batch_persistent_barrier_wait_is_bounded(gpu):
+    """The grid barrier's failure path, forced from the host: with a poll limit of 0 the first
+    block at a barrier that is not yet complete abandons the launch and every other block
+    returns at its next wait, so the grid drains; llg_batch_step reports LLG_E_DEVICE and the
+    step counter stays.  With the default limit the same batch then steps normally."""
+    s, s0, _ = W.sc_ferro(L=32, precision="fp32", seed=5)  # 32,768 spins: 128 tiles of 256
+    B = SB.Batch()
+    k = B.add(s, s0, 300.0)
+    B.set_persistent(True, spin_limit=0)
+    with pytest.raises(sllg.LLGError) as e:
+        B.step(10)
+    assert e.value.code == -3
+    assert B.get_step(k) == 0
+    B.set_persistent(True)
+    B.step(10)
+    assert B.get_step(k) == 10
+    sp = B.get_spins(k)
+    assert np.isfinite(sp).all()
+    assert float(np.abs(np.linalg.norm(sp, axis=0) - 1.0).max()) <= 2e-6
+
+
 def test_batch_budget_admission(gpu):
     """A member is admitted when the bytes held plus its upper-bound estimate fit the budget;
     otherwise LLG_E_NOMEM and nothing is allocated.  (The bytes held can be well below the
