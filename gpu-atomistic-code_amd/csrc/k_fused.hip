@@ -190,14 +190,19 @@ struct TileRecW {
     __device__ uint32_t mask() const { return w[0] & 7u; }
     __device__ uint32_t window(int u) const { return u == 0 ? w[0] & ~7u : w[u]; }
 };
-// Whole 16-byte units, so the record arrives in few wide scalar loads.
-template <class RW>
+// Whole 16-byte units, so the record arrives in few wide scalar loads.  CV: through the
+// constant view, for kernels whose record loads follow stores, fences or atomics (the
+// persistent batched kernel); the per-launch kernels measured slightly faster without it.
+template <class RW, bool CV = false>
 __device__ __forceinline__ RW load_rec(const uint32_t* __restrict__ p) {
-    const auto* __restrict__ q = gview(reinterpret_cast<const u32x4*>(p));
     RW o;
 #pragma unroll
     for (int k = 0; k < RW::W / 4; ++k) {
-        const u32x4 v = q[k];
+        u32x4 v;
+        if constexpr (CV)
+            v = cview(reinterpret_cast<const u32x4*>(p))[k];
+        else
+            v = gview(reinterpret_cast<const u32x4*>(p))[k];
         o.w[4 * k] = v.x;
         o.w[4 * k + 1] = v.y;
         o.w[4 * k + 2] = v.z;
@@ -282,7 +287,7 @@ __device__ __forceinline__ void line_group_sum(const RW& r, const u32x3* x, floa
 // read from the packed entries.
 // LFB: the encoding has slices marked kLineFallback (compiled in only then: the packed
 // reader's registers would otherwise cost the line kernels occupancy).
-template <int NC, bool G3, int LEN, bool OPEN, bool LFB>
+template <int NC, bool G3, int LEN, bool OPEN, bool LFB, bool CV = false>
 __device__ __forceinline__ void exch_row_line(const Exch<float>& e, uint32_t i, const float* __restrict__ S,
                                               const float* sdict, float dot[3]) {
     using RW = LineRecW<NC>;
@@ -304,20 +309,20 @@ __device__ __forceinline__ void exch_row_line(const Exch<float>& e, uint32_t i, 
         // two window buffers used in turn (no register copies, so no early waits on the
         // loads in flight); pairs of groups run in the loop, the last one or two after it
         u32x3 xa[G], xb[G];
-        RW c0 = load_rec<LineRecW<NC>>(rp);
+        RW c0 = load_rec<LineRecW<NC>, CV>(rp);
         line_loads<NC, G3>(sb, lane, base, c0.w + 1, xa);
         uint32_t r = 0;
         for (; r + 2 < ngrp; r += 2) {
-            const RW c1 = load_rec<LineRecW<NC>>(rp + (size_t)(r + 1) * W);
+            const RW c1 = load_rec<LineRecW<NC>, CV>(rp + (size_t)(r + 1) * W);
             line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
             line_group_sum<NC>(c0, xa, p);
-            const RW c2 = load_rec<LineRecW<NC>>(rp + (size_t)(r + 2) * W);
+            const RW c2 = load_rec<LineRecW<NC>, CV>(rp + (size_t)(r + 2) * W);
             line_loads<NC, G3>(sb, lane, base, c1.w + NX, xa);
             line_group_sum<NC>(c1, xb, p);
             c0 = c2;
         }
         if (r + 1 < ngrp) {
-            const RW c1 = load_rec<LineRecW<NC>>(rp + (size_t)(r + 1) * W);
+            const RW c1 = load_rec<LineRecW<NC>, CV>(rp + (size_t)(r + 1) * W);
             line_loads<NC, G3>(sb, lane, base, c0.w + NX, xb);
             line_group_sum<NC>(c0, xa, p);
             line_group_sum<NC>(c1, xb, p);
@@ -523,7 +528,7 @@ __device__ __forceinline__ void fused_tail(const FusedArgs& a, uint32_t i, uint3
 // then read only by the four-spin term).  EDEM: the dipolar field is loaded with the row's
 // own spin, so its latency overlaps the exchange gathers instead of following them (the
 // sums keep their order; the batch kernels, whose members are mostly dipolar films, use it).
-template <int FMT, int NC, int UNR = 2, bool G3 = false, bool LFB = true, bool EDEM = false>
+template <int FMT, int NC, int UNR = 2, bool G3 = false, bool LFB = true, bool EDEM = false, bool CV = false>
 __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32_t iref, const float* __restrict__ src,
                                           const float* __restrict__ gsrc, const float* sdict, float s[3], float f[3],
                                           uint64_t t) {
@@ -548,7 +553,7 @@ __device__ __forceinline__ void fused_rhs(const FusedArgs& a, uint32_t i, uint32
     if constexpr (FMT == FMT_SELL_PACK)
         exch_row_pack_fast<NC, UNR, G3>(a.ex, i, gsrc, sdict, h);
     else if constexpr (is_line_fmt(FMT))
-        exch_row_line<NC, G3, line_len(FMT), line_open(FMT), LFB>(a.ex, i, gsrc, sdict, h);
+        exch_row_line<NC, G3, line_len(FMT), line_open(FMT), LFB, CV>(a.ex, i, gsrc, sdict, h);
     else
         exch_any<FMT, NC, float, float, float>(a.ex, i, src, h);
     fused_tail<EDEM>(a, i, iref, src, s, h, hd, f, t);
@@ -849,14 +854,14 @@ struct BatchMember {
 };
 
 // One half step of Heun for row i of member bm (a live row): the predictor (PHASE 0) or the
-// corrector (PHASE 1) at step t.
-template <int PHASE, int FMT, int NC, bool G3>
+// corrector (PHASE 1) at step t.  CV: line records through the constant view.
+template <int PHASE, int FMT, int NC, bool G3, bool CV = false>
 __device__ __forceinline__ void batch_row(const BatchMember& bm, uint32_t i, uint64_t t, const float* sdict) {
     const FusedArgs& a = bm.a;  // read in place (uniform scalar loads); a local copy lands in scratch
     const uint32_t n = a.n;
     float s[3], f[3];
     if constexpr (PHASE == 0) {
-        fused_rhs<FMT, NC, 2, G3, true, true>(a, i, a.perm.to_ref(i), bm.S, G3 ? bm.S3 : bm.S, sdict, s, f, t);
+        fused_rhs<FMT, NC, 2, G3, true, true, CV>(a, i, a.perm.to_ref(i), bm.S, G3 ? bm.S3 : bm.S, sdict, s, f, t);
         float y[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) y[c] = s[c] + a.rdt * f[c];
@@ -877,7 +882,7 @@ __device__ __forceinline__ void batch_row(const BatchMember& bm, uint32_t i, uin
         float av[3];  // issued ahead of the field evaluation
 #pragma unroll
         for (int c = 0; c < 3; ++c) av[c] = gview(bm.A)[(size_t)c * n + i];
-        fused_rhs<FMT, NC, 2, G3, true, true>(a, i, a.perm.to_ref(i), bm.Y, G3 ? bm.Y3 : bm.Y, sdict, s, f, t);
+        fused_rhs<FMT, NC, 2, G3, true, true, CV>(a, i, a.perm.to_ref(i), bm.Y, G3 ? bm.Y3 : bm.Y, sdict, s, f, t);
         float p[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) p[c] = av[c] + a.half_rdt * f[c];
@@ -909,20 +914,33 @@ __global__ __launch_bounds__(TPB) void k_heun_batch(const BatchMember* __restric
 // Wait at the e-th barrier of the launch until every block of the grid has arrived.  Every
 // wave first waits until its own stores are acknowledged by the L2; then one wave releases the
 // block's results at agent scope (each XCD has an L2 of its own: the release writes its dirty
-// lines back), arrives, polls, and acquires at agent scope (invalidating the CU's L1 and the
-// XCD's L2) before the block reads what other blocks wrote.  A fence per wave would repeat
-// that cache maintenance for every wave.  The wait is bounded: after spin_limit polls the
-// block raises `abort`.  Returns false when the launch is being abandoned.
+// lines back) and arrives on its group's counter.  The group's last block acquires (so the
+// group's releases happen before what follows), releases again, arrives on `count`, polls it
+// until all groups are in, acquires and releases, and raises its group's flag; the group's
+// other blocks poll that flag.  Every block then acquires at agent scope (invalidating the
+// CU's L1 and the XCD's L2) before it reads what other blocks wrote.  A fence per wave would
+// repeat that cache maintenance for every wave.  Every wait is bounded: after spin_limit polls
+// the block raises `abort`, which every other wait also watches.  Returns false when the
+// launch is being abandoned.
 __device__ __forceinline__ bool grid_sync(GridBar* bar, uint32_t e, uint32_t spin_limit) {
     __shared__ uint32_t go;
     __builtin_amdgcn_s_waitcnt(0);  // vmcnt, expcnt and lgkmcnt all zero: this wave's stores are in the L2
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        const uint32_t target = (e + 1u) * gridDim.x;
-        __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t ng = gridDim.x < 8u ? gridDim.x : 8u, g = blockIdx.x & 7u;
+        const uint32_t nb = (gridDim.x - g + 7u) >> 3;  // blocks of group g
+        const uint32_t target = (e + 1u) * ng;
+        const uint32_t old = __hip_atomic_fetch_add(&bar->group[g][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const bool last = old + 1u == (e + 1u) * nb;
+        if (last) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        uint32_t* const word = last ? &bar->count : &bar->flag[g][0];
+        const uint32_t want = last ? target : e + 1u;
         uint32_t it = 0, ok = 1;
-        while (__hip_atomic_load(&bar->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
             if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || it >= spin_limit) {
                 __hip_atomic_store(&bar->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = 0;
@@ -932,7 +950,12 @@ __device__ __forceinline__ bool grid_sync(GridBar* bar, uint32_t e, uint32_t spi
             __builtin_amdgcn_s_sleep(2);
         }
         if (ok && __hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
-        __hip_atomic_fetch_max(&bar->polls, it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (last && ok) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+            __hip_atomic_store(&bar->flag[g][0], e + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (last)  // the wait on `count` (one atomic per group; the other blocks' waits end with it)
+            __hip_atomic_fetch_max(&bar->polls, it, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         go = ok;
     }
@@ -986,8 +1009,8 @@ __global__ __launch_bounds__(TPB) void k_heun_batch_persist(const BatchMember* _
                 }
                 const uint32_t i = (tile - bm.tile0) * TPB + threadIdx.x;
                 if (row_live<FMT>(bm.a, i)) {
-                    if (phase == 0) batch_row<0, FMT, NC, G3>(bm, i, bm.t0 + step_off + k, sdict);
-                    else batch_row<1, FMT, NC, G3>(bm, i, bm.t0 + step_off + k, sdict);
+                    if (phase == 0) batch_row<0, FMT, NC, G3, true>(bm, i, bm.t0 + step_off + k, sdict);
+                    else batch_row<1, FMT, NC, G3, true>(bm, i, bm.t0 + step_off + k, sdict);
                 }
             }
             if (k + 1 == nsteps && phase == 1) break;  // the launch's end orders the last half step
@@ -1583,10 +1606,17 @@ int batch_heun_launch(hipStream_t s, int fmt, int nc, const void* dmem, const vo
     return 0;
 }
 
-// Clears the arrival count and the poll maximum of a grid barrier (its abort flag stays).
+// Clears the arrival counters and the poll maximum of a grid barrier (its abort flag stays).
 __global__ void k_bar_reset(GridBar* bar) {
-    bar->count = 0;
-    bar->polls = 0;
+    const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        bar->count = 0;
+        bar->polls = 0;
+    }
+    if (t < 8) {
+        bar->group[t][0] = 0;
+        bar->flag[t][0] = 0;
+    }
 }
 
 // Polls of a residency probe's barrier: every block of a grid that fits is dispatched within
@@ -1603,7 +1633,7 @@ static int probe_resident(hipStream_t s, const void* f, int tpb, size_t lds, int
                           int* out) {
     for (int p = per; p >= 1; --p) {
         const uint32_t g = (uint32_t)p * (uint32_t)ncu;
-        k_bar_reset<<<1, 1, 0, s>>>(bar);
+        k_bar_reset<<<1, 64, 0, s>>>(bar);
         SLLG_HIP(hipGetLastError());
         const BatchMember* nm = nullptr;
         const uint32_t* nmap = nullptr;
@@ -1652,7 +1682,7 @@ static int launch_persist(hipStream_t s, const BatchMember* m, const uint32_t* m
     uint32_t grid = std::min<uint64_t>(ntiles, (uint64_t)per * (uint64_t)ncu);
     if (xcd && grid >= 8) grid &= ~7u;  // the XCD tile ranges need every XCD to hold blocks
     else xcd = 0;
-    k_bar_reset<<<1, 1, 0, s>>>(bar);
+    k_bar_reset<<<1, 64, 0, s>>>(bar);
     SLLG_HIP(hipGetLastError());
     void* args[] = {(void*)&m, (void*)&map, (void*)&ntiles, (void*)&step_off, (void*)&nsteps,
                     (void*)&xcd, (void*)&bar, (void*)&spin_limit};
@@ -1683,7 +1713,7 @@ int batch_heun_persistent(hipStream_t s, int fmt, int nc, const void* dmem, cons
     if (ntiles == 0 || nsteps == 0) return 0;
     if (tpb != 128 && tpb != 256 && tpb != 512) SLLG_FAIL(LLG_E_INVALID, "batch tile of %u threads", tpb);
     if (!bar) SLLG_FAIL(LLG_E_INVALID, "persistent launch without a grid barrier");
-    if ((uint64_t)nsteps * 2u * ((uint64_t)ntiles + 1u) >= (1ull << 32))
+    if ((uint64_t)nsteps * 2u * ((uint64_t)ntiles + 8u) >= (1ull << 32))
         SLLG_FAIL(LLG_E_INVALID, "persistent launch of %u steps over %u tiles: barrier count overflows", nsteps, ntiles);
     const BatchMember* m = static_cast<const BatchMember*>(dmem);
     const uint32_t* map = static_cast<const uint32_t*>(dmap);

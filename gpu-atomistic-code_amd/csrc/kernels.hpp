@@ -36,14 +36,21 @@ int batch_build_table(const std::vector<llg_ctx*>& members, const std::vector<ui
 int batch_heun_launch(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,
                       uint32_t step_off, int phase, uint32_t tpb, int xcd, bool g3);
 bool batch_needs_y_soa(const llg_ctx* c);  // the member's predictor must also write the SoA Y
-// Grid barrier of a persistent batched launch (16 bytes of device memory).  `count` counts
-// arrivals and is cleared before every launch; `abort` is raised by a block whose wait ran
-// out of polls and stays raised until the host clears it; `polls` is the longest wait.
+// Grid barrier of a persistent batched launch (device memory).  Blocks arrive on one of 8
+// group counters (block % 8, each on its own 128-byte line); the last block of a group
+// arrives on `count`, waits for all groups there and then raises its group's flag, which the
+// group's other blocks wait on.  Agent-scope atomics on one address serialise, and 1,280
+// blocks polling one word contend, so each address sees at most a group.  The counters and
+// flags are cleared before every launch; `abort` is raised by a block whose wait ran out of
+// polls and stays raised until the host clears it; `polls` is the longest wait of a group's
+// last block.
 struct GridBar {
     uint32_t count;
     uint32_t abort;
     uint32_t polls;
-    uint32_t pad_;
+    uint32_t pad_[29];
+    uint32_t group[8][32];
+    uint32_t flag[8][32];
 };
 // Persistent batched Heun: nsteps steps (from step offset step_off) of every member of the
 // table in one cooperative launch of at most as many blocks as fit on the GPU at once, with a
