@@ -59,6 +59,8 @@ struct llg_batch {
     int line = 1;        // line reader for members with a line encoding (else their packed entries)
     int g3 = 0;          // gather from AoS3 copies of S and Y (packed and line readers)
     int persist = 0;     // chunks as persistent launches (grid barriers between half steps)
+    int persist_graph = 1;  // with LLG_BATCH_GRAPH: capture them (0 once a capture was refused)
+    bool gpersist = false;  // the captured graph holds persistent launches
     uint32_t spin_limit = LLG_BATCH_SPIN_DEFAULT;  // polls per grid-barrier wait
     DevBuf bar;          // GridBar of the persistent launches
     uint32_t persist_grid = 0;  // blocks of the last persistent launch
@@ -98,6 +100,7 @@ void drop_graph(llg_batch* b) {
     b->graph = nullptr;
     b->gsteps = 0;
     b->gsig.clear();
+    b->gpersist = false;
 }
 
 // Where the dipolar refreshes of the next chunk fall: they depend only on each dipolar
@@ -169,6 +172,21 @@ int launch_chunk(llg_batch* b, uint32_t nsteps, bool dipolar) {
         for (auto& g : b->groups)
             SLLG_TRY(batch_heun_launch(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, k, 1, b->tpb, b->xcd, b->g3 != 0));
     }
+    return 0;
+}
+
+// Before a persistent chunk is captured: the barrier buffer, and the residency probe of every
+// group's kernel shape (it synchronises the stream, so it cannot run inside a capture).
+int persist_prepare(llg_batch* b) {
+    if (!b->bar.p) {
+        SLLG_TRY(b->bar.alloc(sizeof(GridBar)));
+        SLLG_HIP(hipMemsetAsync(b->bar.p, 0, sizeof(GridBar), b->stream));
+    }
+    for (auto& g : b->groups)
+        SLLG_TRY(batch_heun_persistent(b->stream, g->fmt, g->nc, g->dmem.p, g->dmap.p, g->ntiles, 0, 0, b->tpb,
+                                       b->xcd, b->g3 != 0, static_cast<GridBar*>(b->bar.p), b->spin_limit,
+                                       &b->persist_grid));
+    SLLG_HIP(hipStreamSynchronize(b->stream));
     return 0;
 }
 
@@ -402,7 +420,40 @@ int llg_batch_step(llg_batch* b, uint32_t nsteps) {
         }
     SLLG_TRY(upload_tables(b));
     if (b->persist && !dip_stage) {
-        SLLG_TRY(launch_chunk_persistent(b, nsteps, dip));
+        bool replayed = false;
+        if ((b->flags & LLG_BATCH_GRAPH) && dip_graph && b->persist_graph) {
+            // one hipGraph per chunk shape: barrier resets, cooperative launches and the
+            // refreshes between them
+            std::vector<uint64_t> sig = refresh_signature(b);
+            if (!b->gexec || !b->gpersist || b->gsteps != nsteps || sig != b->gsig) {
+                drop_graph(b);
+                SLLG_TRY(persist_prepare(b));
+                SLLG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+                const int rc = launch_chunk_persistent(b, nsteps, dip);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(b->stream, &g);
+                hipGraphExec_t ge = nullptr;
+                if (!rc && e == hipSuccess && g && hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess) {
+                    b->graph = g;
+                    b->gexec = ge;
+                    b->gsteps = nsteps;
+                    b->gsig.swap(sig);
+                    b->gpersist = true;
+                    b->graphs_built++;
+                } else {
+                    // the runtime would not capture the cooperative launch: launch directly
+                    // from now on (the capture left nothing queued on the stream)
+                    if (g) (void)hipGraphDestroy(g);
+                    (void)hipGetLastError();
+                    b->persist_graph = 0;
+                }
+            }
+            if (b->gexec && b->gpersist) {
+                SLLG_HIP(hipGraphLaunch(b->gexec, b->stream));
+                replayed = true;
+            }
+        }
+        if (!replayed) SLLG_TRY(launch_chunk_persistent(b, nsteps, dip));
         GridBar hb{};
         SLLG_HIP(hipMemcpyAsync(&hb, b->bar.p, sizeof(hb), hipMemcpyDeviceToHost, b->stream));
         SLLG_HIP(hipStreamSynchronize(b->stream));
@@ -415,7 +466,7 @@ int llg_batch_step(llg_batch* b, uint32_t nsteps) {
         }
     } else if ((b->flags & LLG_BATCH_GRAPH) && dip_graph) {
         std::vector<uint64_t> sig = refresh_signature(b);
-        if (!b->gexec || b->gsteps != nsteps || sig != b->gsig) {
+        if (!b->gexec || b->gpersist || b->gsteps != nsteps || sig != b->gsig) {
             drop_graph(b);
             SLLG_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             const int rc = launch_chunk(b, nsteps, dip);

@@ -1679,6 +1679,7 @@ static int launch_persist(hipStream_t s, const BatchMember* m, const uint32_t* m
         }
         per = resident[dev];
     }
+    if (nsteps == 0) return 0;  // residency known; nothing to launch
     uint32_t grid = std::min<uint64_t>(ntiles, (uint64_t)per * (uint64_t)ncu);
     if (xcd && grid >= 8) grid &= ~7u;  // the XCD tile ranges need every XCD to hold blocks
     else xcd = 0;
@@ -1710,7 +1711,7 @@ static int persist_tpb(hipStream_t s, const BatchMember* m, const uint32_t* map,
 int batch_heun_persistent(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,
                           uint32_t step_off, uint32_t nsteps, uint32_t tpb, int xcd, bool g3, GridBar* bar,
                           uint32_t spin_limit, uint32_t* grid) {
-    if (ntiles == 0 || nsteps == 0) return 0;
+    if (ntiles == 0) return 0;
     if (tpb != 128 && tpb != 256 && tpb != 512) SLLG_FAIL(LLG_E_INVALID, "batch tile of %u threads", tpb);
     if (!bar) SLLG_FAIL(LLG_E_INVALID, "persistent launch without a grid barrier");
     if ((uint64_t)nsteps * 2u * ((uint64_t)ntiles + 8u) >= (1ull << 32))

@@ -55,7 +55,9 @@ struct GridBar {
 // Persistent batched Heun: nsteps steps (from step offset step_off) of every member of the
 // table in one cooperative launch of at most as many blocks as fit on the GPU at once, with a
 // grid barrier between half steps whose every wait ends after spin_limit polls.  Clears
-// bar->count first; *grid receives the launch's block count.
+// bar->count first; *grid receives the launch's block count.  nsteps == 0: only find how
+// many blocks per CU are resident together (probe launches, synchronous, once per kernel
+// shape and device), so a later launch can be captured into a hipGraph.
 int batch_heun_persistent(hipStream_t s, int fmt, int nc, const void* dmem, const void* dmap, uint32_t ntiles,
                           uint32_t step_off, uint32_t nsteps, uint32_t tpb, int xcd, bool g3, GridBar* bar,
                           uint32_t spin_limit, uint32_t* grid);

@@ -81,8 +81,11 @@ int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd, int line_mode, int
  * barrier (agent-scope atomics) separates the half steps.  Dipolar refreshes split the chunk:
  * a refresh runs between two persistent launches, before the predictor of its step; a batch
  * with a member that refreshes at the predicted spins (every stage) uses per-half-step
- * launches.  Results are those of the per-half-step launches, bit for bit.  Takes precedence
- * over LLG_BATCH_GRAPH.  Every barrier wait ends after spin_limit polls of about a
+ * launches.  Results are those of the per-half-step launches, bit for bit.  With
+ * LLG_BATCH_GRAPH as well, each chunk (barrier resets, the persistent launches and the
+ * refreshes between them) is captured once into a hipGraph and replayed; the residency probe
+ * runs before the capture, and a runtime that refuses to capture a cooperative launch gets
+ * direct launches instead.  Every barrier wait ends after spin_limit polls of about a
  * microsecond each (LLG_BATCH_SPIN_DEFAULT, the value LLG_BATCH_PERSISTENT sets, is 2^24; 0
  * abandons the launch at the first barrier that is not complete on arrival, which tests the
  * failure path): a wait that runs out abandons the launch, llg_batch_step returns

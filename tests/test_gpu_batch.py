@@ -159,6 +159,34 @@ def test_batch_persistent_dipolar_segments(gpu):
             assert np.array_equal(a, b)
 
 
+def test_batch_persistent_graph_capture(gpu):
+    """Persistent chunks captured into hipGraphs (LLG_BATCH_GRAPH with persistent mode: barrier
+    resets, cooperative launches and the dipolar refreshes between them; the residency probe
+    runs before the capture) replay the per-half-step launches' bits: members of four reader
+    groups and a dipolar film refreshing every 5 steps, three chunks of 20 steps and one of 7
+    (a second graph).  If the runtime refuses to capture the cooperative launch, the batch
+    launches it directly, with the same bits."""
+    res = []
+    for mode in ("eager", "graph_persist"):
+        B = SB.Batch(flags=SB.BATCH_GRAPH if mode == "graph_persist" else 0)
+        if mode == "graph_persist":
+            B.set_persistent(True)
+        ms = _persist_members()
+        f, f0 = W.sc_film_dip(Lx=8, Ly=8, Lz=2, precision="fp32", refresh=5)
+        ms.append((f, f0, 75.0, (0.0, 0.0, 0.0)))
+        ids = [B.add(s, s0, T, Bf) for s, s0, T, Bf in ms]
+        for n in (20, 20, 20, 7):
+            B.step(n)
+        info = B.info()
+        if mode == "graph_persist":
+            print("persistent graphs built:", info["graphs_built"], "grid", info["persist_grid"])
+            assert info["persist_grid"] > 0
+        res.append([B.get_spins(k) for k in ids])
+        B.close()
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
+
+
 def test_batch_persistent_barrier_wait_is_bounded(gpu):
     """The grid barrier's failure path, forced from the host: with a poll limit of 0 the first
     block at a barrier that is not yet complete abandons the launch and every other block
