@@ -583,7 +583,8 @@ def batched(args):
     """64 replicas of the 128x128x16 film (NN exchange + FFT dipolar on the 256x256x32 padded
     grid) at T_r = 900 r/63 K, Heun FP32, one batched launch per half step, chunks of 100 steps
     replayed from hipGraphs that hold the dipolar refreshes (every `update` = 100 steps); then
-    the same steps as persistent chunks (one launch per 100 steps)."""
+    the same steps as persistent chunks (one cooperative launch per 100 steps, also replayed
+    from hipGraphs, with the refreshes between the launches)."""
     import sllg
     from sllg import batch as SB
     from sllg import workloads as W
@@ -611,7 +612,8 @@ def batched(args):
     sec = time.perf_counter() - t
     info = B.info()
     # the same number of steps again as persistent chunks (one cooperative launch per chunk of
-    # 100 steps, grid barriers between the half steps, refreshes between the chunks)
+    # 100 steps, grid barriers between the half steps, refreshes between the chunks; the batch
+    # replays graphs, so each chunk is a captured hipGraph)
     B.set_persistent(True)
     B.step(100)
     B.reduce(ids)
