@@ -529,6 +529,8 @@ int llg_batch_set_launch(llg_batch* b, uint32_t tpb, int xcd, int line_mode, int
 
 int llg_batch_set_persistent(llg_batch* b, int on, uint32_t spin_limit) {
     if (!b) SLLG_FAIL(LLG_E_INVALID, "NULL batch");
+    // a captured chunk holds the poll limit among its launch arguments
+    if ((on ? 1 : 0) != b->persist || spin_limit != b->spin_limit) drop_graph(b);
     b->persist = on ? 1 : 0;
     b->spin_limit = spin_limit;
     return 0;

@@ -203,6 +203,18 @@ def test_batch_persistent_barrier_wait_is_bounded(gpu):
     B.set_persistent(True)
     B.step(10)
     assert B.get_step(k) == 10
+    # the same through a captured chunk: a new poll limit takes effect on the next chunk
+    G = SB.Batch(flags=SB.BATCH_GRAPH)
+    kg = G.add(s, s0, 300.0)
+    G.set_persistent(True)
+    G.step(10)
+    G.set_persistent(True, spin_limit=0)
+    with pytest.raises(sllg.LLGError) as e:
+        G.step(10)
+    assert e.value.code == -3
+    G.set_persistent(True)
+    G.step(10)
+    assert G.get_step(kg) == 20
     sp = B.get_spins(k)
     assert np.isfinite(sp).all()
     assert float(np.abs(np.linalg.norm(sp, axis=0) - 1.0).max()) <= 2e-6
