@@ -208,7 +208,7 @@ def make_requests(specs):
 
 
 def serve(requests, device=0, budget_bytes=0, max_members=0, chunk=0, sequential=False, precision=1, graph=False,
-          want_series=False, progress=None, progress_every=0, max_live_spins=0):
+          want_series=False, progress=None, progress_every=0, max_live_spins=0, persistent=False):
     """Run llg_serve_run; returns (results list of dicts, stats dict, series list or None).
 
     With want_series, series[i] is request i's <M> samples (ceil(steps/update) x 3) and, for
@@ -218,7 +218,8 @@ def serve(requests, device=0, budget_bytes=0, max_members=0, chunk=0, sequential
     L = _bind()
     n = len(requests)
     hook = ProgressFn(lambda done, total, wall, user: progress(done, total, wall)) if progress else ProgressFn()
-    cfg = ServeConfig(device, BATCH_GRAPH if graph else 0, int(budget_bytes), max_members, chunk,
+    flags = (BATCH_GRAPH if graph else 0) | (BATCH_PERSISTENT if persistent else 0)
+    cfg = ServeConfig(device, flags, int(budget_bytes), max_members, chunk,
                       1 if sequential else 0, precision, hook, None, progress_every if progress else 0,
                       int(max_live_spins))
     res = (Result * n)()

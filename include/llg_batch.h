@@ -155,7 +155,7 @@ typedef struct llg_result {
 
 typedef struct llg_serve_config {
     int device;
-    uint32_t flags;           /* LLG_BATCH_GRAPH */
+    uint32_t flags;           /* LLG_BATCH_GRAPH, LLG_BATCH_PERSISTENT */
     uint64_t hbm_budget_bytes;
     uint32_t max_members;     /* 0: unlimited */
     uint32_t chunk;           /* steps per batched launch chunk; 0: the smallest update interval */

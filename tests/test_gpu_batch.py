@@ -288,20 +288,23 @@ def test_serve_results_same_alone_and_in_mix(gpu):
 
 
 def test_serve_graph_chunks_equal_eager(gpu):
-    """The request loop with hipGraph chunks (captured again whenever members join or leave)
-    gives the eager loop's results bit for bit: every sampled series and final moment."""
+    """The request loop with hipGraph chunks (captured again whenever members join or leave),
+    and with persistent chunks (direct and graph-captured), gives the eager loop's results bit
+    for bit: every sampled series and final moment."""
     specs = [dict(L=8, steps=500, update=100, T=200.0, seed=15, arrival_s=0.0),
              dict(L=10, steps=300, update=100, T=50.0, seed=16, arrival_s=0.0, type=SB.REQ_RAMP,
                   B0=(0, 0, -1.0), B1=(0, 0, 1.0)),
              dict(L=6, steps=400, update=100, T=400.0, seed=17, arrival_s=0.005),
              dict(L=8, steps=200, update=100, T=100.0, seed=18, arrival_s=0.01, type=SB.REQ_SW)]
     reqs = SB.make_requests(specs)
-    out = [SB.serve(reqs, want_series=True, graph=g) for g in (False, True)]
-    (r0, s0, ser0), (r1, s1, ser1) = out
-    assert s0["completed"] == s1["completed"] == len(specs)
-    for k in range(len(specs)):
-        assert np.array_equal(ser0[k], ser1[k]), k
-        assert r0[k]["m_final"] == r1[k]["m_final"]
+    out = [SB.serve(reqs, want_series=True, graph=g, persistent=p)
+           for g, p in ((False, False), (True, False), (False, True), (True, True))]
+    r0, s0, ser0 = out[0]
+    for r1, s1, ser1 in out[1:]:
+        assert s0["completed"] == s1["completed"] == len(specs)
+        for k in range(len(specs)):
+            assert np.array_equal(ser0[k], ser1[k]), k
+            assert r0[k]["m_final"] == r1[k]["m_final"]
 
 
 def test_batch_autotune_restores_and_shapes_agree(gpu):
