@@ -288,12 +288,19 @@ struct YZArgs {
 
 // The forward pass splits a plane's kz range over `parts` blocks (kzp sequences each, none
 // empty; the plane's spins are read by each part) for parallelism: the plane count alone is
-// small.
+// small. The parts of a plane write interleaved pieces of the same 128-byte lines, so they
+// are kept on one XCD (one L2, where the pieces merge before write-back): workgroups are
+// dealt to the 8 XCDs round-robin by id, so within each full group of 8 * parts ids, id b
+// runs on XCD b % 8 and takes part (b / 8) % parts of plane (b / (8 parts)) * 8 + b % 8;
+// ids past the last full group map plainly. The map is a bijection onto (plane, part).
 template <int GY>
 __global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restrict__ S, uint32_t parts, uint32_t kzp) {
     constexpr int LG = ilog2c(GY);
     extern __shared__ float2 yz_lds[];
-    const uint32_t kz = a.kz, plane = blockIdx.x / parts, part = blockIdx.x % parts;
+    const uint32_t b = blockIdx.x, full = gridDim.x - gridDim.x % (8u * parts);
+    const uint32_t plane = b < full ? (b / 8u / parts) * 8u + b % 8u : b / parts;
+    const uint32_t part = b < full ? (b / 8u) % parts : b % parts;
+    const uint32_t kz = a.kz;
     const uint32_t sc = plane / a.lx, x = plane % a.lx, sp = sc / 3, c = sc % 3;
     const uint32_t k0 = part * kzp, nk = (k0 + kzp <= kz ? kzp : kz - k0);  // this block's kz range
     float2* const buf = yz_lds;                                        // kzp (GY + GY/32)
