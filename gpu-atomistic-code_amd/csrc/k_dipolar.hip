@@ -131,7 +131,10 @@ __global__ __launch_bounds__(kTPB) void k_dip_mult(size_t Ph, const R* __restric
 
 constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v / 2); }
 
-constexpr int kXT = 512;  // threads of k_dip_xfft
+constexpr int kXT = 512;   // threads of k_dip_xfft
+constexpr int kYF = 512;   // threads of the forward (y, z) plane kernel
+constexpr int kYI = 1024;  // threads of the inverse one (one block per plane: 1024 measured 13.4 us against
+                           // 15.1 at 512 on the 256x256x32 grid; the forward's 768 blocks 16.5 against 15.8)
 
 // (y, kz) columns per block of the x kernels: 3 nms COLS sequences of Gx, about 12.5 KB of
 // LDS, so blocks outnumber the resident slots and one block's loads overlap another's LDS
@@ -187,9 +190,9 @@ __device__ __forceinline__ void xf_load(float2* buf, float2* w, const float2* __
 }
 
 // nseq sequences of GX elements (a compile-time constant at the x kernels' call sites)
-template <int GX>
+template <int GX, int NT = kXT>
 __device__ __forceinline__ void xf_single_stage(float2* buf, uint32_t nseq) {  // length 2: w[0] = 1
-    for (uint32_t q = threadIdx.x; q < nseq * (GX / 2); q += kXT) {
+    for (uint32_t q = threadIdx.x; q < nseq * (GX / 2); q += NT) {
         const uint32_t sq = q / (GX / 2), i0 = xf_at<GX>(sq, 2 * (q % (GX / 2))), i1 = i0 + 1;
         const float2 u = buf[i0], v = buf[i1];
         buf[i0] = cadd(u, v);
@@ -201,14 +204,14 @@ __device__ __forceinline__ void xf_single_stage(float2* buf, uint32_t nseq) {  /
 // forward DIT: an odd stage count starts with one length-2 stage; then stages L and 2L run
 // as one pass, a thread taking the quartet pos, pos + L/2, pos + L, pos + 3L/2 of a group
 // of 2L (both butterflies of both stages in registers)
-template <int GX>
+template <int GX, int NT = kXT>
 __device__ __forceinline__ void xf_forward(float2* buf, const float2* w, uint32_t nseq) {
     constexpr int LG = ilog2c(GX);
-    if constexpr (LG & 1) xf_single_stage<GX>(buf, nseq);
+    if constexpr (LG & 1) xf_single_stage<GX, NT>(buf, nseq);
 #pragma unroll
     for (int L = (LG & 1) ? 4 : 2; 2 * L <= GX; L <<= 2) {
         const int qh = L >> 1;
-        for (uint32_t q = threadIdx.x; q < nseq * (GX / 4); q += kXT) {
+        for (uint32_t q = threadIdx.x; q < nseq * (GX / 4); q += NT) {
             const uint32_t sq = q / (GX / 4), k = q % (GX / 4), pos = k % qh;
             const uint32_t b = (k / qh) * 2 * L + pos;
             const uint32_t i0 = xf_at<GX>(sq, b), i1 = xf_at<GX>(sq, b + qh), i2 = xf_at<GX>(sq, b + L),
@@ -232,13 +235,13 @@ __device__ __forceinline__ void xf_forward(float2* buf, const float2* w, uint32_
 
 // inverse DIF (twiddles conjugated): stages 2L and L as one pass from the top; an odd stage
 // count ends with one length-2 stage
-template <int GX>
+template <int GX, int NT = kXT>
 __device__ __forceinline__ void xf_inverse(float2* buf, const float2* w, uint32_t nseq) {
     constexpr int LG = ilog2c(GX);
 #pragma unroll
     for (int L = GX / 2; L >= ((LG & 1) ? 4 : 2); L >>= 2) {
         const int qh = L >> 1;
-        for (uint32_t q = threadIdx.x; q < nseq * (GX / 4); q += kXT) {
+        for (uint32_t q = threadIdx.x; q < nseq * (GX / 4); q += NT) {
             const uint32_t sq = q / (GX / 4), k = q % (GX / 4), pos = k % qh;
             const uint32_t b = (k / qh) * 2 * L + pos;
             const uint32_t i0 = xf_at<GX>(sq, b), i1 = xf_at<GX>(sq, b + qh), i2 = xf_at<GX>(sq, b + L),
@@ -254,7 +257,7 @@ __device__ __forceinline__ void xf_inverse(float2* buf, const float2* w, uint32_
         }
         __syncthreads();
     }
-    if constexpr (LG & 1) xf_single_stage<GX>(buf, nseq);
+    if constexpr (LG & 1) xf_single_stage<GX, NT>(buf, nseq);
 }
 
 template <int GX, int NS, int COLS>
@@ -294,7 +297,7 @@ struct YZArgs {
 // runs on XCD b % 8 and takes part (b / 8) % parts of plane (b / (8 parts)) * 8 + b % 8;
 // ids past the last full group map plainly. The map is a bijection onto (plane, part).
 template <int GY>
-__global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restrict__ S, uint32_t parts, uint32_t kzp) {
+__global__ __launch_bounds__(kYF) void k_yz_fwd(YZArgs a, const float* __restrict__ S, uint32_t parts, uint32_t kzp) {
     constexpr int LG = ilog2c(GY);
     extern __shared__ float2 yz_lds[];
     const uint32_t b = blockIdx.x, full = gridDim.x - gridDim.x % (8u * parts);
@@ -307,16 +310,16 @@ __global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restric
     float2* const wy = buf + (size_t)kzp * (GY + GY / 32);              // GY / 2
     float2* const wz = wy + GY / 2;                                     // gz
     float* const R = reinterpret_cast<float*>(wz + a.gz);               // ly lz
-    for (uint32_t t = threadIdx.x; t < (uint32_t)(GY / 2); t += kXT) wy[t] = a.twy[t];
-    for (uint32_t t = threadIdx.x; t < a.gz; t += kXT) wz[t] = a.twz[t];
+    for (uint32_t t = threadIdx.x; t < (uint32_t)(GY / 2); t += kYF) wy[t] = a.twy[t];
+    for (uint32_t t = threadIdx.x; t < a.gz; t += kYF) wz[t] = a.twz[t];
     const uint32_t* const mp = a.map + ((size_t)sp * a.lx + x) * a.ly * a.lz;
     const uint32_t i0 = x * a.ly * a.lz;  // first spin of the plane when the map is the identity
-    for (uint32_t e = threadIdx.x; e < a.ly * a.lz; e += kXT) {
+    for (uint32_t e = threadIdx.x; e < a.ly * a.lz; e += kYF) {
         const uint32_t i = a.dense ? i0 + e : mp[e];
         R[e] = i == 0xFFFFFFFFu ? 0.f : (a.mu ? a.mu[i] : 1.f) * S[(size_t)c * a.n + i];
     }
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * nk; e += kXT) {
+    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * nk; e += kYF) {
         const uint32_t y = e / nk, kk = e % nk, k = k0 + kk;
         float2 X = make_float2(0.f, 0.f);
         if (y < a.ly) {
@@ -333,29 +336,29 @@ __global__ __launch_bounds__(kXT) void k_yz_fwd(YZArgs a, const float* __restric
         buf[xf_at<GY>(kk, __brev(y) >> (32 - LG))] = X;
     }
     __syncthreads();
-    xf_forward<GY>(buf, wy, nk);
+    xf_forward<GY, kYF>(buf, wy, nk);
     float2* const out = a.spec + ((size_t)sc * a.lx + x) * GY * kz + k0;
-    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * nk; e += kXT)
+    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * nk; e += kYF)
         out[(size_t)(e / nk) * kz + e % nk] = buf[xf_at<GY>(e % nk, e / nk)];
 }
 
 template <int GY>
-__global__ __launch_bounds__(kXT) void k_yz_inv(YZArgs a, float invP, float* __restrict__ out) {
+__global__ __launch_bounds__(kYI) void k_yz_inv(YZArgs a, float invP, float* __restrict__ out) {
     constexpr int LG = ilog2c(GY);
     extern __shared__ float2 yz_lds[];
     const uint32_t kz = a.kz, sc = blockIdx.x / a.lx, x = blockIdx.x % a.lx, sp = sc / 3, c = sc % 3;
     float2* const buf = yz_lds;
     float2* const wy = buf + (size_t)kz * (GY + GY / 32);
     float2* const wz = wy + GY / 2;
-    for (uint32_t t = threadIdx.x; t < (uint32_t)(GY / 2); t += kXT) wy[t] = a.twy[t];
-    for (uint32_t t = threadIdx.x; t < a.gz; t += kXT) wz[t] = a.twz[t];
+    for (uint32_t t = threadIdx.x; t < (uint32_t)(GY / 2); t += kYI) wy[t] = a.twy[t];
+    for (uint32_t t = threadIdx.x; t < a.gz; t += kYI) wz[t] = a.twz[t];
     const float2* const in = a.spec + ((size_t)sc * a.lx + x) * GY * kz;
-    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * kz; e += kXT) buf[xf_at<GY>(e % kz, e / kz)] = in[e];
+    for (uint32_t e = threadIdx.x; e < (uint32_t)GY * kz; e += kYI) buf[xf_at<GY>(e % kz, e / kz)] = in[e];
     __syncthreads();
-    xf_inverse<GY>(buf, wy, kz);
+    xf_inverse<GY, kYI>(buf, wy, kz);
     const uint32_t* const mp = a.map + ((size_t)sp * a.lx + x) * a.ly * a.lz;
     const uint32_t nyq = a.gz / 2, i0 = x * a.ly * a.lz;
-    for (uint32_t e = threadIdx.x; e < a.ly * a.lz; e += kXT) {
+    for (uint32_t e = threadIdx.x; e < a.ly * a.lz; e += kYI) {
         const uint32_t i = a.dense ? i0 + e : mp[e];
         if (i == 0xFFFFFFFFu) continue;
         const uint32_t y = e / a.lz, z = e % a.lz, yb = __brev(y) >> (32 - LG);
@@ -386,8 +389,8 @@ static int launch_yz(bool fwd, int gy, uint32_t planes, const YZArgs& a, const f
     const size_t lds = yz_lds_bytes(gy, a.gz, a.ly, a.lz, kzp);
 #define SLLG_YZ(G_)                                                                   \
     case G_:                                                                          \
-        if (fwd) k_yz_fwd<G_><<<planes * parts, kXT, lds, s>>>(a, S, parts, kzp);    \
-        else k_yz_inv<G_><<<planes, kXT, lds, s>>>(a, invP, out);                     \
+        if (fwd) k_yz_fwd<G_><<<planes * parts, kYF, lds, s>>>(a, S, parts, kzp);    \
+        else k_yz_inv<G_><<<planes, kYI, lds, s>>>(a, invP, out);                     \
         break;
     switch (gy) {
         SLLG_YZ(16) SLLG_YZ(32) SLLG_YZ(64) SLLG_YZ(128) SLLG_YZ(256) SLLG_YZ(512) SLLG_YZ(1024)
